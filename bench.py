@@ -1,0 +1,236 @@
+"""Benchmark: domain-wall propagator solve on one MI355X.
+
+Workload (the metric's configuration): 24^3 x 64 lattice, links
+U = exp(i 0.3 H) (seeded, antiperiodic in time), Moebius Ls = 12, M5 = 1.8,
+b = 1.5, c = 0.5, mass = 0.01; a 12-column point-source propagator solved
+to a 1e-8 FP64 true residual of the preconditioned odd system.  One step is
+one full propagator solve with the mixed-precision solver (FP32 CG with
+FP64 reliable updates); the FP64 solver is timed once on the same inputs.
+Inputs (gauge field, source propagator) are resident in device memory
+before the timed region; solutions stay in device memory.
+
+Prints one JSON line (value = mixed-precision time-to-solution in seconds).
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CG time-to-1e-8 & dslash HBM GB/s, 24^3x64 Ls=12: mixed multi-RHS vs FP64"
+PEAK_HBM_GBS = 8000.0
+
+CONFIGS = {
+    "B": dict(site=(24, 24, 24, 64), ls=12, sigma=0.3, mass=0.01, m5=1.8, scale=2.0, seed=2024),
+    # small configuration for quick local runs (not a bench line)
+    "S": dict(site=(8, 8, 8, 16), ls=12, sigma=0.3, mass=0.01, m5=1.8, scale=2.0, seed=2024),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def hop_bytes(site, ls, real_bytes, link_reals=18):
+    """Algorithmic bytes of one hop application (one output parity):
+    read the input half spinor once, write the output once, read every link
+    once: N*24*w*2 + V*4*r*w with N = V/2*Ls."""
+    V = 1
+    for n in site:
+        V *= n
+    N = V // 2 * ls
+    return N * 24 * real_bytes * 2 + V * 4 * link_reals * real_bytes
+
+
+def hop_flops(site, ls):
+    V = 1
+    for n in site:
+        V *= n
+    return 1320 * (V // 2) * ls
+
+
+def cpu_baseline(cfg, U, iters_fp64, sample_hermops):
+    """The oracle (C++ restatement of the reference CPU path, FP64, OpenMP)
+    on this host: time `sample_hermops` applications of Mpc^+ Mpc at the
+    full lattice, then extrapolate the 12-column time-to-solution with the
+    FP64 GPU iteration count (extrapolated, per BASELINE plan)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    site, ls = cfg["site"], cfg["ls"]
+    orc = O.OracleInverter(site, U, mass=cfg["mass"], ls=ls, m5=cfg["m5"], mobius_scale=cfg["scale"])
+    V = int(np.prod(site))
+    x = np.ones((V // 2, ls, 12), dtype=np.complex128)
+    t0 = time.perf_counter()
+    for _ in range(sample_hermops):
+        x = orc.apply_half("hermop_sym2", x, 1)
+    dt = (time.perf_counter() - t0) / sample_hermops
+    threads = O.num_threads()
+    return dict(
+        value=dt * iters_fp64,
+        unit="s (12-column time-to-solution, extrapolated: FP64 GPU iteration count x oracle hermop time)",
+        cores=threads,
+        kind="port",
+        sample=f"{sample_hermops} applications of Mpc^+Mpc (multiply_hermop_sym2) at 24^3x64 Ls=12, "
+               f"{dt:.3f} s each on {threads} threads ({platform.processor() or platform.machine()})",
+        hermop_seconds=dt,
+    )
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--config", default="B", choices=sorted(CONFIGS))
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--cpu-hermops", type=int, default=3, help="oracle hermop applications (0 = skip)")
+    ap.add_argument("--kernel-iters", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the solve would shard by propagator column or by 4D domain).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import qlattice_amd as q
+    from qlattice_amd import _lib
+    from qlattice_amd.gauge import rand_gauge_links, twist_boundary_at_boundary
+
+    cfg = CONFIGS[args.config]
+    site, ls = cfg["site"], cfg["ls"]
+    V = int(np.prod(site))
+    log(f"config {args.config}: site {site} ls {ls}; generating gauge field")
+    U = rand_gauge_links(site, cfg["sigma"], cfg["seed"])
+    twist_boundary_at_boundary(U, site, -0.5, 3)
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    fa = q.FermionAction(mass=cfg["mass"], ls=ls, m5=cfg["m5"], mobius_scale=cfg["scale"])
+    inv = q.InverterDomainWall(gf=gf, fa=fa, solver="mixed")
+    inv.set_stop_rsd(1e-8)
+    L = _lib.load()
+
+    # kernel timing (HIP events on the inverter's stream)
+    log("timing hop kernels")
+    hop32_ms = q.bench_kernel(inv, 0, 32, args.kernel_iters)
+    hop64_ms = q.bench_kernel(inv, 0, 64, args.kernel_iters)
+    m5_32_ms = q.bench_kernel(inv, 2, 32, args.kernel_iters)
+    herm32_ms = q.bench_kernel(inv, 3, 32, max(1, args.kernel_iters // 5))
+    herm64_ms = q.bench_kernel(inv, 3, 64, max(1, args.kernel_iters // 5))
+    b32 = hop_bytes(site, ls, 4)
+    b64 = hop_bytes(site, ls, 8)
+    gbs32 = b32 / (hop32_ms * 1e-3) / 1e9
+    gbs64 = b64 / (hop64_ms * 1e-3) / 1e9
+    log(f"hop fp32 {hop32_ms:.3f} ms {gbs32:.0f} GB/s | hop fp64 {hop64_ms:.3f} ms {gbs64:.0f} GB/s | "
+        f"m5 fp32 {m5_32_ms:.3f} ms | hermop fp32 {herm32_ms:.3f} ms fp64 {herm64_ms:.3f} ms")
+
+    # point-source propagator, resident on the device
+    ncol = 12
+    src_h = np.zeros((V, 12, 12), dtype=np.complex128)
+    src_h[0] = np.eye(12)
+    src = torch.from_numpy(src_h).cuda()
+    sol = torch.zeros_like(src)
+    it = ctypes.c_long()
+
+    def solve(solver):
+        inv.set_solver(solver)
+        torch.cuda.synchronize()
+        _lib.check(L.qlat_dwf_invert_prop_device(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src),
+                                                 ctypes.byref(it)), "qlat_dwf_invert_prop_device")
+        torch.cuda.synchronize()
+        return inv.stats()
+
+    for w in range(args.warmup):
+        log(f"warmup {w}")
+        solve("mixed")
+    log(f"timed mixed solves: {args.steps}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        st_mixed = solve("mixed")
+        log(f"step {s}: iters {st_mixed['total_iter']} reliable {st_mixed['reliable_updates']} "
+            f"true_res {st_mixed['true_residual']:.2e} solve {st_mixed['seconds']:.2f} s")
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    sol_mixed = sol.clone()
+
+    t_fp64 = None
+    st64 = None
+    if not args.skip_fp64:
+        log("FP64 solve")
+        t0 = time.perf_counter()
+        st64 = solve("fp64")
+        torch.cuda.synchronize()
+        t_fp64 = time.perf_counter() - t0
+        log(f"fp64: iters {st64['total_iter']} cycles {st64['cycles']} true_res {st64['true_residual']:.2e} "
+            f"{t_fp64:.2f} s")
+        diff = (torch.linalg.vector_norm(sol_mixed - sol) / torch.linalg.vector_norm(sol)).item()
+    else:
+        diff = None
+
+    cpu = None
+    if args.cpu_hermops > 0:
+        iters_ref = st64["total_iter"] if st64 else st_mixed["total_iter"]
+        log("cpu baseline (oracle)")
+        cpu = cpu_baseline(cfg, U, iters_ref, args.cpu_hermops)
+
+    N = V // 2 * ls
+    line = {
+        "metric": METRIC,
+        "value": t_mixed,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (inner CG) + fp64 (reliable updates, solution)",
+        "data": "synthetic: seeded exp(i*0.3*H) SU(3) links, antiperiodic t, point sources",
+        "config": {
+            "workload": f"12-column point-source propagator, {site[0]}^3x{site[3]}, Ls={ls}, Moebius b=1.5 c=0.5, "
+                        f"M5=1.8, mass=0.01, stop 1e-8 (FP64 true residual), single-RHS mixed solver",
+            "lattice": list(site), "ls": ls, "columns": ncol, "stop_rsd": 1e-8,
+        },
+        "mixed": {"seconds": t_mixed, "iters": st_mixed["total_iter"],
+                  "reliable_updates": st_mixed["reliable_updates"],
+                  "true_residual": st_mixed["true_residual"]},
+        "fp64": None if st64 is None else {"seconds": t_fp64, "iters": st64["total_iter"],
+                                           "cycles": st64["cycles"], "true_residual": st64["true_residual"]},
+        "speedup_vs_fp64": None if t_fp64 is None else t_fp64 / t_mixed,
+        "solution_rel_diff_mixed_vs_fp64": diff,
+        "dslash": {
+            "hop_fp32_ms": hop32_ms, "hop_fp32_gbps": gbs32, "hop_fp32_gflops": hop_flops(site, ls) / hop32_ms / 1e6,
+            "hop_fp64_ms": hop64_ms, "hop_fp64_gbps": gbs64,
+            "m5_fp32_ms": m5_32_ms, "hermop_fp32_ms": herm32_ms, "hermop_fp64_ms": herm64_ms,
+            "hermop_ref_flops_convention": 5500 * ls * V,
+        },
+        "roofline": {
+            "kernel": "k_hop<float> (Wilson hop, FP32, 18-real links)",
+            "bound": "hbm",
+            "achieved": gbs32,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": gbs32 / PEAK_HBM_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": b32,
+            "units_per_launch": N,
+        },
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

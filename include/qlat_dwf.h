@@ -1,0 +1,200 @@
+/*
+ * qlat_dwf.h -- C ABI of the MI355X (gfx950) domain-wall fermion library.
+ *
+ * Drop-in replacement for the even-odd Moebius domain-wall operator and its
+ * CG inverter of Qlattice (qlat/qlat/lib/dslash.cpp,
+ * qlat/qlat/include/qlat/dslash.h).  All entry points take plain pointers and
+ * sizes; no C++ or framework types cross this boundary.
+ *
+ * Field layouts at the boundary are the reference's (complex128 stored as
+ * interleaved re,im doubles, array of structures):
+ *   gauge field  [V][4][3][3]      GaugeField, core.h:1187
+ *   5D full      [V][Ls][12]       FermionField5d, core.h:1203; 12 = spin*3+colour
+ *   5D half      [V/2][Ls][12]     eo half field, half index = full index / 2
+ *                                  (core.h:232-261); parity eo: 2 even, 1 odd
+ *   4D fermion   [V][12]           FermionField4d
+ *   propagator   [V][12][12]       Propagator4d, [site][row][source column]
+ * with site index x + X*(y + Y*(z + Z*t)) (coordinate.h:169-188).
+ *
+ * Functions without a `_device` suffix take HOST pointers: the library
+ * uploads, converts to its device layout, computes on the GPU and copies the
+ * result back before returning.  `_device` functions take pointers to device
+ * memory holding the same layouts and leave the result in device memory.
+ * Every call is ordered on the inverter's own HIP stream and synchronises it
+ * before returning.
+ *
+ * Errors: every function returns QLAT_DWF_OK (0) or a negative status and
+ * records a message retrievable with qlat_dwf_last_error() (thread local).
+ * This mirrors the reference, whose Qassert throws std::string
+ * (qassert.h:77-93) and whose Python layer turns it into RuntimeError
+ * (cqlat/lib.h:28-35).  Non-convergence is not an error: as in cg_with_f
+ * (dslash.h:394-398) the iteration count is then max_num_iter + 1.
+ */
+#ifndef QLAT_DWF_H
+#define QLAT_DWF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QLAT_DWF_OK 0
+#define QLAT_DWF_EINVAL (-1)
+#define QLAT_DWF_EHIP (-2)
+#define QLAT_DWF_ENOMEM (-3)
+#define QLAT_DWF_ESTATE (-4)
+
+/* Solver of the odd-site normal equations (qlat_dwf_inverter_set_solver). */
+#define QLAT_DWF_SOLVER_FP64 0       /* the reference algorithm, FP64 throughout */
+#define QLAT_DWF_SOLVER_MIXED_FP32 1 /* FP32 CG with FP64 reliable updates      */
+
+typedef struct qlat_dwf_inverter qlat_dwf_inverter;
+
+/* FermionAction (fermion-action.h:8-53).  bs/cs point to `ls` complex
+ * numbers (re,im pairs); when NULL they are derived from mobius_scale as
+ * b = 0.5*scale + 0.5, c = 0.5*scale - 0.5 (fermion-action.h:50-51). */
+typedef struct {
+  double mass;
+  int ls;
+  double m5;
+  double mobius_scale;
+  int is_multiplying_dminus; /* default 1 */
+  int is_using_zmobius;
+  int cg_diagonal_mee;       /* only 2 (sym2 preconditioning) is supported, as in dslash.h:483-521 */
+  const double* bs;
+  const double* cs;
+} qlat_dwf_fermion_action;
+
+/* Statistics of the last solve (accumulated over columns for propagators). */
+typedef struct {
+  long total_iter;        /* inner CG iterations                                 */
+  long cycles;            /* outer restart cycles (FP64 reference algorithm)     */
+  long reliable_updates;  /* FP64 residual replacements (mixed solver)           */
+  long hermop_fp64;       /* FP64 applications of Mpc^+ Mpc                      */
+  long hermop_low;        /* FP32 applications of Mpc^+ Mpc                      */
+  double true_residual;   /* max over columns of |b' - A x| / |b'| in FP64 (odd) */
+  double seconds;         /* wall clock of the solve call                        */
+} qlat_dwf_solve_stats;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* qlat_dwf_last_error(void);
+
+/* ABI version of this header (incremented on incompatible changes). */
+int qlat_dwf_abi_version(void);
+
+/* ---------------------------------------------------------------- setup */
+
+/* InverterDomainWall::setup(gf, fa) (dslash.h:154-162) and setup_inverter
+ * (dslash.h:199-216): the inverter keeps its own device copy of the gauge
+ * field.  total_site: lattice extents X Y Z T (all even). */
+int qlat_dwf_inverter_create(qlat_dwf_inverter** inv, const int total_site[4], const double* gauge,
+                             const qlat_dwf_fermion_action* fa);
+
+/* ~InverterDomainWall; Python InverterDomainWall.__del__ (inverter.py:92-94). */
+int qlat_dwf_inverter_destroy(qlat_dwf_inverter* inv);
+
+/* InverterDomainWall::stop_rsd(), max_num_iter(), max_mixed_precision_cycle()
+ * (dslash.h:186-196); defaults 1e-8, 200, 300 (InverterParams, dslash.h:114-133). */
+int qlat_dwf_inverter_set_params(qlat_dwf_inverter* inv, double stop_rsd, long max_num_iter,
+                                 long max_mixed_precision_cycle);
+int qlat_dwf_inverter_get_params(const qlat_dwf_inverter* inv, double* stop_rsd, long* max_num_iter,
+                                 long* max_mixed_precision_cycle);
+
+/* Choose the inner solver, the role of a back-end's cg_with_herm_sym_2
+ * override (grid.h:460-484).  reliable_delta: residual reduction between
+ * FP64 residual replacements of the mixed solver (0 selects 0.1). */
+int qlat_dwf_inverter_set_solver(qlat_dwf_inverter* inv, int solver, double reliable_delta);
+
+int qlat_dwf_inverter_get_stats(const qlat_dwf_inverter* inv, qlat_dwf_solve_stats* stats);
+
+/* ------------------------------------------- operators on 5D half fields
+ * `out` and `in` are [V/2][Ls][12]; out may equal in ("out can be the same
+ * object as in", dslash.cpp:593). */
+
+/* multiply_m_e_e (dslash.h:244-245, :285-286; dslash.cpp:591-639) */
+int qlat_dwf_multiply_m_e_e(qlat_dwf_inverter* inv, double* out, const double* in);
+/* multiply_mdag_e_e (dslash.h:247-248, :288-289; dslash.cpp:641-692) */
+int qlat_dwf_multiply_mdag_e_e(qlat_dwf_inverter* inv, double* out, const double* in);
+/* multiply_m_e_e_inv (dslash.h:250-251, :291-292; dslash.cpp:694-769) */
+int qlat_dwf_multiply_m_e_e_inv(qlat_dwf_inverter* inv, double* out, const double* in);
+/* multiply_mdag_e_e_inv (dslash.h:253-254, :294-295; dslash.cpp:771-857) */
+int qlat_dwf_multiply_mdag_e_e_inv(qlat_dwf_inverter* inv, double* out, const double* in);
+/* multiply_wilson_d_e_o_no_comm (dslash.h:256-258; dslash.cpp:859-908):
+ * input parity eo_in, output parity 3 - eo_in. */
+int qlat_dwf_multiply_wilson_d_e_o(qlat_dwf_inverter* inv, double* out, const double* in, int eo_in);
+/* multiply_wilson_ddag_e_o_no_comm (dslash.h:260-262; dslash.cpp:910-959) */
+int qlat_dwf_multiply_wilson_ddag_e_o(qlat_dwf_inverter* inv, double* out, const double* in, int eo_in);
+/* multiply_m_e_o (dslash.h:264-265, :297-298; dslash.cpp:961-1003) */
+int qlat_dwf_multiply_m_e_o(qlat_dwf_inverter* inv, double* out, const double* in, int eo_in);
+/* multiply_mdag_e_o (dslash.h:267-268, :300-301; dslash.cpp:1005-1060) */
+int qlat_dwf_multiply_mdag_e_o(qlat_dwf_inverter* inv, double* out, const double* in, int eo_in);
+/* multiply_mpc_sym2 (dslash.h:276-277, :317-318; dslash.cpp:1102-1119); eo = parity of in and out */
+int qlat_dwf_multiply_mpc_sym2(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
+/* multiply_mpcdag_sym2 (dslash.h:279-280, :320-321; dslash.cpp:1121-1138) */
+int qlat_dwf_multiply_mpcdag_sym2(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
+/* multiply_hermop_sym2 (dslash.h:282-283, :323-324; dslash.cpp:1140-1149) */
+int qlat_dwf_multiply_hermop_sym2(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
+
+/* ------------------------------------------- operators on 5D full fields */
+
+/* multiply_m (dslash.h:270-271, :311-312; dslash.cpp:1062-1080) */
+int qlat_dwf_multiply_m(qlat_dwf_inverter* inv, double* out, const double* in);
+/* multiply_mdag (dslash.h:273-274, :314-315; dslash.cpp:1082-1100) */
+int qlat_dwf_multiply_mdag(qlat_dwf_inverter* inv, double* out, const double* in);
+/* multiply_d_minus (dslash.h:227-231; dslash.cpp:448-488) */
+int qlat_dwf_multiply_d_minus(qlat_dwf_inverter* inv, double* out, const double* in);
+
+/* ------------------------------------------------------------ BLAS-1 */
+
+/* dot_product (dslash.h:329; dslash.cpp:1298-1321): a^+ b over n_wv
+ * WilsonVectors (12 complex each). */
+int qlat_dwf_dot_product(qlat_dwf_inverter* inv, const double* a, const double* b, long n_wv, double* re,
+                         double* im);
+/* qnorm (field.h:170-195) */
+int qlat_dwf_qnorm(qlat_dwf_inverter* inv, const double* a, long n_wv, double* result);
+
+/* ------------------------------------------------------------ solvers */
+
+/* cg_with_herm_sym_2 (dslash.h:575-578; dslash.cpp:1323-1333): CG on
+ * Mpc^+ Mpc over odd half fields; `sol` holds the initial guess on entry.
+ * Always runs the FP64 reference algorithm. */
+int qlat_dwf_cg_with_herm_sym_2(qlat_dwf_inverter* inv, double* sol, const double* src, double stop_rsd,
+                                long max_num_iter, long* iter);
+
+/* invert(FermionField5d&, const FermionField5d&, const InverterDomainWall&)
+ * (dslash.h:580-581; dslash.cpp:1335-1340 -> invert_with_cg, dslash.h:440-530) */
+int qlat_dwf_invert_5d(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
+
+/* invert(FermionField4d&, ...) (dslash.h:583-584; dslash.cpp:1342-1347 ->
+ * invert_dwf, qcd-prop.h:146-167) */
+int qlat_dwf_invert_4d(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
+
+/* invert(Propagator4d&, const Propagator4d&, const Inverter&)
+ * (qcd-prop.h:169-186) as reached from Python InverterDomainWall.__mul__
+ * (inverter.py:96-109 -> cqlat/inverter.cpp:23-36). */
+int qlat_dwf_invert_prop(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter);
+
+/* find_max_eigen_value_hermop_sym2 (dslash.h:586-588; dslash.cpp:1349-1369)
+ * from a caller-supplied odd half-field start vector. */
+int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,
+                                              double* result);
+
+/* ------------------------------------------ device-resident variants
+ * Same semantics as above; all field pointers are device memory. */
+int qlat_dwf_multiply_hermop_sym2_device(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
+int qlat_dwf_invert_5d_device(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
+int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter);
+
+/* --------------------------------------------------------- measurement
+ * Average duration (ms) of one launch of a kernel, timed with HIP events
+ * on the inverter's stream over `iters` back-to-back launches after one
+ * warm-up launch.  which: 0 hop H, 1 hop H^+, 2 dense s-matrix (M_ee^-1),
+ * 3 Mpc^+ Mpc (all its kernels).  precision: 64 or 32.
+ * Has no counterpart in the reference (its Timer, timer.h:290, is host
+ * wall clock); it exists to report kernel bandwidth. */
+int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int iters, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QLAT_DWF_H */

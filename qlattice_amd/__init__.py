@@ -1,0 +1,53 @@
+"""qlattice_amd -- MI355X (gfx950) domain-wall fermion operator and inverter.
+
+A drop-in for Qlattice's even-odd Moebius domain-wall path
+(qlat/qlat/lib/dslash.cpp, qlat/qlat/include/qlat/dslash.h) with the same
+Python surface as ``qlat`` for it: ``FermionAction``, ``InverterDomainWall``,
+``GaugeField``, ``Prop``, ``mk_point_src``, and the ``multiply_*`` /
+``invert`` / ``cg_with_herm_sym_2`` functions.  All arithmetic runs in the
+hand-written HIP library ``libqlat_dwf_hip.so`` (C ABI: include/qlat_dwf.h).
+"""
+
+from .fermion_action import FermionAction
+from .fields import (
+    Coordinate,
+    CoordinateD,
+    FermionField4d,
+    FermionField5d,
+    GaugeField,
+    Geometry,
+    Prop,
+    fermion_field_4d_from_5d,
+    fermion_field_5d_from_4d,
+    get_half_fermion,
+    mk_point_src,
+    set_half_fermion,
+    set_point_src_fermion_field,
+)
+from .inverter import (
+    Inverter,
+    InverterDomainWall,
+    bench_kernel,
+    cg_with_herm_sym_2,
+    dot_product,
+    find_max_eigen_value_hermop_sym2,
+    invert,
+    multiply_d_minus,
+    multiply_hermop_sym2,
+    multiply_m,
+    multiply_m_e_e,
+    multiply_m_e_e_inv,
+    multiply_m_e_o,
+    multiply_mdag,
+    multiply_mdag_e_e,
+    multiply_mdag_e_e_inv,
+    multiply_mdag_e_o,
+    multiply_mpc_sym2,
+    multiply_mpcdag_sym2,
+    multiply_wilson_d_e_o_no_comm,
+    multiply_wilson_ddag_e_o_no_comm,
+    qnorm,
+)
+from ._lib import QlatDwfError, load as load_library
+
+__all__ = [n for n in dir() if not n.startswith("_")]

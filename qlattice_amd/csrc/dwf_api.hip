@@ -1,0 +1,1292 @@
+// Host side of the gfx950 domain-wall library: the C ABI of
+// include/qlat_dwf.h, the inverter state, the composition of the
+// even-odd operators from the kernels in dwf_kernels.hip, and the solvers.
+//
+// Operator factorisation used on the device (all algebra from
+// qlat/qlat/lib/dslash.cpp):
+//   M_ee  = diag(b~) - diag(c~) S          (dslash.cpp:591-639)
+//   M_eo  = H P,  P = diag(b) + diag(c) S  (dslash.cpp:961-1003)
+//   M_eo^+ = P^+ H^+                       (dslash.cpp:1005-1060)
+//   Mpc   = 1 - H K H K,        K   = P M_ee^-1      (dslash.cpp:1102-1119)
+//   Mpc^+ = 1 - K^+ H^+ K^+ H^+, K^+ = M_ee^-+ P^+   (dslash.cpp:1121-1138)
+// where H is the Wilson hop and S the fifth-dimension shift with the mass
+// corners.  Every s-operator (M_ee, its inverse, P, K, ...) is one dense
+// Ls x Ls complex matrix per chirality, built here in FP64.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qlat_dwf.h"
+#include "dwf_internal.h"
+
+using qdwf::cplx;
+using qdwf::LatGeo;
+typedef std::complex<double> cd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct MemError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define QDWF_HIP(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw HipError(std::string(#expr) + " failed: " + hipGetErrorString(e_) + " (" + \
+                     __FILE__ + ":" + std::to_string(__LINE__) + ")");                     \
+  } while (0)
+
+#define QDWF_REQUIRE(cond, msg)         \
+  do {                                  \
+    if (!(cond)) throw ArgError(msg);   \
+  } while (0)
+
+// ------------------------------------------------------------ s-matrices
+enum MatId { MAT_MEE, MAT_MEE_DAG, MAT_MEE_INV, MAT_MEE_INVDAG, MAT_P, MAT_P_DAG, MAT_K, MAT_K_DAG, MAT_NEGC, MAT_COUNT };
+
+typedef std::vector<cd> Mat;  // ls*ls row-major
+
+Mat mat_mul(const Mat& a, const Mat& b, int n)
+{
+  Mat c(n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j) c[i * n + j] += a[i * n + k] * b[k * n + j];
+  return c;
+}
+Mat mat_adj(const Mat& a, int n)
+{
+  Mat c(n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) c[i * n + j] = std::conj(a[j * n + i]);
+  return c;
+}
+Mat mat_inv(const Mat& a0, int n)
+{
+  // Gauss-Jordan elimination with partial pivoting.
+  Mat a = a0, inv(n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::abs(a[r * n + c]) > std::abs(a[piv * n + c])) piv = r;
+    if (std::abs(a[piv * n + c]) == 0.0) throw ArgError("singular fifth-dimension matrix (check mass/m5/bs/cs)");
+    if (piv != c)
+      for (int j = 0; j < n; ++j) {
+        std::swap(a[c * n + j], a[piv * n + j]);
+        std::swap(inv[c * n + j], inv[piv * n + j]);
+      }
+    const cd d = 1.0 / a[c * n + c];
+    for (int j = 0; j < n; ++j) {
+      a[c * n + j] *= d;
+      inv[c * n + j] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const cd f = a[r * n + c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < n; ++j) {
+        a[r * n + j] -= f * a[c * n + j];
+        inv[r * n + j] -= f * inv[c * n + j];
+      }
+    }
+  }
+  return inv;
+}
+
+// Fifth-dimension shift S per chirality (dslash.cpp:627-634):
+// chirality 0 (spins 0,1, P+): (S psi)_s = psi_{s-1}, psi_{-1} = -m psi_{Ls-1}
+// chirality 1 (spins 2,3, P-): (S psi)_s = psi_{s+1}, psi_{Ls} = -m psi_0
+Mat shift_matrix(int chi, int ls, double mass)
+{
+  Mat S(ls * ls, 0.0);
+  for (int s = 0; s < ls; ++s) {
+    if (chi == 0) {
+      if (s >= 1) S[s * ls + s - 1] = 1.0;
+      else S[0 * ls + ls - 1] = -mass;
+    } else {
+      if (s <= ls - 2) S[s * ls + s + 1] = 1.0;
+      else S[(ls - 1) * ls + 0] = -mass;
+    }
+  }
+  return S;
+}
+Mat diag_plus(const std::vector<cd>& d, const std::vector<cd>& e, const Mat& S, int ls)
+{
+  // diag(d) + diag(e) S
+  Mat M(ls * ls, 0.0);
+  for (int i = 0; i < ls; ++i) {
+    M[i * ls + i] += d[i];
+    for (int j = 0; j < ls; ++j) M[i * ls + j] += e[i] * S[i * ls + j];
+  }
+  return M;
+}
+
+// ---------------------------------------------------------------- state
+enum Work64 {
+  W_IN_E, W_IN_O, W_DM_E, W_DM_O, W_OUT_EP, W_IN_OP, W_OUT_OP, W_TMP, W_ITMP, W_OUT_E, W_OUT_O,
+  W_CG_R, W_CG_P, W_CG_AP, W_CG_X, W_OP_T1, W_OP_T2, W_OP_Y, W_COUNT64
+};
+enum Work32 { V_R, V_P, V_AP, V_X, V_T1, V_T2, V_Y, W_COUNT32 };
+
+}  // namespace
+
+struct qlat_dwf_inverter {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  LatGeo geo{};
+  int ls = 0;
+  double mass = 0, m5 = 0, mobius_scale = 1;
+  bool dminus = true;
+  std::vector<cd> bs, cs;
+  double stop_rsd = 1e-8;
+  long max_num_iter = 200;
+  long max_cycle = 300;
+  int solver = QLAT_DWF_SOLVER_FP64;
+  double delta = 0.1;
+  int hop_spt64 = 2, hop_spt32 = 4;
+
+  double2* l64 = nullptr;
+  float2* l32 = nullptr;
+  cplx<double>* mat64 = nullptr;  // [MAT_COUNT][2][ls][ls]
+  cplx<float>* mat32 = nullptr;
+  cplx<double>* beta64 = nullptr;  // [2][ls]: ones, c~
+  cplx<float>* beta32 = nullptr;
+  double2* w64[W_COUNT64] = {};
+  float4* w32[W_COUNT32] = {};
+  double2* stage[2] = {nullptr, nullptr};
+  size_t stage_bytes = 0;
+  double* part = nullptr;
+  double* red = nullptr;     // device reduction results
+  double* red_host = nullptr;  // pinned
+
+  qlat_dwf_solve_stats stats{};
+};
+
+namespace {
+
+typedef qlat_dwf_inverter Inv;
+
+size_t field64_bytes(const Inv& I) { return (size_t)I.ls * 12 * I.geo.stride * sizeof(double2); }
+size_t field32_bytes(const Inv& I) { return (size_t)I.ls * 6 * I.geo.stride * sizeof(float4); }
+int64_t n64(const Inv& I) { return (int64_t)I.ls * 12 * I.geo.stride; }
+int64_t n32(const Inv& I) { return (int64_t)I.ls * 6 * I.geo.stride; }
+
+void* dev_alloc(size_t bytes)
+{
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw MemError("hipMalloc(" + std::to_string(bytes) + " bytes) failed: " + hipGetErrorString(e));
+  }
+  return p;
+}
+
+double2* W(Inv& I, int id)
+{
+  if (!I.w64[id]) {
+    I.w64[id] = (double2*)dev_alloc(field64_bytes(I));
+    QDWF_HIP(hipMemsetAsync(I.w64[id], 0, field64_bytes(I), I.stream));
+  }
+  return I.w64[id];
+}
+float4* V(Inv& I, int id)
+{
+  if (!I.w32[id]) {
+    I.w32[id] = (float4*)dev_alloc(field32_bytes(I));
+    QDWF_HIP(hipMemsetAsync(I.w32[id], 0, field32_bytes(I), I.stream));
+  }
+  return I.w32[id];
+}
+size_t stage_size(const Inv& I);
+
+// Two device staging buffers for host-pointer entry points, each large
+// enough for a 5D full field or a propagator, allocated on first use.
+double2* stage(Inv& I, int k, size_t bytes)
+{
+  if (bytes > stage_size(I)) throw ArgError("field larger than the staging buffer");
+  if (!I.stage[k]) {
+    I.stage_bytes = stage_size(I);
+    I.stage[k] = (double2*)dev_alloc(I.stage_bytes);
+  }
+  return I.stage[k];
+}
+
+void sync(Inv& I) { QDWF_HIP(hipStreamSynchronize(I.stream)); }
+void check_launch() { QDWF_HIP(hipGetLastError()); }
+
+// ---------------------------------------------------------- typed ops
+template <class T>
+struct Prec;
+template <>
+struct Prec<double> {
+  typedef double2 slot;
+  static const cplx<double>* mat(const Inv& I, int id) { return I.mat64 + (size_t)id * 2 * I.ls * I.ls; }
+  static const cplx<double>* beta(const Inv& I, int which) { return I.beta64 + (size_t)which * I.ls; }
+  static const void* links(const Inv& I) { return I.l64; }
+  static int spt(const Inv& I) { return I.hop_spt64; }
+  static int64_t n(const Inv& I) { return n64(I); }
+};
+template <>
+struct Prec<float> {
+  typedef float4 slot;
+  static const cplx<float>* mat(const Inv& I, int id) { return I.mat32 + (size_t)id * 2 * I.ls * I.ls; }
+  static const cplx<float>* beta(const Inv& I, int which) { return I.beta32 + (size_t)which * I.ls; }
+  static const void* links(const Inv& I) { return I.l32; }
+  static int spt(const Inv& I) { return I.hop_spt32; }
+  static int64_t n(const Inv& I) { return n32(I); }
+};
+
+// out = alpha * Mat in + beta * x
+template <class T>
+void m5(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int mat, T alpha = 1,
+        const typename Prec<T>::slot* x = nullptr, T beta = 0)
+{
+  qdwf::M5Args<T> a;
+  a.geo = I.geo;
+  a.ls = I.ls;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.mat = Prec<T>::mat(I, mat);
+  a.in = in;
+  a.out = out;
+  a.x = x;
+  qdwf::launch_m5<T>(a, I.stream);
+  check_launch();
+}
+
+// out(parity eo_out) = alpha * H in + beta_s * x   (beta: 0 ones, 1 c~)
+template <class T>
+void hop(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo_out, bool dag,
+         T alpha = 1, const typename Prec<T>::slot* x = nullptr, int beta = 0)
+{
+  qdwf::HopArgs<T> a;
+  a.geo = I.geo;
+  a.ls = I.ls;
+  a.eo_out = eo_out;
+  a.nchunk = 0;
+  a.alpha = alpha;
+  a.in = in;
+  a.out = out;
+  a.x = x;
+  a.beta = Prec<T>::beta(I, beta);
+  a.links = Prec<T>::links(I);
+  qdwf::launch_hop<T>(a, dag, Prec<T>::spt(I), I.stream);
+  check_launch();
+}
+
+// Temporaries of the typed composite operators.
+template <class T>
+struct Tmp;
+template <>
+struct Tmp<double> {
+  static double2* t1(Inv& I) { return W(I, W_OP_T1); }
+  static double2* t2(Inv& I) { return W(I, W_OP_T2); }
+  static double2* y(Inv& I) { return W(I, W_OP_Y); }
+};
+template <>
+struct Tmp<float> {
+  static float4* t1(Inv& I) { return V(I, V_T1); }
+  static float4* t2(Inv& I) { return V(I, V_T2); }
+  static float4* y(Inv& I) { return V(I, V_Y); }
+};
+
+// multiply_m_e_o: out(3-eo_in) = H P in
+template <class T>
+void op_m_e_o(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo_in)
+{
+  auto* t = Tmp<T>::t1(I);
+  m5<T>(I, t, in, MAT_P);
+  hop<T>(I, out, t, 3 - eo_in, false);
+}
+
+// multiply_mdag_e_o: out(3-eo_in) = P^+ H^+ in
+template <class T>
+void op_mdag_e_o(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo_in)
+{
+  auto* t = Tmp<T>::t1(I);
+  hop<T>(I, t, in, 3 - eo_in, true);
+  m5<T>(I, out, t, MAT_P_DAG);
+}
+
+// Mpc = 1 - H K H K
+template <class T>
+void op_mpc(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo)
+{
+  auto* t1 = Tmp<T>::t1(I);
+  auto* t2 = Tmp<T>::t2(I);
+  m5<T>(I, t1, in, MAT_K);
+  hop<T>(I, t2, t1, 3 - eo, false);
+  m5<T>(I, t1, t2, MAT_K);
+  hop<T>(I, out, t1, eo, false, (T)-1, in, 0);
+}
+
+// Mpc^+ = 1 - K^+ H^+ K^+ H^+
+template <class T>
+void op_mpcdag(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo)
+{
+  auto* t1 = Tmp<T>::t1(I);
+  auto* t2 = Tmp<T>::t2(I);
+  hop<T>(I, t1, in, 3 - eo, true);
+  m5<T>(I, t2, t1, MAT_K_DAG);
+  hop<T>(I, t1, t2, eo, true);
+  m5<T>(I, out, t1, MAT_K_DAG, (T)-1, in, (T)1);
+}
+
+// Mpc^+ Mpc
+template <class T>
+void op_hermop(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo)
+{
+  auto* y = Tmp<T>::y(I);
+  op_mpc<T>(I, y, in, eo);
+  op_mpcdag<T>(I, out, y, eo);
+}
+
+// ------------------------------------------------------------ BLAS-1
+double dev_norm2(Inv& I, const double2* a)
+{
+  qdwf::launch_norm2<double2>(a, n64(I), I.part, I.red, I.stream);
+  check_launch();
+  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, sizeof(double), hipMemcpyDeviceToHost, I.stream));
+  sync(I);
+  return I.red_host[0];
+}
+template <class S>
+cd dev_dot(Inv& I, const S* a, const S* b, int64_t n)
+{
+  qdwf::launch_dot<S>(a, b, n, I.part, I.red, I.stream);
+  check_launch();
+  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, 2 * sizeof(double), hipMemcpyDeviceToHost, I.stream));
+  sync(I);
+  return cd(I.red_host[0], I.red_host[1]);
+}
+template <class S>
+double dev_cg_update(Inv& I, double alpha, const S* p, const S* ap, S* x, S* r, int64_t n)
+{
+  qdwf::launch_cg_update<S>(alpha, p, ap, x, r, n, I.part, I.red, I.stream);
+  check_launch();
+  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, sizeof(double), hipMemcpyDeviceToHost, I.stream));
+  sync(I);
+  return I.red_host[0];
+}
+void copy64(Inv& I, double2* dst, const double2* src)
+{
+  if (dst != src) QDWF_HIP(hipMemcpyAsync(dst, src, field64_bytes(I), hipMemcpyDeviceToDevice, I.stream));
+}
+void zero64(Inv& I, double2* dst) { QDWF_HIP(hipMemsetAsync(dst, 0, field64_bytes(I), I.stream)); }
+void zero32(Inv& I, float4* dst) { QDWF_HIP(hipMemsetAsync(dst, 0, field32_bytes(I), I.stream)); }
+
+// --------------------------------------------------- full-lattice ops
+// multiply_d_minus (dslash.cpp:448-480): D_- psi = psi + D_W(-M5)(-c psi)
+//   = diag(c~) psi + H (-c psi), c~ = 1 - c (4 - M5)
+void op_d_minus(Inv& I, double2* out_e, double2* out_o, const double2* in_e, const double2* in_o)
+{
+  double2* t1 = W(I, W_OP_T1);
+  double2* t2 = W(I, W_OP_T2);
+  m5<double>(I, t1, in_o, MAT_NEGC);
+  m5<double>(I, t2, in_e, MAT_NEGC);
+  hop<double>(I, out_e, t1, 2, false, 1.0, in_e, 1);
+  hop<double>(I, out_o, t2, 1, false, 1.0, in_o, 1);
+}
+
+// multiply_m (dslash.cpp:1062-1080): out_p = M_ee in_p + H P in_{3-p}
+void op_m_full(Inv& I, double2* out_e, double2* out_o, const double2* in_e, const double2* in_o)
+{
+  double2* t1 = W(I, W_OP_T1);
+  double2* t2 = W(I, W_OP_T2);
+  m5<double>(I, t1, in_o, MAT_P);
+  m5<double>(I, t2, in_e, MAT_MEE);
+  hop<double>(I, out_e, t1, 2, false, 1.0, t2, 0);
+  m5<double>(I, t1, in_e, MAT_P);
+  m5<double>(I, t2, in_o, MAT_MEE);
+  hop<double>(I, out_o, t1, 1, false, 1.0, t2, 0);
+}
+
+// multiply_mdag (dslash.cpp:1082-1100): out_p = M_ee^+ in_p + P^+ H^+ in_{3-p}
+void op_mdag_full(Inv& I, double2* out_e, double2* out_o, const double2* in_e, const double2* in_o)
+{
+  double2* t1 = W(I, W_OP_T1);
+  double2* t2 = W(I, W_OP_T2);
+  hop<double>(I, t1, in_o, 2, true);
+  m5<double>(I, t2, in_e, MAT_MEE_DAG);
+  m5<double>(I, out_e, t1, MAT_P_DAG, 1.0, t2, 1.0);
+  hop<double>(I, t1, in_e, 1, true);
+  m5<double>(I, t2, in_o, MAT_MEE_DAG);
+  m5<double>(I, out_o, t1, MAT_P_DAG, 1.0, t2, 1.0);
+}
+
+// ------------------------------------------------------------- solvers
+// cg_with_f (dslash.h:331-399) with f = multiply_hermop_sym2, FP64.
+// x: initial guess (guess_zero skips the A x evaluation, which is exact).
+long cg_fp64(Inv& I, double2* x, const double2* b, double stop_rsd, long max_num_iter, bool guess_zero)
+{
+  if (max_num_iter == 0) return 0;
+  double2* r = W(I, W_CG_R);
+  double2* p = W(I, W_CG_P);
+  double2* ap = W(I, W_CG_AP);
+  const int64_t n = n64(I);
+  if (guess_zero) {
+    copy64(I, r, b);
+  } else {
+    op_hermop<double>(I, ap, x, 1);
+    I.stats.hermop_fp64 += 1;
+    copy64(I, r, b);
+    qdwf::launch_axpy<double2>(-1.0, ap, r, n, I.stream);
+    check_launch();
+  }
+  copy64(I, p, r);
+  const double qnorm_in = dev_norm2(I, b);
+  double qnorm_r = dev_norm2(I, r);
+  for (long iter = 1; iter <= max_num_iter; ++iter) {
+    op_hermop<double>(I, ap, p, 1);
+    I.stats.hermop_fp64 += 1;
+    const double alpha = qnorm_r / dev_dot<double2>(I, p, ap, n).real();
+    const double new_qnorm_r = dev_cg_update<double2>(I, alpha, p, ap, x, r, n);
+    if (!std::isfinite(new_qnorm_r)) throw std::runtime_error("CG breakdown: non-finite residual (FP64)");
+    if (new_qnorm_r <= qnorm_in * stop_rsd * stop_rsd) return iter;
+    const double beta = new_qnorm_r / qnorm_r;
+    qdwf::launch_xpay<double2>(r, beta, p, n, I.stream);
+    check_launch();
+    qnorm_r = new_qnorm_r;
+  }
+  return max_num_iter + 1;
+}
+
+// Mixed-precision CG on A x = b (odd half fields): the Krylov recurrence
+// runs in FP32; whenever the iterated residual has dropped by `delta` since
+// the last replacement (and at convergence) the solution increment is
+// folded into the FP64 x and the residual is recomputed as b - A x in FP64.
+// Returns iterations; stops when |b - A x| <= stop_rsd |b| in FP64.
+long cg_mixed(Inv& I, double2* x64, const double2* b64, double stop_rsd, long max_total_iter, double* true_rel)
+{
+  double2* r64 = W(I, W_CG_R);
+  double2* ax64 = W(I, W_CG_AP);
+  float4* r = V(I, V_R);
+  float4* p = V(I, V_P);
+  float4* ap = V(I, V_AP);
+  float4* x = V(I, V_X);
+  const int64_t n = n32(I), nd = n64(I);
+  const double b2 = dev_norm2(I, b64);
+  const double target = stop_rsd * stop_rsd * b2;
+  // r64 = b - A x64
+  op_hermop<double>(I, ax64, x64, 1);
+  I.stats.hermop_fp64 += 1;
+  copy64(I, r64, b64);
+  qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
+  check_launch();
+  double rr = dev_norm2(I, r64);
+  if (rr <= target) {
+    *true_rel = std::sqrt(rr / b2);
+    return 0;
+  }
+  qdwf::launch_d2f(r64, r, I.ls, I.geo.stride, I.stream);
+  check_launch();
+  QDWF_HIP(hipMemcpyAsync(p, r, field32_bytes(I), hipMemcpyDeviceToDevice, I.stream));
+  zero32(I, x);
+  double maxrr = rr;
+  const double d2 = I.delta * I.delta;
+  long iter = 0;
+  while (iter < max_total_iter) {
+    op_hermop<float>(I, ap, p, 1);
+    I.stats.hermop_low += 1;
+    const double pap = dev_dot<float4>(I, p, ap, n).real();
+    const double alpha = rr / pap;
+    double rr_new = dev_cg_update<float4>(I, alpha, p, ap, x, r, n);
+    ++iter;
+    if (!std::isfinite(rr_new)) throw std::runtime_error("CG breakdown: non-finite residual (FP32)");
+    if (rr_new <= d2 * maxrr || rr_new <= target) {
+      // reliable update: x64 += x32, r64 = b - A x64 (FP64)
+      qdwf::launch_f2d(x, x64, I.ls, I.geo.stride, 1, I.stream);
+      check_launch();
+      zero32(I, x);
+      op_hermop<double>(I, ax64, x64, 1);
+      I.stats.hermop_fp64 += 1;
+      copy64(I, r64, b64);
+      qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
+      check_launch();
+      const double rr_true = dev_norm2(I, r64);
+      I.stats.reliable_updates += 1;
+      if (rr_true <= target) {
+        *true_rel = std::sqrt(rr_true / b2);
+        return iter;
+      }
+      qdwf::launch_d2f(r64, r, I.ls, I.geo.stride, I.stream);
+      check_launch();
+      rr_new = rr_true;
+      maxrr = rr_true;
+    }
+    const double beta = rr_new / rr;
+    qdwf::launch_xpay<float4>(r, beta, p, n, I.stream);
+    check_launch();
+    rr = rr_new;
+  }
+  // not converged: fold in what we have
+  qdwf::launch_f2d(x, x64, I.ls, I.geo.stride, 1, I.stream);
+  check_launch();
+  op_hermop<double>(I, ax64, x64, 1);
+  I.stats.hermop_fp64 += 1;
+  copy64(I, r64, b64);
+  qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
+  check_launch();
+  *true_rel = std::sqrt(dev_norm2(I, r64) / b2);
+  return max_total_iter + 1;
+}
+
+double true_residual_fp64(Inv& I, const double2* x, const double2* b)
+{
+  double2* r = W(I, W_CG_R);
+  double2* ax = W(I, W_CG_AP);
+  op_hermop<double>(I, ax, x, 1);
+  I.stats.hermop_fp64 += 1;
+  copy64(I, r, b);
+  qdwf::launch_axpy<double2>(-1.0, ax, r, n64(I), I.stream);
+  check_launch();
+  const double b2 = dev_norm2(I, b);
+  return b2 > 0 ? std::sqrt(dev_norm2(I, r) / b2) : 0.0;
+}
+
+// invert_with_cg (dslash.h:440-530) on device halves; in_e/in_o: 5D source
+// halves (FP64), results in out_e/out_o.
+long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* out_e, double2* out_o)
+{
+  double2* dm_e = W(I, W_DM_E);
+  double2* dm_o = W(I, W_DM_O);
+  if (I.dminus) {
+    op_d_minus(I, dm_e, dm_o, in_e, in_o);
+  } else {
+    copy64(I, dm_e, in_e);
+    copy64(I, dm_o, in_o);
+  }
+  const double dm_in_qnorm = dev_norm2(I, dm_e) + dev_norm2(I, dm_o);
+  if (dm_in_qnorm == 0.0) {
+    zero64(I, out_e);
+    zero64(I, out_o);
+    return 0;
+  }
+  // set_odd_prec_field_sym2 (dslash.h:401-417)
+  double2* out_e_p = W(I, W_OUT_EP);
+  double2* in_o_p = W(I, W_IN_OP);
+  double2* tmp = W(I, W_TMP);
+  m5<double>(I, out_e_p, dm_e, MAT_MEE_INV);
+  op_m_e_o<double>(I, tmp, out_e_p, 2);
+  copy64(I, in_o_p, dm_o);
+  qdwf::launch_axpy<double2>(-1.0, tmp, in_o_p, n64(I), I.stream);
+  check_launch();
+  op_mpcdag<double>(I, in_o_p, in_o_p, 1);
+  double2* out_o_p = W(I, W_OUT_OP);
+  zero64(I, out_o_p);
+  long total_iter = 0;
+  double true_rel = 0.0;
+  if (I.solver == QLAT_DWF_SOLVER_FP64) {
+    // outer restart loop (dslash.h:495-514)
+    double2* itmp = W(I, W_ITMP);
+    double2* x = W(I, W_CG_X);
+    const double qnorm_in_o_p = dev_norm2(I, in_o_p);
+    copy64(I, itmp, in_o_p);
+    double qnorm_itmp = qnorm_in_o_p;
+    long cycle;
+    for (cycle = 1; cycle <= I.max_cycle; ++cycle) {
+      zero64(I, x);
+      const long iter =
+          cg_fp64(I, x, itmp, I.stop_rsd * std::sqrt(qnorm_in_o_p / qnorm_itmp), I.max_num_iter, true);
+      total_iter += iter;
+      qdwf::launch_axpy<double2>(1.0, x, out_o_p, n64(I), I.stream);
+      check_launch();
+      if (iter <= I.max_num_iter) break;
+      op_hermop<double>(I, itmp, out_o_p, 1);
+      I.stats.hermop_fp64 += 1;
+      qdwf::launch_xpay<double2>(in_o_p, -1.0, itmp, n64(I), I.stream);
+      check_launch();
+      qnorm_itmp = dev_norm2(I, itmp);
+    }
+    I.stats.cycles += (cycle > I.max_cycle) ? I.max_cycle : cycle;
+    true_rel = true_residual_fp64(I, out_o_p, in_o_p);
+  } else {
+    const long cap = I.max_num_iter * I.max_cycle;
+    total_iter = cg_mixed(I, out_o_p, in_o_p, I.stop_rsd, cap, &true_rel);
+    I.stats.cycles += 1;
+  }
+  I.stats.total_iter += total_iter;
+  if (true_rel > I.stats.true_residual) I.stats.true_residual = true_rel;
+  // restore_field_from_odd_prec_sym2 (dslash.h:419-438)
+  m5<double>(I, out_o, out_o_p, MAT_MEE_INV);
+  op_m_e_o<double>(I, tmp, out_o, 1);
+  m5<double>(I, W(I, W_ITMP), tmp, MAT_MEE_INV);
+  copy64(I, out_e, out_e_p);
+  qdwf::launch_axpy<double2>(-1.0, W(I, W_ITMP), out_e, n64(I), I.stream);
+  check_launch();
+  return total_iter;
+}
+
+void reset_stats(Inv& I) { std::memset(&I.stats, 0, sizeof(I.stats)); }
+
+// Upload a host AoS buffer into a device staging buffer.
+const double2* upload(Inv& I, int k, const double* host, size_t bytes)
+{
+  double2* d = stage(I, k, bytes);
+  QDWF_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, I.stream));
+  return d;
+}
+void download(Inv& I, double* host, const double2* dev, size_t bytes)
+{
+  QDWF_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, I.stream));
+  sync(I);
+}
+
+size_t half_aos_bytes(const Inv& I) { return (size_t)I.geo.vh * I.ls * 12 * sizeof(double2); }
+size_t full_aos_bytes(const Inv& I) { return 2 * half_aos_bytes(I); }
+size_t prop_bytes(const Inv& I) { return (size_t)I.geo.vh * 2 * 144 * sizeof(double2); }
+size_t stage_size(const Inv& I)
+{
+  const size_t a = full_aos_bytes(I), b = prop_bytes(I);
+  return a > b ? a : b;
+}
+
+template <class F>
+int guarded(F&& f)
+{
+  try {
+    f();
+    g_last_error.clear();
+    return QLAT_DWF_OK;
+  } catch (const ArgError& e) {
+    g_last_error = e.what();
+    return QLAT_DWF_EINVAL;
+  } catch (const MemError& e) {
+    g_last_error = e.what();
+    return QLAT_DWF_ENOMEM;
+  } catch (const HipError& e) {
+    g_last_error = e.what();
+    return QLAT_DWF_EHIP;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return QLAT_DWF_ESTATE;
+  }
+}
+
+void check_inv(const Inv* I)
+{
+  if (!I) throw ArgError("null inverter handle");
+  QDWF_HIP(hipSetDevice(I->device));
+}
+void check_eo(int eo) { QDWF_REQUIRE(eo == 1 || eo == 2, "parity eo must be 1 (odd) or 2 (even)"); }
+
+enum HalfOp { H_MEE, H_MEE_DAG, H_MEE_INV, H_MEE_INVDAG, H_HOP, H_HOP_DAG, H_MEO, H_MDAG_EO, H_MPC, H_MPCDAG, H_HERMOP };
+
+void apply_half_dev(Inv& I, int op, double2* out_soa, const double2* in_soa, int eo)
+{
+  switch (op) {
+    case H_MEE: m5<double>(I, out_soa, in_soa, MAT_MEE); break;
+    case H_MEE_DAG: m5<double>(I, out_soa, in_soa, MAT_MEE_DAG); break;
+    case H_MEE_INV: m5<double>(I, out_soa, in_soa, MAT_MEE_INV); break;
+    case H_MEE_INVDAG: m5<double>(I, out_soa, in_soa, MAT_MEE_INVDAG); break;
+    case H_HOP: hop<double>(I, out_soa, in_soa, 3 - eo, false); break;
+    case H_HOP_DAG: hop<double>(I, out_soa, in_soa, 3 - eo, true); break;
+    case H_MEO: op_m_e_o<double>(I, out_soa, in_soa, eo); break;
+    case H_MDAG_EO: op_mdag_e_o<double>(I, out_soa, in_soa, eo); break;
+    case H_MPC: op_mpc<double>(I, out_soa, in_soa, eo); break;
+    case H_MPCDAG: op_mpcdag<double>(I, out_soa, in_soa, eo); break;
+    case H_HERMOP: op_hermop<double>(I, out_soa, in_soa, eo); break;
+    default: throw ArgError("unknown operator");
+  }
+}
+
+int apply_half_host(Inv* I, int op, double* out, const double* in, int eo)
+{
+  return guarded([&] {
+    check_inv(I);
+    check_eo(eo);
+    QDWF_REQUIRE(out && in, "null field pointer");
+    const size_t bytes = half_aos_bytes(*I);
+    const double2* d_in = upload(*I, 0, in, bytes);
+    double2* a = W(*I, W_IN_E);
+    double2* b = W(*I, W_OUT_E);
+    qdwf::launch_pack5(d_in, a, I->geo, I->ls, eo, 0, I->stream);
+    check_launch();
+    apply_half_dev(*I, op, b, a, eo);
+    double2* d_out = stage(*I, 1, bytes);
+    qdwf::launch_unpack5(b, d_out, I->geo, I->ls, eo, 0, I->stream);
+    check_launch();
+    download(*I, out, d_out, bytes);
+  });
+}
+
+enum FullOp { F_M, F_MDAG, F_DMINUS };
+
+void apply_full_dev(Inv& I, int op, double2* out_aos, const double2* in_aos)
+{
+  double2* ie = W(I, W_IN_E);
+  double2* io = W(I, W_IN_O);
+  double2* oe = W(I, W_OUT_E);
+  double2* oo = W(I, W_OUT_O);
+  qdwf::launch_pack5(in_aos, ie, I.geo, I.ls, 2, 1, I.stream);
+  qdwf::launch_pack5(in_aos, io, I.geo, I.ls, 1, 1, I.stream);
+  check_launch();
+  switch (op) {
+    case F_M: op_m_full(I, oe, oo, ie, io); break;
+    case F_MDAG: op_mdag_full(I, oe, oo, ie, io); break;
+    case F_DMINUS: op_d_minus(I, oe, oo, ie, io); break;
+    default: throw ArgError("unknown operator");
+  }
+  qdwf::launch_unpack5(oe, out_aos, I.geo, I.ls, 2, 1, I.stream);
+  qdwf::launch_unpack5(oo, out_aos, I.geo, I.ls, 1, 1, I.stream);
+  check_launch();
+}
+
+int apply_full_host(Inv* I, int op, double* out, const double* in)
+{
+  return guarded([&] {
+    check_inv(I);
+    QDWF_REQUIRE(out && in, "null field pointer");
+    const size_t bytes = full_aos_bytes(*I);
+    const double2* d_in = upload(*I, 0, in, bytes);
+    double2* d_out = stage(*I, 1, bytes);
+    apply_full_dev(*I, op, d_out, d_in);
+    download(*I, out, d_out, bytes);
+  });
+}
+
+// 5D full AoS (device) -> solve -> 5D full AoS (device)
+long invert_5d_dev(Inv& I, double2* out_aos, const double2* in_aos)
+{
+  double2* ie = W(I, W_IN_E);
+  double2* io = W(I, W_IN_O);
+  double2* oe = W(I, W_OUT_E);
+  double2* oo = W(I, W_OUT_O);
+  qdwf::launch_pack5(in_aos, ie, I.geo, I.ls, 2, 1, I.stream);
+  qdwf::launch_pack5(in_aos, io, I.geo, I.ls, 1, 1, I.stream);
+  check_launch();
+  const long it = invert_halves(I, ie, io, oe, oo);
+  qdwf::launch_unpack5(oe, out_aos, I.geo, I.ls, 2, 1, I.stream);
+  qdwf::launch_unpack5(oo, out_aos, I.geo, I.ls, 1, 1, I.stream);
+  check_launch();
+  return it;
+}
+
+// Propagator (or 4D fermion, ncol = 1 with col = -1) on device.
+long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
+{
+  double2* ie = W(I, W_IN_E);
+  double2* io = W(I, W_IN_O);
+  double2* oe = W(I, W_OUT_E);
+  double2* oo = W(I, W_OUT_O);
+  long total = 0;
+  const int ncol = is_prop ? 12 : 1;
+  for (int j = 0; j < ncol; ++j) {
+    const int col = is_prop ? j : -1;
+    qdwf::launch_prop_col_to_5d(src, ie, I.geo, I.ls, 2, col, I.stream);
+    qdwf::launch_prop_col_to_5d(src, io, I.geo, I.ls, 1, col, I.stream);
+    check_launch();
+    total += invert_halves(I, ie, io, oe, oo);
+    qdwf::launch_5d_to_prop_col(oe, sol, I.geo, I.ls, 2, col, I.stream);
+    qdwf::launch_5d_to_prop_col(oo, sol, I.geo, I.ls, 1, col, I.stream);
+    check_launch();
+  }
+  return total;
+}
+
+void build_matrices(Inv& I)
+{
+  const int ls = I.ls;
+  std::vector<cd> bt(ls), ct(ls), negc(ls);
+  for (int s = 0; s < ls; ++s) {
+    bt[s] = 1.0 + I.bs[s] * (4.0 - I.m5);  // dslash.cpp:617-621
+    ct[s] = 1.0 - I.cs[s] * (4.0 - I.m5);
+    negc[s] = -I.cs[s];
+  }
+  std::vector<cd> ones(ls, 1.0), zeros(ls, 0.0);
+  std::vector<cd> all((size_t)MAT_COUNT * 2 * ls * ls);
+  for (int chi = 0; chi < 2; ++chi) {
+    const Mat S = shift_matrix(chi, ls, I.mass);
+    std::vector<cd> mct(ls);
+    for (int s = 0; s < ls; ++s) mct[s] = -ct[s];
+    const Mat mee = diag_plus(bt, mct, S, ls);
+    const Mat mee_inv = mat_inv(mee, ls);
+    const Mat pm = diag_plus(I.bs, I.cs, S, ls);
+    const Mat k = mat_mul(pm, mee_inv, ls);
+    Mat negcm(ls * ls, 0.0);
+    for (int s = 0; s < ls; ++s) negcm[s * ls + s] = negc[s];
+    const Mat list[MAT_COUNT] = {mee, mat_adj(mee, ls), mee_inv, mat_adj(mee_inv, ls), pm, mat_adj(pm, ls),
+                                 k, mat_adj(k, ls), negcm};
+    for (int m = 0; m < MAT_COUNT; ++m)
+      std::memcpy(&all[((size_t)m * 2 + chi) * ls * ls], list[m].data(), sizeof(cd) * ls * ls);
+  }
+  std::vector<cplx<double>> h64(all.size());
+  std::vector<cplx<float>> h32(all.size());
+  for (size_t i = 0; i < all.size(); ++i) {
+    h64[i].re = all[i].real();
+    h64[i].im = all[i].imag();
+    h32[i].re = (float)all[i].real();
+    h32[i].im = (float)all[i].imag();
+  }
+  I.mat64 = (cplx<double>*)dev_alloc(h64.size() * sizeof(cplx<double>));
+  I.mat32 = (cplx<float>*)dev_alloc(h32.size() * sizeof(cplx<float>));
+  QDWF_HIP(hipMemcpy(I.mat64, h64.data(), h64.size() * sizeof(cplx<double>), hipMemcpyHostToDevice));
+  QDWF_HIP(hipMemcpy(I.mat32, h32.data(), h32.size() * sizeof(cplx<float>), hipMemcpyHostToDevice));
+  std::vector<cplx<double>> b64(2 * ls);
+  std::vector<cplx<float>> b32(2 * ls);
+  for (int s = 0; s < ls; ++s) {
+    b64[s].re = 1.0;
+    b64[s].im = 0.0;
+    b64[ls + s].re = ct[s].real();
+    b64[ls + s].im = ct[s].imag();
+    b32[s].re = 1.0f;
+    b32[s].im = 0.0f;
+    b32[ls + s].re = (float)ct[s].real();
+    b32[ls + s].im = (float)ct[s].imag();
+  }
+  I.beta64 = (cplx<double>*)dev_alloc(b64.size() * sizeof(cplx<double>));
+  I.beta32 = (cplx<float>*)dev_alloc(b32.size() * sizeof(cplx<float>));
+  QDWF_HIP(hipMemcpy(I.beta64, b64.data(), b64.size() * sizeof(cplx<double>), hipMemcpyHostToDevice));
+  QDWF_HIP(hipMemcpy(I.beta32, b32.data(), b32.size() * sizeof(cplx<float>), hipMemcpyHostToDevice));
+}
+
+void free_all(Inv* I)
+{
+  if (!I) return;
+  (void)hipSetDevice(I->device);
+  if (I->stream) (void)hipStreamSynchronize(I->stream);
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(I->l64);
+  f(I->l32);
+  f(I->mat64);
+  f(I->mat32);
+  f(I->beta64);
+  f(I->beta32);
+  for (auto* p : I->w64) f(p);
+  for (auto* p : I->w32) f(p);
+  for (auto* p : I->stage) f(p);
+  f(I->part);
+  f(I->red);
+  if (I->red_host) (void)hipHostFree(I->red_host);
+  if (I->stream) (void)hipStreamDestroy(I->stream);
+  delete I;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" {
+
+const char* qlat_dwf_last_error(void) { return g_last_error.c_str(); }
+
+int qlat_dwf_abi_version(void) { return 1; }
+
+int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], const double* gauge,
+                             const qlat_dwf_fermion_action* fa)
+{
+  Inv* I = nullptr;
+  const int rc = guarded([&] {
+    QDWF_REQUIRE(out && total_site && gauge && fa, "null argument to qlat_dwf_inverter_create");
+    *out = nullptr;
+    int ndev = 0;
+    QDWF_HIP(hipGetDeviceCount(&ndev));
+    QDWF_REQUIRE(ndev > 0, "no HIP device visible");
+    int64_t V = 1;
+    for (int i = 0; i < 4; ++i) {
+      QDWF_REQUIRE(total_site[i] >= 2 && total_site[i] % 2 == 0,
+                   "every lattice extent must be even and >= 2 (even-odd layout, core.h:232-261)");
+      V *= total_site[i];
+    }
+    QDWF_REQUIRE(fa->ls >= 2 && fa->ls <= 32, "ls must be in [2, 32]");
+    QDWF_REQUIRE(fa->cg_diagonal_mee == 2 || fa->cg_diagonal_mee == 0,
+                 "only cg_diagonal_mee == 2 is supported (dslash.h:483, :520)");
+    I = new Inv;
+    QDWF_HIP(hipGetDevice(&I->device));
+    QDWF_HIP(hipStreamCreate(&I->stream));
+    for (int i = 0; i < 4; ++i) I->geo.n[i] = total_site[i];
+    I->geo.xh = total_site[0] / 2;
+    I->geo.vh = V / 2;
+    I->geo.stride = (I->geo.vh + qdwf::kStrideAlign - 1) / qdwf::kStrideAlign * qdwf::kStrideAlign;
+    I->ls = fa->ls;
+    I->mass = fa->mass;
+    I->m5 = fa->m5;
+    I->mobius_scale = fa->mobius_scale;
+    I->dminus = fa->is_multiplying_dminus != 0;
+    I->bs.resize(fa->ls);
+    I->cs.resize(fa->ls);
+    for (int s = 0; s < fa->ls; ++s) {
+      if (fa->bs && fa->cs) {
+        I->bs[s] = cd(fa->bs[2 * s], fa->bs[2 * s + 1]);
+        I->cs[s] = cd(fa->cs[2 * s], fa->cs[2 * s + 1]);
+      } else {
+        I->bs[s] = 0.5 * fa->mobius_scale + 0.5;
+        I->cs[s] = 0.5 * fa->mobius_scale - 0.5;
+      }
+    }
+    // gauge upload and conversion
+    const size_t gbytes = (size_t)V * 36 * sizeof(double2);
+    double2* g = (double2*)dev_alloc(gbytes);
+    QDWF_HIP(hipMemcpyAsync(g, gauge, gbytes, hipMemcpyHostToDevice, I->stream));
+    const size_t lcount = (size_t)2 * 8 * 9 * I->geo.stride;
+    I->l64 = (double2*)dev_alloc(lcount * sizeof(double2));
+    I->l32 = (float2*)dev_alloc(lcount * sizeof(float2));
+    QDWF_HIP(hipMemsetAsync(I->l64, 0, lcount * sizeof(double2), I->stream));
+    QDWF_HIP(hipMemsetAsync(I->l32, 0, lcount * sizeof(float2), I->stream));
+    qdwf::launch_pack_gauge(g, I->l64, I->l32, I->geo, I->stream);
+    check_launch();
+    QDWF_HIP(hipStreamSynchronize(I->stream));
+    QDWF_HIP(hipFree(g));
+    build_matrices(*I);
+    I->part = (double*)dev_alloc(qdwf::kMaxBlasGrid * 4 * sizeof(double));
+    I->red = (double*)dev_alloc(8 * sizeof(double));
+    QDWF_HIP(hipHostMalloc((void**)&I->red_host, 8 * sizeof(double), hipHostMallocDefault));
+    *out = I;
+  });
+  if (rc != QLAT_DWF_OK && I) {
+    const std::string msg = g_last_error;
+    free_all(I);
+    g_last_error = msg;
+  }
+  return rc;
+}
+
+int qlat_dwf_inverter_destroy(qlat_dwf_inverter* inv)
+{
+  return guarded([&] {
+    if (inv) free_all(inv);
+  });
+}
+
+int qlat_dwf_inverter_set_params(qlat_dwf_inverter* inv, double stop_rsd, long max_num_iter,
+                                 long max_mixed_precision_cycle)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(inv, "null inverter handle");
+    QDWF_REQUIRE(stop_rsd >= 0 && max_num_iter >= 0 && max_mixed_precision_cycle >= 1, "invalid solver parameters");
+    inv->stop_rsd = stop_rsd;
+    inv->max_num_iter = max_num_iter;
+    inv->max_cycle = max_mixed_precision_cycle;
+  });
+}
+
+int qlat_dwf_inverter_get_params(const qlat_dwf_inverter* inv, double* stop_rsd, long* max_num_iter,
+                                 long* max_mixed_precision_cycle)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(inv, "null inverter handle");
+    if (stop_rsd) *stop_rsd = inv->stop_rsd;
+    if (max_num_iter) *max_num_iter = inv->max_num_iter;
+    if (max_mixed_precision_cycle) *max_mixed_precision_cycle = inv->max_cycle;
+  });
+}
+
+int qlat_dwf_inverter_set_solver(qlat_dwf_inverter* inv, int solver, double reliable_delta)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(inv, "null inverter handle");
+    QDWF_REQUIRE(solver == QLAT_DWF_SOLVER_FP64 || solver == QLAT_DWF_SOLVER_MIXED_FP32, "unknown solver");
+    QDWF_REQUIRE(reliable_delta >= 0 && reliable_delta < 1, "reliable_delta must be in [0, 1)");
+    inv->solver = solver;
+    inv->delta = reliable_delta > 0 ? reliable_delta : 0.1;
+  });
+}
+
+int qlat_dwf_inverter_get_stats(const qlat_dwf_inverter* inv, qlat_dwf_solve_stats* st)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(inv && st, "null argument");
+    *st = inv->stats;
+  });
+}
+
+int qlat_dwf_multiply_m_e_e(qlat_dwf_inverter* inv, double* out, const double* in)
+{
+  return apply_half_host(inv, H_MEE, out, in, 1);
+}
+int qlat_dwf_multiply_mdag_e_e(qlat_dwf_inverter* inv, double* out, const double* in)
+{
+  return apply_half_host(inv, H_MEE_DAG, out, in, 1);
+}
+int qlat_dwf_multiply_m_e_e_inv(qlat_dwf_inverter* inv, double* out, const double* in)
+{
+  return apply_half_host(inv, H_MEE_INV, out, in, 1);
+}
+int qlat_dwf_multiply_mdag_e_e_inv(qlat_dwf_inverter* inv, double* out, const double* in)
+{
+  return apply_half_host(inv, H_MEE_INVDAG, out, in, 1);
+}
+int qlat_dwf_multiply_wilson_d_e_o(qlat_dwf_inverter* inv, double* out, const double* in, int eo_in)
+{
+  return apply_half_host(inv, H_HOP, out, in, eo_in);
+}
+int qlat_dwf_multiply_wilson_ddag_e_o(qlat_dwf_inverter* inv, double* out, const double* in, int eo_in)
+{
+  return apply_half_host(inv, H_HOP_DAG, out, in, eo_in);
+}
+int qlat_dwf_multiply_m_e_o(qlat_dwf_inverter* inv, double* out, const double* in, int eo_in)
+{
+  return apply_half_host(inv, H_MEO, out, in, eo_in);
+}
+int qlat_dwf_multiply_mdag_e_o(qlat_dwf_inverter* inv, double* out, const double* in, int eo_in)
+{
+  return apply_half_host(inv, H_MDAG_EO, out, in, eo_in);
+}
+int qlat_dwf_multiply_mpc_sym2(qlat_dwf_inverter* inv, double* out, const double* in, int eo)
+{
+  return apply_half_host(inv, H_MPC, out, in, eo);
+}
+int qlat_dwf_multiply_mpcdag_sym2(qlat_dwf_inverter* inv, double* out, const double* in, int eo)
+{
+  return apply_half_host(inv, H_MPCDAG, out, in, eo);
+}
+int qlat_dwf_multiply_hermop_sym2(qlat_dwf_inverter* inv, double* out, const double* in, int eo)
+{
+  return apply_half_host(inv, H_HERMOP, out, in, eo);
+}
+int qlat_dwf_multiply_m(qlat_dwf_inverter* inv, double* out, const double* in)
+{
+  return apply_full_host(inv, F_M, out, in);
+}
+int qlat_dwf_multiply_mdag(qlat_dwf_inverter* inv, double* out, const double* in)
+{
+  return apply_full_host(inv, F_MDAG, out, in);
+}
+int qlat_dwf_multiply_d_minus(qlat_dwf_inverter* inv, double* out, const double* in)
+{
+  return apply_full_host(inv, F_DMINUS, out, in);
+}
+
+int qlat_dwf_dot_product(qlat_dwf_inverter* inv, const double* a, const double* b, long n_wv, double* re,
+                         double* im)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(a && b && re && im && n_wv >= 0, "invalid argument");
+    const size_t bytes = (size_t)n_wv * 12 * sizeof(double2);
+    QDWF_REQUIRE(bytes <= stage_size(*inv), "dot_product: vector larger than a 5D full field");
+    const double2* da = upload(*inv, 0, a, bytes);
+    const double2* db = upload(*inv, 1, b, bytes);
+    const cd d = dev_dot<double2>(*inv, da, db, (int64_t)n_wv * 12);
+    *re = d.real();
+    *im = d.imag();
+  });
+}
+
+int qlat_dwf_qnorm(qlat_dwf_inverter* inv, const double* a, long n_wv, double* result)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(a && result && n_wv >= 0, "invalid argument");
+    const size_t bytes = (size_t)n_wv * 12 * sizeof(double2);
+    QDWF_REQUIRE(bytes <= stage_size(*inv), "qnorm: vector larger than a 5D full field");
+    const double2* da = upload(*inv, 0, a, bytes);
+    qdwf::launch_norm2<double2>(da, (int64_t)n_wv * 12, inv->part, inv->red, inv->stream);
+    check_launch();
+    QDWF_HIP(hipMemcpyAsync(inv->red_host, inv->red, sizeof(double), hipMemcpyDeviceToHost, inv->stream));
+    sync(*inv);
+    *result = inv->red_host[0];
+  });
+}
+
+int qlat_dwf_cg_with_herm_sym_2(qlat_dwf_inverter* inv, double* sol, const double* src, double stop_rsd,
+                                long max_num_iter, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(sol && src && iter && max_num_iter >= 0, "invalid argument");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = half_aos_bytes(*inv);
+    const double2* d_src = upload(*inv, 0, src, bytes);
+    double2* b = W(*inv, W_IN_OP);
+    double2* x = W(*inv, W_OUT_OP);
+    qdwf::launch_pack5(d_src, b, inv->geo, inv->ls, 1, 0, inv->stream);
+    const double2* d_sol = upload(*inv, 1, sol, bytes);
+    qdwf::launch_pack5(d_sol, x, inv->geo, inv->ls, 1, 0, inv->stream);
+    check_launch();
+    *iter = cg_fp64(*inv, x, b, stop_rsd, max_num_iter, false);
+    inv->stats.total_iter = *iter;
+    double2* d_out = stage(*inv, 1, bytes);
+    qdwf::launch_unpack5(x, d_out, inv->geo, inv->ls, 1, 0, inv->stream);
+    check_launch();
+    download(*inv, sol, d_out, bytes);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_invert_5d(qlat_dwf_inverter* inv, double* out, const double* in, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(out && in && iter, "invalid argument");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = full_aos_bytes(*inv);
+    const double2* d_in = upload(*inv, 0, in, bytes);
+    double2* d_out = stage(*inv, 1, bytes);
+    *iter = invert_5d_dev(*inv, d_out, d_in);
+    download(*inv, out, d_out, bytes);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_invert_4d(qlat_dwf_inverter* inv, double* out, const double* in, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(out && in && iter, "invalid argument");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = (size_t)inv->geo.vh * 2 * 12 * sizeof(double2);
+    const double2* d_in = upload(*inv, 0, in, bytes);
+    double2* d_out = stage(*inv, 1, bytes);
+    *iter = invert_prop_dev(*inv, d_out, d_in, false);
+    download(*inv, out, d_out, bytes);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_invert_prop(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(sol && src && iter, "invalid argument");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = prop_bytes(*inv);
+    const double2* d_in = upload(*inv, 0, src, bytes);
+    double2* d_out = stage(*inv, 1, bytes);
+    *iter = invert_prop_dev(*inv, d_out, d_in, true);
+    download(*inv, sol, d_out, bytes);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,
+                                              double* result)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(start && result && max_iter >= 0, "invalid argument");
+    const size_t bytes = half_aos_bytes(*inv);
+    const double2* d_in = upload(*inv, 0, start, bytes);
+    double2* ff = W(*inv, W_IN_OP);
+    qdwf::launch_pack5(d_in, ff, inv->geo, inv->ls, 1, 0, inv->stream);
+    check_launch();
+    const int64_t n = n64(*inv);
+    // dslash.cpp:1349-1369: normalise, then repeatedly apply and renormalise
+    const double s = 1.0 / std::sqrt(dev_norm2(*inv, ff));
+    double ratio = 1.0;
+    double2* tmp = W(*inv, W_TMP);  // zero vector: ff = 0 + scale * ff
+    zero64(*inv, tmp);
+    qdwf::launch_xpay<double2>(tmp, s, ff, n, inv->stream);
+    check_launch();
+    for (long i = 0; i < max_iter; ++i) {
+      op_hermop<double>(*inv, ff, ff, 1);
+      ratio = std::sqrt(dev_norm2(*inv, ff));
+      qdwf::launch_xpay<double2>(tmp, 1.0 / ratio, ff, n, inv->stream);
+      check_launch();
+    }
+    sync(*inv);
+    *result = ratio;
+  });
+}
+
+int qlat_dwf_multiply_hermop_sym2_device(qlat_dwf_inverter* inv, double* out, const double* in, int eo)
+{
+  return guarded([&] {
+    check_inv(inv);
+    check_eo(eo);
+    QDWF_REQUIRE(out && in, "null field pointer");
+    double2* a = W(*inv, W_IN_E);
+    double2* b = W(*inv, W_OUT_E);
+    qdwf::launch_pack5((const double2*)in, a, inv->geo, inv->ls, eo, 0, inv->stream);
+    check_launch();
+    op_hermop<double>(*inv, b, a, eo);
+    qdwf::launch_unpack5(b, (double2*)out, inv->geo, inv->ls, eo, 0, inv->stream);
+    check_launch();
+    sync(*inv);
+  });
+}
+
+int qlat_dwf_invert_5d_device(qlat_dwf_inverter* inv, double* out, const double* in, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(out && in && iter, "invalid argument");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    *iter = invert_5d_dev(*inv, (double2*)out, (const double2*)in);
+    sync(*inv);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(sol && src && iter, "invalid argument");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    *iter = invert_prop_dev(*inv, (double2*)sol, (const double2*)src, true);
+    sync(*inv);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int iters, double* avg_ms)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(avg_ms && iters > 0, "invalid argument");
+    QDWF_REQUIRE(precision == 64 || precision == 32, "precision must be 64 or 32");
+    QDWF_REQUIRE(which >= 0 && which <= 3, "which must be 0..3");
+    Inv& I = *inv;
+    auto run = [&]() {
+      if (precision == 64) {
+        double2* a = W(I, W_IN_E);
+        double2* b = W(I, W_OUT_E);
+        if (which == 0) hop<double>(I, b, a, 1, false);
+        else if (which == 1) hop<double>(I, b, a, 1, true);
+        else if (which == 2) m5<double>(I, b, a, MAT_MEE_INV);
+        else op_hermop<double>(I, b, a, 1);
+      } else {
+        float4* a = V(I, V_R);
+        float4* b = V(I, V_AP);
+        if (which == 0) hop<float>(I, b, a, 1, false);
+        else if (which == 1) hop<float>(I, b, a, 1, true);
+        else if (which == 2) m5<float>(I, b, a, MAT_MEE_INV);
+        else op_hermop<float>(I, b, a, 1);
+      }
+    };
+    // finite, non-trivial input values
+    if (precision == 64) QDWF_HIP(hipMemsetAsync(W(I, W_IN_E), 0x3f, field64_bytes(I), I.stream));
+    else QDWF_HIP(hipMemsetAsync(V(I, V_R), 0x3f, field32_bytes(I), I.stream));
+    run();
+    hipEvent_t e0, e1;
+    QDWF_HIP(hipEventCreate(&e0));
+    QDWF_HIP(hipEventCreate(&e1));
+    QDWF_HIP(hipEventRecord(e0, I.stream));
+    for (int i = 0; i < iters; ++i) run();
+    QDWF_HIP(hipEventRecord(e1, I.stream));
+    QDWF_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    QDWF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / iters;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// Internal interface between the kernel file and the C-ABI host code.
+#pragma once
+
+#include "dwf_common.h"
+
+namespace qdwf {
+
+constexpr int kHopBlock = 256;
+constexpr int kM5Block = 256;
+constexpr int kBlasBlock = 256;
+constexpr int kMaxBlasGrid = 2048;
+
+template <class T>
+struct HopArgs {
+  LatGeo geo;
+  int ls;
+  int eo_out;          // parity of the output half field (2 even, 1 odd)
+  int nchunk;          // set by the launcher
+  T alpha;             // out = alpha * H in (+ beta_s * x)
+  const void* in;      // SoA spinor, parity 3 - eo_out
+  void* out;           // SoA spinor, parity eo_out
+  const void* x;       // optional xpay operand (parity eo_out) or nullptr
+  const cplx<T>* beta; // per-s xpay coefficients (device), used with x
+  const void* links;   // 8-link layout (dwf_common.h)
+};
+
+template <class T>
+struct M5Args {
+  LatGeo geo;
+  int ls;
+  T alpha;            // out = alpha * Mat in + beta * x
+  T beta;
+  const cplx<T>* mat; // [2][ls][ls] (device)
+  const void* in;
+  void* out;
+  const void* x;      // optional
+};
+
+template <class T>
+void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st);
+template <class T>
+void launch_m5(const M5Args<T>& a, hipStream_t st);
+
+template <class S>
+void launch_norm2(const S* a, int64_t n, double* part, double* out, hipStream_t st);
+template <class S>
+void launch_dot(const S* a, const S* b, int64_t n, double* part, double* out, hipStream_t st);
+template <class S>
+void launch_axpy(double a, const S* x, S* y, int64_t n, hipStream_t st);
+template <class S>
+void launch_xpay(const S* x, double a, S* y, int64_t n, hipStream_t st);
+template <class S>
+void launch_cg_update(double a, const S* p, const S* ap, S* x, S* r, int64_t n, double* part, double* out,
+                      hipStream_t st);
+
+void launch_d2f(const double2* in, float4* out, int ls, int64_t st, hipStream_t s);
+void launch_f2d(const float4* in, double2* out, int ls, int64_t st, int acc, hipStream_t s);
+
+void launch_pack5(const double2* aos, double2* soa, const LatGeo& g, int ls, int eo, int full, hipStream_t st);
+void launch_unpack5(const double2* soa, double2* aos, const LatGeo& g, int ls, int eo, int full, hipStream_t st);
+void launch_prop_col_to_5d(const double2* src, double2* soa, const LatGeo& g, int ls, int eo, int col,
+                           hipStream_t st);
+void launch_5d_to_prop_col(const double2* soa, double2* dst, const LatGeo& g, int ls, int eo, int col,
+                           hipStream_t st);
+void launch_pack_gauge(const double2* gauge, double2* l64, float2* l32, const LatGeo& g, hipStream_t st);
+
+}  // namespace qdwf

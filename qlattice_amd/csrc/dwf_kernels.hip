@@ -1,0 +1,592 @@
+// gfx950 kernels of the domain-wall library: Wilson hopping term, dense
+// fifth-dimension (s) matrices, BLAS-1 with block reductions, and layout
+// conversion between the reference's array-of-structures complex128 fields
+// and the device structure-of-arrays layout (dwf_common.h).
+#include "dwf_internal.h"
+
+namespace qdwf {
+
+// ================================================================== hop
+// out(h,s) = alpha * (H in)(h,s) + beta_s * x(h,s)
+// (H psi)(x) = -1/2 sum_mu [ U_mu(x)(1 -+ g_mu) psi(x+mu)
+//                          + U_mu(x-mu)^+ (1 +- g_mu) psi(x-mu) ]
+// upper signs: DAG=false (multiply_wilson_d_e_o_no_comm, dslash.cpp:859-908),
+// lower signs: DAG=true (multiply_wilson_ddag_e_o_no_comm, dslash.cpp:910-959).
+// One thread = one output half-site x SPT consecutive s values; each link is
+// loaded once per thread and reused across its s values.
+template <class T, int SPT, bool DAG, bool XPAY>
+__global__ void __launch_bounds__(kHopBlock) k_hop(HopArgs<T> a)
+{
+  typedef typename SpinorIO<T>::slot_t slot_t;
+  const int64_t nblk = (int64_t)gridDim.x;
+  const int64_t lb = xcd_logical_block(blockIdx.x, nblk);
+  const int nchunk = a.nchunk;
+  const int64_t hblk = lb / nchunk;
+  const int chunk = (int)(lb - hblk * nchunk);
+  const int64_t h = hblk * kHopBlock + threadIdx.x;
+  if (h >= a.geo.vh) return;
+  const int s0 = chunk * SPT;
+  int x[4];
+  half_coord(a.geo, h, a.eo_out, x);
+  int64_t nb[8];
+  neighbours(a.geo, x, nb);
+  const int pslot = (a.eo_out == 2) ? 0 : 1;
+  const slot_t* __restrict__ in = (const slot_t*)a.in;
+  const typename LinkIO<T>::elem_t* __restrict__ links = (const typename LinkIO<T>::elem_t*)a.links;
+
+  cplx<T> acc[SPT][12];
+#pragma unroll
+  for (int j = 0; j < SPT; ++j)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[j][k] = cmk<T>(0, 0);
+
+  // forward: sign sg = -1 (D) / +1 (D^+); backward: opposite.
+  constexpr int SF = DAG ? 1 : -1;
+  constexpr int SB = -SF;
+#define QDWF_HOP_DIR(D, MU, SG)                                          \
+  {                                                                      \
+    cplx<T> W[9];                                                        \
+    LinkIO<T>::load(links, a.geo.stride, pslot * 8 + (D), h, W);         \
+    _Pragma("unroll") for (int j = 0; j < SPT; ++j)                      \
+    {                                                                    \
+      if (SPT == 1 || s0 + j < a.ls) {                                   \
+        cplx<T> psi[12];                                                 \
+        SpinorIO<T>::load(in, a.geo.stride, s0 + j, nb[D], psi);         \
+        hop_term<MU, SG>(W, psi, acc[j]);                                \
+      }                                                                  \
+    }                                                                    \
+  }
+  QDWF_HOP_DIR(0, 0, SF)
+  QDWF_HOP_DIR(1, 1, SF)
+  QDWF_HOP_DIR(2, 2, SF)
+  QDWF_HOP_DIR(3, 3, SF)
+  QDWF_HOP_DIR(4, 0, SB)
+  QDWF_HOP_DIR(5, 1, SB)
+  QDWF_HOP_DIR(6, 2, SB)
+  QDWF_HOP_DIR(7, 3, SB)
+#undef QDWF_HOP_DIR
+
+  slot_t* __restrict__ out = (slot_t*)a.out;
+  const T scale = (T)(-0.5) * a.alpha;
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) {
+    const int s = s0 + j;
+    if (SPT > 1 && s >= a.ls) break;
+    cplx<T> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cmk<T>(scale * acc[j][k].re, scale * acc[j][k].im);
+    if (XPAY) {
+      cplx<T> xv[12];
+      SpinorIO<T>::load((const slot_t*)a.x, a.geo.stride, s, h, xv);
+      const cplx<T> b = a.beta[s];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v[k] = cfma(v[k], b, xv[k]);
+    }
+    SpinorIO<T>::store(out, a.geo.stride, s, h, v);
+  }
+}
+
+template <class T, int SPT>
+static void launch_hop_spt(const HopArgs<T>& a0, bool dag, hipStream_t st)
+{
+  HopArgs<T> a = a0;
+  a.nchunk = (a.ls + SPT - 1) / SPT;
+  const int64_t hblocks = (a.geo.vh + kHopBlock - 1) / kHopBlock;
+  const dim3 grid((unsigned)(hblocks * a.nchunk));
+  const bool xpay = a.x != nullptr;
+  if (!dag && !xpay) hipLaunchKernelGGL((k_hop<T, SPT, false, false>), grid, dim3(kHopBlock), 0, st, a);
+  if (!dag && xpay) hipLaunchKernelGGL((k_hop<T, SPT, false, true>), grid, dim3(kHopBlock), 0, st, a);
+  if (dag && !xpay) hipLaunchKernelGGL((k_hop<T, SPT, true, false>), grid, dim3(kHopBlock), 0, st, a);
+  if (dag && xpay) hipLaunchKernelGGL((k_hop<T, SPT, true, true>), grid, dim3(kHopBlock), 0, st, a);
+}
+
+template <class T>
+void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st)
+{
+  switch (spt) {
+    case 1: launch_hop_spt<T, 1>(a, dag, st); break;
+    case 2: launch_hop_spt<T, 2>(a, dag, st); break;
+    case 3: launch_hop_spt<T, 3>(a, dag, st); break;
+    case 4: launch_hop_spt<T, 4>(a, dag, st); break;
+    case 6: launch_hop_spt<T, 6>(a, dag, st); break;
+    default: launch_hop_spt<T, 2>(a, dag, st); break;
+  }
+}
+
+template void launch_hop<double>(const HopArgs<double>&, bool, int, hipStream_t);
+template void launch_hop<float>(const HopArgs<float>&, bool, int, hipStream_t);
+
+// ================================================================== M5
+// out = alpha * (Mat_chi in) + beta * x, where Mat_chi is a dense Ls x Ls
+// complex matrix per chirality (chi = 0: spins 0,1; chi = 1: spins 2,3).
+// Every fifth-dimension operator of the path is of this form: M_ee, M_ee^+,
+// M_ee^-1, M_ee^-+ (dslash.cpp:591-857), the Moebius pre-factor b + cS of
+// M_eo (dslash.cpp:961-1003) and its adjoint (dslash.cpp:1005-1060), and
+// their products.  Thread = (half-site, 16-byte slot); the slot index is
+// blockIdx.y so chirality and matrix reads are wave-uniform.
+template <class T, int LSMAX, bool XPAY>
+__global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
+{
+  typedef typename SpinorIO<T>::slot_t slot_t;
+  const int64_t h = (int64_t)blockIdx.x * kM5Block + threadIdx.x;
+  if (h >= a.geo.vh) return;
+  const int slot = blockIdx.y;
+  constexpr int NS = SpinorIO<T>::kSlots;
+  const int chi = (slot < NS / 2) ? 0 : 1;
+  const int ls = a.ls;
+  const cplx<T>* __restrict__ M = a.mat + (int64_t)chi * ls * ls;
+  const slot_t* __restrict__ in = (const slot_t*)a.in;
+  slot_t* __restrict__ out = (slot_t*)a.out;
+  const int64_t st = a.geo.stride;
+  // a slot holds one (FP64) or two (FP32) complex numbers
+  constexpr int NC = (NS == 12) ? 1 : 2;
+  cplx<T> v[LSMAX][NC];
+#pragma unroll
+  for (int t = 0; t < LSMAX; ++t) {
+    if (t < ls) {
+      const slot_t q = in[((int64_t)t * NS + slot) * st + h];
+      if constexpr (NC == 1) {
+        v[t][0] = cmk<T>(q.x, q.y);
+      } else {
+        v[t][0] = cmk<T>(q.x, q.y);
+        v[t][1] = cmk<T>(q.z, q.w);
+      }
+    }
+  }
+  for (int s = 0; s < ls; ++s) {
+    cplx<T> r[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) r[c] = cmk<T>(0, 0);
+#pragma unroll
+    for (int t = 0; t < LSMAX; ++t) {
+      if (t < ls) {
+        const cplx<T> m = M[s * ls + t];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) r[c] = cfma(r[c], m, v[t][c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) r[c] = cmk<T>(a.alpha * r[c].re, a.alpha * r[c].im);
+    const int64_t off = ((int64_t)s * NS + slot) * st + h;
+    if (XPAY) {
+      const slot_t q = ((const slot_t*)a.x)[off];
+      if constexpr (NC == 1) {
+        r[0].re += a.beta * q.x;
+        r[0].im += a.beta * q.y;
+      } else {
+        r[0].re += a.beta * q.x;
+        r[0].im += a.beta * q.y;
+        r[1].re += a.beta * q.z;
+        r[1].im += a.beta * q.w;
+      }
+    }
+    slot_t o;
+    if constexpr (NC == 1) {
+      o.x = r[0].re;
+      o.y = r[0].im;
+    } else {
+      o.x = r[0].re;
+      o.y = r[0].im;
+      o.z = r[1].re;
+      o.w = r[1].im;
+    }
+    out[off] = o;
+  }
+}
+
+template <class T, int LSMAX>
+static void launch_m5_ls(const M5Args<T>& a, hipStream_t st)
+{
+  const dim3 grid((unsigned)((a.geo.vh + kM5Block - 1) / kM5Block), SpinorIO<T>::kSlots);
+  if (a.x) hipLaunchKernelGGL((k_m5<T, LSMAX, true>), grid, dim3(kM5Block), 0, st, a);
+  else hipLaunchKernelGGL((k_m5<T, LSMAX, false>), grid, dim3(kM5Block), 0, st, a);
+}
+
+template <class T>
+void launch_m5(const M5Args<T>& a, hipStream_t st)
+{
+  if (a.ls <= 8) launch_m5_ls<T, 8>(a, st);
+  else if (a.ls <= 12) launch_m5_ls<T, 12>(a, st);
+  else if (a.ls <= 16) launch_m5_ls<T, 16>(a, st);
+  else if (a.ls <= 24) launch_m5_ls<T, 24>(a, st);
+  else launch_m5_ls<T, 32>(a, st);
+}
+
+template void launch_m5<double>(const M5Args<double>&, hipStream_t);
+template void launch_m5<float>(const M5Args<float>&, hipStream_t);
+
+// ================================================================ BLAS-1
+// All vectors are flat arrays of 16-byte slots (n slots); pads are zero.
+
+__device__ __forceinline__ double warp_sum(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Block reduction of NV doubles; thread 0 writes them to part[blockIdx.x*NV+i].
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double v[NV], double* part)
+{
+  __shared__ double sh[kBlasBlock / 64][NV];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = warp_sum(v[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sh[wid][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double t = 0;
+      for (int w = 0; w < kBlasBlock / 64; ++w) t += sh[w][i];
+      part[(int64_t)blockIdx.x * NV + i] = t;
+    }
+  }
+}
+
+template <class S>
+__device__ __forceinline__ double slot_norm2(const S& a);
+template <>
+__device__ __forceinline__ double slot_norm2<double2>(const double2& a)
+{
+  return a.x * a.x + a.y * a.y;
+}
+template <>
+__device__ __forceinline__ double slot_norm2<float4>(const float4& a)
+{
+  return (double)a.x * a.x + (double)a.y * a.y + (double)a.z * a.z + (double)a.w * a.w;
+}
+// conj(a) b: (re, im)
+__device__ __forceinline__ void slot_dot(const double2& a, const double2& b, double& re, double& im)
+{
+  re += a.x * b.x + a.y * b.y;
+  im += a.x * b.y - a.y * b.x;
+}
+__device__ __forceinline__ void slot_dot(const float4& a, const float4& b, double& re, double& im)
+{
+  re += (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
+  im += (double)a.x * b.y - (double)a.y * b.x + (double)a.z * b.w - (double)a.w * b.z;
+}
+
+__device__ __forceinline__ double2 slot_axpy(double a, double2 x, double2 y)
+{
+  return make_double2(y.x + a * x.x, y.y + a * x.y);
+}
+__device__ __forceinline__ float4 slot_axpy(double a, float4 x, float4 y)
+{
+  const float f = (float)a;
+  return make_float4(y.x + f * x.x, y.y + f * x.y, y.z + f * x.z, y.w + f * x.w);
+}
+// x + a*y
+__device__ __forceinline__ double2 slot_xpay(double2 x, double a, double2 y)
+{
+  return make_double2(a * y.x + x.x, a * y.y + x.y);
+}
+__device__ __forceinline__ float4 slot_xpay(float4 x, double a, float4 y)
+{
+  const float f = (float)a;
+  return make_float4(f * y.x + x.x, f * y.y + x.y, f * y.z + x.z, f * y.w + x.w);
+}
+
+template <class S>
+__global__ void __launch_bounds__(kBlasBlock) k_norm2(const S* __restrict__ a, int64_t n, double* part)
+{
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock)
+    v[0] += slot_norm2(a[i]);
+  block_reduce_store<1>(v, part);
+}
+
+template <class S>
+__global__ void __launch_bounds__(kBlasBlock) k_dot(const S* __restrict__ a, const S* __restrict__ b, int64_t n,
+                                                    double* part)
+{
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock)
+    slot_dot(a[i], b[i], v[0], v[1]);
+  block_reduce_store<2>(v, part);
+}
+
+// y = a*x + y
+template <class S>
+__global__ void __launch_bounds__(kBlasBlock) k_axpy(double a, const S* __restrict__ x, S* __restrict__ y, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock)
+    y[i] = slot_axpy(a, x[i], y[i]);
+}
+
+// y = x + a*y
+template <class S>
+__global__ void __launch_bounds__(kBlasBlock) k_xpay(const S* __restrict__ x, double a, S* __restrict__ y, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock)
+    y[i] = slot_xpay(x[i], a, y[i]);
+}
+
+// x += a p ; r -= a ap ; partial |r|^2
+template <class S>
+__global__ void __launch_bounds__(kBlasBlock) k_cg_update(double a, const S* __restrict__ p, const S* __restrict__ ap,
+                                                          S* __restrict__ x, S* __restrict__ r, int64_t n,
+                                                          double* part)
+{
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
+    x[i] = slot_axpy(a, p[i], x[i]);
+    const S rn = slot_axpy(-a, ap[i], r[i]);
+    r[i] = rn;
+    v[0] += slot_norm2(rn);
+  }
+  block_reduce_store<1>(v, part);
+}
+
+__global__ void k_reduce_final(const double* __restrict__ part, int nblk, int nv, double* out)
+{
+  // one thread per value; partial sums added in block order (deterministic)
+  const int i = threadIdx.x;
+  if (i < nv) {
+    double t = 0;
+    for (int b = 0; b < nblk; ++b) t += part[(int64_t)b * nv + i];
+    out[i] = t;
+  }
+}
+
+// FP64 [s][12][st] double2  <->  FP32 [s][6][st] float4
+__global__ void __launch_bounds__(kBlasBlock) k_d2f(const double2* __restrict__ in, float4* __restrict__ out,
+                                                    int64_t nslot32, int64_t st)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < nslot32; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t h = i % st, q = i / st;  // q = s*6 + kp
+    const int64_t s = q / 6, kp = q - s * 6;
+    const double2 a = in[(s * 12 + 2 * kp) * st + h];
+    const double2 b = in[(s * 12 + 2 * kp + 1) * st + h];
+    out[i] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+  }
+}
+
+// out64 (+)= in32 : acc=0 overwrite, acc=1 add
+__global__ void __launch_bounds__(kBlasBlock) k_f2d(const float4* __restrict__ in, double2* __restrict__ out,
+                                                    int64_t nslot32, int64_t st, int acc)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < nslot32; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t h = i % st, q = i / st;
+    const int64_t s = q / 6, kp = q - s * 6;
+    const float4 a = in[i];
+    double2* o0 = &out[(s * 12 + 2 * kp) * st + h];
+    double2* o1 = &out[(s * 12 + 2 * kp + 1) * st + h];
+    if (acc) {
+      const double2 p0 = *o0, p1 = *o1;
+      *o0 = make_double2(p0.x + a.x, p0.y + a.y);
+      *o1 = make_double2(p1.x + a.z, p1.y + a.w);
+    } else {
+      *o0 = make_double2(a.x, a.y);
+      *o1 = make_double2(a.z, a.w);
+    }
+  }
+}
+
+static int blas_grid(int64_t n)
+{
+  int64_t g = (n + kBlasBlock - 1) / kBlasBlock;
+  if (g > kMaxBlasGrid) g = kMaxBlasGrid;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <class S>
+void launch_norm2(const S* a, int64_t n, double* part, double* out, hipStream_t st)
+{
+  const int g = blas_grid(n);
+  hipLaunchKernelGGL(k_norm2<S>, dim3(g), dim3(kBlasBlock), 0, st, a, n, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, part, g, 1, out);
+}
+template <class S>
+void launch_dot(const S* a, const S* b, int64_t n, double* part, double* out, hipStream_t st)
+{
+  const int g = blas_grid(n);
+  hipLaunchKernelGGL(k_dot<S>, dim3(g), dim3(kBlasBlock), 0, st, a, b, n, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, part, g, 2, out);
+}
+template <class S>
+void launch_axpy(double a, const S* x, S* y, int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_axpy<S>, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, a, x, y, n);
+}
+template <class S>
+void launch_xpay(const S* x, double a, S* y, int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_xpay<S>, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, x, a, y, n);
+}
+template <class S>
+void launch_cg_update(double a, const S* p, const S* ap, S* x, S* r, int64_t n, double* part, double* out,
+                      hipStream_t st)
+{
+  const int g = blas_grid(n);
+  hipLaunchKernelGGL(k_cg_update<S>, dim3(g), dim3(kBlasBlock), 0, st, a, p, ap, x, r, n, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, part, g, 1, out);
+}
+
+template void launch_norm2<double2>(const double2*, int64_t, double*, double*, hipStream_t);
+template void launch_norm2<float4>(const float4*, int64_t, double*, double*, hipStream_t);
+template void launch_dot<double2>(const double2*, const double2*, int64_t, double*, double*, hipStream_t);
+template void launch_dot<float4>(const float4*, const float4*, int64_t, double*, double*, hipStream_t);
+template void launch_axpy<double2>(double, const double2*, double2*, int64_t, hipStream_t);
+template void launch_axpy<float4>(double, const float4*, float4*, int64_t, hipStream_t);
+template void launch_xpay<double2>(const double2*, double, double2*, int64_t, hipStream_t);
+template void launch_xpay<float4>(const float4*, double, float4*, int64_t, hipStream_t);
+template void launch_cg_update<double2>(double, const double2*, const double2*, double2*, double2*, int64_t,
+                                        double*, double*, hipStream_t);
+template void launch_cg_update<float4>(double, const float4*, const float4*, float4*, float4*, int64_t, double*,
+                                       double*, hipStream_t);
+
+void launch_d2f(const double2* in, float4* out, int ls, int64_t st, hipStream_t s)
+{
+  const int64_t n = (int64_t)ls * 6 * st;
+  hipLaunchKernelGGL(k_d2f, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n, st);
+}
+void launch_f2d(const float4* in, double2* out, int ls, int64_t st, int acc, hipStream_t s)
+{
+  const int64_t n = (int64_t)ls * 6 * st;
+  hipLaunchKernelGGL(k_f2d, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n, st, acc);
+}
+
+// ============================================================ layouts
+// Reference 5D field AoS complex128 [site][s][12] (full, V sites) or
+// [half][s][12] (half field).  `full` selects which: for a full field the
+// half-site h of parity eo sits at full index of half_coord(h, eo).
+__global__ void __launch_bounds__(kBlasBlock) k_pack5(const double2* __restrict__ aos, double2* __restrict__ soa,
+                                                      LatGeo g, int ls, int eo, int full)
+{
+  const int64_t n = g.vh * ls * 12;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
+    // i enumerates (s, k, h) with h fastest
+    const int64_t h = i % g.vh, q = i / g.vh;
+    const int64_t s = q / 12, k = q - s * 12;
+    int64_t site = h;
+    if (full) {
+      int x[4];
+      half_coord(g, h, eo, x);
+      site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
+    }
+    soa[(s * 12 + k) * g.stride + h] = aos[(site * ls + s) * 12 + k];
+  }
+}
+
+__global__ void __launch_bounds__(kBlasBlock) k_unpack5(const double2* __restrict__ soa, double2* __restrict__ aos,
+                                                        LatGeo g, int ls, int eo, int full)
+{
+  const int64_t n = g.vh * ls * 12;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t h = i % g.vh, q = i / g.vh;
+    const int64_t s = q / 12, k = q - s * 12;
+    int64_t site = h;
+    if (full) {
+      int x[4];
+      half_coord(g, h, eo, x);
+      site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
+    }
+    aos[(site * ls + s) * 12 + k] = soa[(s * 12 + k) * g.stride + h];
+  }
+}
+
+void launch_pack5(const double2* aos, double2* soa, const LatGeo& g, int ls, int eo, int full, hipStream_t st)
+{
+  const int64_t n = g.vh * ls * 12;
+  hipLaunchKernelGGL(k_pack5, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, aos, soa, g, ls, eo, full);
+}
+void launch_unpack5(const double2* soa, double2* aos, const LatGeo& g, int ls, int eo, int full, hipStream_t st)
+{
+  const int64_t n = g.vh * ls * 12;
+  hipLaunchKernelGGL(k_unpack5, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, soa, aos, g, ls, eo, full);
+}
+
+// Propagator column j ([site][12 rows][12 cols] complex128) -> 5D half field
+// of parity eo: the chiral embedding of fermion_field_5d_from_4d
+// (qcd-prop.h:96-116) with upper = 0, lower = Ls-1 (qcd-prop.h:157):
+// spins 0,1 go to s = 0, spins 2,3 to s = Ls-1, everything else is zero.
+// With col < 0 the input is a 4D fermion field [site][12].
+__global__ void __launch_bounds__(kBlasBlock) k_prop_col_to_5d(const double2* __restrict__ src, double2* __restrict__ soa,
+                                                               LatGeo g, int ls, int eo, int col)
+{
+  const int64_t n = g.vh * ls * 12;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t h = i % g.vh, q = i / g.vh;
+    const int s = (int)(q / 12), k = (int)(q - (int64_t)(q / 12) * 12);
+    double2 v = make_double2(0.0, 0.0);
+    const bool take = (k < 6) ? (s == 0) : (s == ls - 1);
+    if (take) {
+      int x[4];
+      half_coord(g, h, eo, x);
+      const int64_t site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
+      v = (col >= 0) ? src[(site * 12 + k) * 12 + col] : src[site * 12 + k];
+    }
+    soa[((int64_t)s * 12 + k) * g.stride + h] = v;
+  }
+}
+
+// 5D half field of parity eo -> propagator column j (fermion_field_4d_from_5d,
+// qcd-prop.h:118-138, upper = Ls-1, lower = 0, qcd-prop.h:164).
+__global__ void __launch_bounds__(kBlasBlock) k_5d_to_prop_col(const double2* __restrict__ soa, double2* __restrict__ dst,
+                                                               LatGeo g, int ls, int eo, int col)
+{
+  const int64_t n = g.vh * 12;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t h = i % g.vh;
+    const int k = (int)(i / g.vh);
+    const int s = (k < 6) ? ls - 1 : 0;
+    int x[4];
+    half_coord(g, h, eo, x);
+    const int64_t site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
+    const double2 v = soa[((int64_t)s * 12 + k) * g.stride + h];
+    if (col >= 0) dst[(site * 12 + k) * 12 + col] = v;
+    else dst[site * 12 + k] = v;
+  }
+}
+
+void launch_prop_col_to_5d(const double2* src, double2* soa, const LatGeo& g, int ls, int eo, int col, hipStream_t st)
+{
+  const int64_t n = g.vh * ls * 12;
+  hipLaunchKernelGGL(k_prop_col_to_5d, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, src, soa, g, ls, eo, col);
+}
+void launch_5d_to_prop_col(const double2* soa, double2* dst, const LatGeo& g, int ls, int eo, int col, hipStream_t st)
+{
+  const int64_t n = g.vh * 12;
+  hipLaunchKernelGGL(k_5d_to_prop_col, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, soa, dst, g, ls, eo, col);
+}
+
+// Gauge AoS [site][mu][3][3] complex128 -> per-output-parity 8-link layout
+// (dwf_common.h), FP64 and FP32 copies.  Backward links are stored already
+// adjointed: U_mu(x - mu)^+.
+__global__ void __launch_bounds__(kBlasBlock) k_pack_gauge(const double2* __restrict__ gauge, double2* __restrict__ l64,
+                                                           float2* __restrict__ l32, LatGeo g)
+{
+  const int64_t n = g.vh * 2 * 8;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t h = i % g.vh, q = i / g.vh;
+    const int p = (int)(q / 8), d = (int)(q % 8);
+    const int eo = (p == 0) ? 2 : 1;
+    int x[4];
+    half_coord(g, h, eo, x);
+    const int mu = d & 3;
+    if (d >= 4) x[mu] = wrap_dec(x[mu], g.n[mu]);
+    const int64_t site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
+    const double2* U = &gauge[(site * 4 + mu) * 9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double2 v = (d < 4) ? U[r * 3 + c] : make_double2(U[c * 3 + r].x, -U[c * 3 + r].y);
+        const int64_t off = (((int64_t)p * 8 + d) * 9 + r * 3 + c) * g.stride + h;
+        l64[off] = v;
+        l32[off] = make_float2((float)v.x, (float)v.y);
+      }
+  }
+}
+
+void launch_pack_gauge(const double2* gauge, double2* l64, float2* l32, const LatGeo& g, hipStream_t st)
+{
+  const int64_t n = g.vh * 16;
+  hipLaunchKernelGGL(k_pack_gauge, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, gauge, l64, l32, g);
+}
+
+}  // namespace qdwf

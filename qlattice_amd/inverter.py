@@ -1,0 +1,341 @@
+"""Domain-wall inverter and operators on the MI355X, qlat-compatible surface.
+
+``InverterDomainWall`` mirrors qlat's class of the same name
+(qlat/qlat/inverter.py:75-129): construct with ``gf=`` and ``fa=``, apply
+with ``inv * prop`` (a ``Prop`` or a list of them), and get/set
+``stop_rsd``, ``max_num_iter`` and ``max_mixed_precision_cycle``.  The
+module-level ``multiply_*``, ``cg_with_herm_sym_2``, ``invert`` and
+``find_max_eigen_value_hermop_sym2`` functions take the arguments of the C++
+overloads in qlat/qlat/include/qlat/dslash.h:218-588.
+
+Everything is computed by the HIP library through the C ABI of
+include/qlat_dwf.h; this module only marshals numpy arrays.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .fermion_action import FermionAction
+from .fields import FermionField4d, FermionField5d, GaugeField, Prop
+
+SOLVERS = {"fp64": _lib.QLAT_DWF_SOLVER_FP64, "mixed": _lib.QLAT_DWF_SOLVER_MIXED_FP32}
+
+
+class Inverter:
+    pass
+
+
+class InverterDomainWall(Inverter):
+    """InverterDomainWall (dslash.h:135-197; inverter.py:75-129).
+
+    solver: "fp64" runs the reference algorithm (invert_with_cg with the
+    sym2 odd preconditioning and the restart cycle loop, dslash.h:440-530)
+    in FP64; "mixed" runs the inner CG in FP32 with FP64 reliable updates
+    (the role of a back-end's cg_with_herm_sym_2 override, grid.h:460-484).
+    """
+
+    def __init__(self, *, gf, fa, qtimer=None, solver="fp64", reliable_delta=0.1):
+        assert isinstance(gf, GaugeField)
+        assert isinstance(fa, FermionAction)
+        L = _lib.load()
+        self.geo_ = gf.geo
+        self.fa = fa
+        self.timer = qtimer
+        self._bs = np.ascontiguousarray(fa.bs, dtype=np.complex128)
+        self._cs = np.ascontiguousarray(fa.cs, dtype=np.complex128)
+        cfa = _lib.FermionActionC(
+            mass=fa.mass(), ls=fa.ls(), m5=fa.m5(), mobius_scale=fa.mobius_scale(),
+            is_multiplying_dminus=int(fa.is_multiplying_dminus),
+            is_using_zmobius=int(fa.is_using_zmobius), cg_diagonal_mee=2,
+            bs=_lib.ptr(self._bs), cs=_lib.ptr(self._cs),
+        )
+        gauge = np.ascontiguousarray(gf.data, dtype=np.complex128)
+        site = (ctypes.c_int * 4)(*gf.geo.total_site())
+        h = ctypes.c_void_p()
+        _lib.check(L.qlat_dwf_inverter_create(ctypes.byref(h), site, _lib.ptr(gauge),
+                                              ctypes.byref(cfa)),
+                   "qlat_dwf_inverter_create")
+        self.cdata = h
+        self.set_solver(solver, reliable_delta)
+
+    def __del__(self):
+        h = getattr(self, "cdata", None)
+        if h:
+            _lib.load().qlat_dwf_inverter_destroy(h)
+            self.cdata = None
+
+    # ----------------------------------------------------------- params
+    def geo(self):
+        return self.geo_
+
+    def _params(self):
+        s, n, c = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
+        _lib.check(_lib.load().qlat_dwf_inverter_get_params(self.cdata, ctypes.byref(s),
+                                                            ctypes.byref(n), ctypes.byref(c)),
+                   "qlat_dwf_inverter_get_params")
+        return s.value, n.value, c.value
+
+    def _set_params(self, s, n, c):
+        _lib.check(_lib.load().qlat_dwf_inverter_set_params(self.cdata, s, n, c),
+                   "qlat_dwf_inverter_set_params")
+
+    def stop_rsd(self):
+        return self._params()[0]
+
+    def set_stop_rsd(self, stop_rsd):
+        _, n, c = self._params()
+        self._set_params(float(stop_rsd), n, c)
+
+    def max_num_iter(self):
+        return self._params()[1]
+
+    def set_max_num_iter(self, max_num_iter):
+        s, _, c = self._params()
+        self._set_params(s, int(max_num_iter), c)
+
+    def max_mixed_precision_cycle(self):
+        return self._params()[2]
+
+    def set_max_mixed_precision_cycle(self, max_mixed_precision_cycle):
+        s, n, _ = self._params()
+        self._set_params(s, n, int(max_mixed_precision_cycle))
+
+    def set_solver(self, solver, reliable_delta=0.1):
+        if solver not in SOLVERS:
+            raise ValueError(f"solver must be one of {sorted(SOLVERS)}")
+        _lib.check(_lib.load().qlat_dwf_inverter_set_solver(self.cdata, SOLVERS[solver],
+                                                            float(reliable_delta)),
+                   "qlat_dwf_inverter_set_solver")
+        self.solver = solver
+
+    def stats(self):
+        st = _lib.SolveStatsC()
+        _lib.check(_lib.load().qlat_dwf_inverter_get_stats(self.cdata, ctypes.byref(st)),
+                   "qlat_dwf_inverter_get_stats")
+        return {k: getattr(st, k) for k, _ in _lib.SolveStatsC._fields_}
+
+    # ------------------------------------------------------------ apply
+    def __mul__(self, src):
+        """prop_src: Prop, FermionField4d, full FermionField5d, or a list."""
+        if isinstance(src, list):
+            return [self * p for p in src]
+        if self.timer is not None:
+            self.timer.start()
+        try:
+            if isinstance(src, Prop):
+                sol = Prop(self.geo_)
+                invert(sol, src, self)
+            elif isinstance(src, FermionField4d):
+                sol = FermionField4d(self.geo_)
+                invert(sol, src, self)
+            elif isinstance(src, FermionField5d):
+                sol = FermionField5d(self.geo_, self.fa.ls(), 0)
+                invert(sol, src, self)
+            else:
+                raise Exception("InverterDomainWall")
+        finally:
+            if self.timer is not None:
+                self.timer.stop()
+        return sol
+
+
+# ================================================================ operators
+
+def _half_out(in_, eo):
+    out = FermionField5d.__new__(FermionField5d)
+    out.geo, out.ls, out.eo = in_.geo, in_.ls, eo
+    out.data = np.empty_like(in_.data)
+    return out
+
+
+def _check_half(in_, inv):
+    assert isinstance(in_, FermionField5d) and in_.eo in (1, 2)
+    assert in_.data.shape == (inv.geo().total_volume() // 2, inv.fa.ls(), 12)
+
+
+def _apply_half(name, out, in_, inv, eo_arg, eo_out):
+    _check_half(in_, inv)
+    src = np.ascontiguousarray(in_.data, dtype=np.complex128)
+    res = _half_out(in_, eo_out)
+    fn = getattr(_lib.load(), name)
+    args = (inv.cdata, _lib.ptr(res.data), _lib.ptr(src))
+    if eo_arg is not None:
+        args = args + (int(eo_arg),)
+    _lib.check(fn(*args), name)
+    if out is not None:
+        out.geo, out.ls, out.eo, out.data = res.geo, res.ls, res.eo, res.data
+        return out
+    return res
+
+
+def multiply_m_e_e(out, in_, inv):
+    """dslash.cpp:591-639 (works for _o_o as well)."""
+    return _apply_half("qlat_dwf_multiply_m_e_e", out, in_, inv, None, in_.eo)
+
+
+def multiply_mdag_e_e(out, in_, inv):
+    """dslash.cpp:641-692."""
+    return _apply_half("qlat_dwf_multiply_mdag_e_e", out, in_, inv, None, in_.eo)
+
+
+def multiply_m_e_e_inv(out, in_, inv):
+    """dslash.cpp:694-769."""
+    return _apply_half("qlat_dwf_multiply_m_e_e_inv", out, in_, inv, None, in_.eo)
+
+
+def multiply_mdag_e_e_inv(out, in_, inv):
+    """dslash.cpp:771-857."""
+    return _apply_half("qlat_dwf_multiply_mdag_e_e_inv", out, in_, inv, None, in_.eo)
+
+
+def multiply_wilson_d_e_o_no_comm(out, in_, inv):
+    """dslash.cpp:859-908: output parity 3 - in.eo."""
+    return _apply_half("qlat_dwf_multiply_wilson_d_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+
+
+def multiply_wilson_ddag_e_o_no_comm(out, in_, inv):
+    """dslash.cpp:910-959."""
+    return _apply_half("qlat_dwf_multiply_wilson_ddag_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+
+
+def multiply_m_e_o(out, in_, inv):
+    """dslash.cpp:961-1003 (works for _o_e as well)."""
+    return _apply_half("qlat_dwf_multiply_m_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+
+
+def multiply_mdag_e_o(out, in_, inv):
+    """dslash.cpp:1005-1060."""
+    return _apply_half("qlat_dwf_multiply_mdag_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+
+
+def multiply_mpc_sym2(out, in_, inv):
+    """dslash.cpp:1102-1119 (odd <- odd, works for even <- even)."""
+    return _apply_half("qlat_dwf_multiply_mpc_sym2", out, in_, inv, in_.eo, in_.eo)
+
+
+def multiply_mpcdag_sym2(out, in_, inv):
+    """dslash.cpp:1121-1138."""
+    return _apply_half("qlat_dwf_multiply_mpcdag_sym2", out, in_, inv, in_.eo, in_.eo)
+
+
+def multiply_hermop_sym2(out, in_, inv):
+    """dslash.cpp:1140-1149."""
+    return _apply_half("qlat_dwf_multiply_hermop_sym2", out, in_, inv, in_.eo, in_.eo)
+
+
+def _apply_full(name, out, in_, inv):
+    assert isinstance(in_, FermionField5d) and in_.eo == 0
+    src = np.ascontiguousarray(in_.data, dtype=np.complex128)
+    res = FermionField5d.__new__(FermionField5d)
+    res.geo, res.ls, res.eo = in_.geo, in_.ls, 0
+    res.data = np.empty_like(src)
+    _lib.check(getattr(_lib.load(), name)(inv.cdata, _lib.ptr(res.data), _lib.ptr(src)), name)
+    if out is not None:
+        out.geo, out.ls, out.eo, out.data = res.geo, res.ls, 0, res.data
+        return out
+    return res
+
+
+def multiply_m(out, in_, inv):
+    """dslash.cpp:1062-1080."""
+    return _apply_full("qlat_dwf_multiply_m", out, in_, inv)
+
+
+def multiply_mdag(out, in_, inv):
+    """dslash.cpp:1082-1100."""
+    return _apply_full("qlat_dwf_multiply_mdag", out, in_, inv)
+
+
+def multiply_d_minus(out, in_, inv):
+    """dslash.cpp:448-488."""
+    return _apply_full("qlat_dwf_multiply_d_minus", out, in_, inv)
+
+
+def dot_product(ff1, ff2, inv):
+    """ff1^+ ff2 (dslash.cpp:1298-1321), reduced on the GPU of `inv`."""
+    a = np.ascontiguousarray(ff1.data, dtype=np.complex128)
+    b = np.ascontiguousarray(ff2.data, dtype=np.complex128)
+    assert a.shape == b.shape
+    re, im = ctypes.c_double(), ctypes.c_double()
+    _lib.check(_lib.load().qlat_dwf_dot_product(inv.cdata, _lib.ptr(a), _lib.ptr(b), a.size // 12,
+                                                ctypes.byref(re), ctypes.byref(im)),
+               "qlat_dwf_dot_product")
+    return complex(re.value, im.value)
+
+
+def qnorm(ff, inv):
+    """field.h:170-195, reduced on the GPU of `inv`."""
+    a = np.ascontiguousarray(ff.data, dtype=np.complex128)
+    r = ctypes.c_double()
+    _lib.check(_lib.load().qlat_dwf_qnorm(inv.cdata, _lib.ptr(a), a.size // 12, ctypes.byref(r)),
+               "qlat_dwf_qnorm")
+    return r.value
+
+
+def cg_with_herm_sym_2(sol, src, inv, stop_rsd=1e-8, max_num_iter=50000):
+    """dslash.cpp:1323-1333 on odd half fields; `sol` is the initial guess
+    when it holds data (cg_with_f, dslash.h:341-346).  Returns iterations,
+    max_num_iter + 1 if not converged."""
+    _check_half(src, inv)
+    b = np.ascontiguousarray(src.data, dtype=np.complex128)
+    if sol.data is None or getattr(sol, "eo", None) != src.eo:
+        x = np.zeros_like(b)
+    else:
+        x = np.ascontiguousarray(sol.data, dtype=np.complex128).copy()
+    it = ctypes.c_long()
+    _lib.check(_lib.load().qlat_dwf_cg_with_herm_sym_2(inv.cdata, _lib.ptr(x), _lib.ptr(b),
+                                                       float(stop_rsd), int(max_num_iter),
+                                                       ctypes.byref(it)),
+               "qlat_dwf_cg_with_herm_sym_2")
+    sol.geo, sol.ls, sol.eo, sol.data = src.geo, src.ls, src.eo, x
+    return it.value
+
+
+def invert(out, in_, inv):
+    """invert(5d) (dslash.cpp:1335-1340), invert(4d) (dslash.cpp:1342-1347)
+    or invert(Prop) (qcd-prop.h:169-186), by the type of `in_`.  Returns the
+    total number of CG iterations."""
+    L = _lib.load()
+    it = ctypes.c_long()
+    src = np.ascontiguousarray(in_.data, dtype=np.complex128)
+    res = np.empty_like(src)
+    if isinstance(in_, Prop):
+        _lib.check(L.qlat_dwf_invert_prop(inv.cdata, _lib.ptr(res), _lib.ptr(src), ctypes.byref(it)),
+                   "qlat_dwf_invert_prop")
+    elif isinstance(in_, FermionField4d):
+        _lib.check(L.qlat_dwf_invert_4d(inv.cdata, _lib.ptr(res), _lib.ptr(src), ctypes.byref(it)),
+                   "qlat_dwf_invert_4d")
+    elif isinstance(in_, FermionField5d):
+        assert in_.eo == 0
+        _lib.check(L.qlat_dwf_invert_5d(inv.cdata, _lib.ptr(res), _lib.ptr(src), ctypes.byref(it)),
+                   "qlat_dwf_invert_5d")
+        out.ls, out.eo = in_.ls, 0
+    else:
+        raise TypeError("invert: unsupported field type")
+    out.geo = in_.geo
+    out.data = res
+    return it.value
+
+
+def find_max_eigen_value_hermop_sym2(inv, start, max_iter=100):
+    """Power iteration for the largest eigenvalue of Mpc^+ Mpc
+    (dslash.cpp:1349-1369) from the odd half-field `start`."""
+    _check_half(start, inv)
+    a = np.ascontiguousarray(start.data, dtype=np.complex128)
+    r = ctypes.c_double()
+    _lib.check(_lib.load().qlat_dwf_find_max_eigen_value_hermop_sym2(inv.cdata, _lib.ptr(a),
+                                                                     int(max_iter), ctypes.byref(r)),
+               "qlat_dwf_find_max_eigen_value_hermop_sym2")
+    return r.value
+
+
+def bench_kernel(inv, which, precision, iters):
+    """Average launch time (ms) of one kernel, HIP events on the inverter's
+    stream.  which: 0 hop, 1 hop^+, 2 s-matrix, 3 Mpc^+ Mpc."""
+    ms = ctypes.c_double()
+    _lib.check(_lib.load().qlat_dwf_bench_kernel(inv.cdata, int(which), int(precision), int(iters),
+                                                 ctypes.byref(ms)),
+               "qlat_dwf_bench_kernel")
+    return ms.value

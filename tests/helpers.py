@@ -1,0 +1,50 @@
+"""Shared fixtures-as-functions for the test suite (inputs only)."""
+
+import numpy as np
+
+from qlattice_amd.gauge import rand_gauge_links, twist_boundary_at_boundary
+
+
+def unit_gauge(V):
+    U = np.zeros((V, 4, 3, 3), dtype=np.complex128)
+    for i in range(3):
+        U[:, :, i, i] = 1.0
+    return U
+
+
+def rand_gauge(site, sigma, seed, antiperiodic=True):
+    U = rand_gauge_links(site, sigma, seed)
+    if antiperiodic:
+        twist_boundary_at_boundary(U, site, -0.5, 3)
+    return U
+
+
+def rand_field(shape, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def point_src(V, site_index=0):
+    p = np.zeros((V, 12, 12), dtype=np.complex128)
+    p[site_index] = np.eye(12)
+    return p
+
+
+ZMOBIUS_OMEGA = [
+    1.0903256131299373,
+    0.9570283702230611,
+    0.7048886040934104,
+    0.48979921782791747,
+    0.328608311201356,
+    0.21664245377015995,
+    0.14121112711957107,
+    0.0907785101745156,
+    complex(0.05608303440064219, -0.007537158177840385),
+    complex(0.05608303440064219, 0.007537158177840385),
+    complex(0.0365221637144842, -0.03343945161367745),
+    complex(0.0365221637144842, 0.03343945161367745),
+]
