@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-hermops", type=int, default=3, help="oracle hermop applications (0 = skip)")
     ap.add_argument("--kernel-iters", type=int, default=50)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time the kernels and stop (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -133,6 +135,12 @@ def main():
     gbs64 = b64 / (hop64_ms * 1e-3) / 1e9
     log(f"hop fp32 {hop32_ms:.3f} ms {gbs32:.0f} GB/s | hop fp64 {hop64_ms:.3f} ms {gbs64:.0f} GB/s | "
         f"m5 fp32 {m5_32_ms:.3f} ms | hermop fp32 {herm32_ms:.3f} ms fp64 {herm64_ms:.3f} ms")
+
+    if args.kernels_only:
+        print(json.dumps({"hop_fp32_ms": hop32_ms, "hop_fp32_gbps": gbs32, "hop_fp64_ms": hop64_ms,
+                          "hop_fp64_gbps": gbs64, "m5_fp32_ms": m5_32_ms, "hermop_fp32_ms": herm32_ms,
+                          "hermop_fp64_ms": herm64_ms}), flush=True)
+        return
 
     # point-source propagator, resident on the device
     ncol = 12

@@ -188,7 +188,7 @@ int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const doubl
  * Average duration (ms) of one launch of a kernel, timed with HIP events
  * on the inverter's stream over `iters` back-to-back launches after one
  * warm-up launch.  which: 0 hop H, 1 hop H^+, 2 dense s-matrix (M_ee^-1),
- * 3 Mpc^+ Mpc (all its kernels).  precision: 64 or 32.
+ * 3 Mpc^+ Mpc (all its kernels), 4 plain copy of one field.  precision: 64 or 32.
  * Has no counterpart in the reference (its Timer, timer.h:290, is host
  * wall clock); it exists to report kernel bandwidth. */
 int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int iters, double* avg_ms);

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -160,7 +161,8 @@ struct qlat_dwf_inverter {
   long max_cycle = 300;
   int solver = QLAT_DWF_SOLVER_FP64;
   double delta = 0.1;
-  int hop_spt64 = 2, hop_spt32 = 4;
+  // hop launch shapes (launch_hop): LDS-staged links; FP64 with streaming stores
+  int hop_spt64 = -102, hop_spt32 = -4;
 
   double2* l64 = nullptr;
   float2* l32 = nullptr;
@@ -915,6 +917,16 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     QDWF_HIP(hipStreamCreate(&I->stream));
     for (int i = 0; i < 4; ++i) I->geo.n[i] = total_site[i];
     I->geo.xh = total_site[0] / 2;
+    // device site order: tiles of `by` rows in y (dwf_common.h); by = Y is
+    // the reference order.  QLAT_DWF_TILE_Y overrides the default.
+    // default: the divisor of Y closest to 6 rows (measured best at 24^3x64)
+    I->geo.by = total_site[1];
+    for (int by = 1; by <= total_site[1]; ++by)
+      if (total_site[1] % by == 0 && std::abs(by - 6) < std::abs(I->geo.by - 6)) I->geo.by = by;
+    if (const char* e = std::getenv("QLAT_DWF_TILE_Y")) {
+      const int by = std::atoi(e);
+      if (by > 0 && total_site[1] % by == 0) I->geo.by = by;
+    }
     I->geo.vh = V / 2;
     I->geo.stride = (I->geo.vh + qdwf::kStrideAlign - 1) / qdwf::kStrideAlign * qdwf::kStrideAlign;
     I->ls = fa->ls;
@@ -947,6 +959,9 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     QDWF_HIP(hipStreamSynchronize(I->stream));
     QDWF_HIP(hipFree(g));
     build_matrices(*I);
+    // hop launch shape: QLAT_DWF_HOP_SPT32 / QLAT_DWF_HOP_SPT64 (see launch_hop)
+    if (const char* e = std::getenv("QLAT_DWF_HOP_SPT32")) I->hop_spt32 = std::atoi(e);
+    if (const char* e = std::getenv("QLAT_DWF_HOP_SPT64")) I->hop_spt64 = std::atoi(e);
     I->part = (double*)dev_alloc(qdwf::kMaxBlasGrid * 4 * sizeof(double));
     I->red = (double*)dev_alloc(8 * sizeof(double));
     QDWF_HIP(hipHostMalloc((void**)&I->red_host, 8 * sizeof(double), hipHostMallocDefault));
@@ -1251,7 +1266,7 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
     check_inv(inv);
     QDWF_REQUIRE(avg_ms && iters > 0, "invalid argument");
     QDWF_REQUIRE(precision == 64 || precision == 32, "precision must be 64 or 32");
-    QDWF_REQUIRE(which >= 0 && which <= 3, "which must be 0..3");
+    QDWF_REQUIRE(which >= 0 && which <= 4, "which must be 0..4");
     Inv& I = *inv;
     auto run = [&]() {
       if (precision == 64) {
@@ -1260,14 +1275,22 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
         if (which == 0) hop<double>(I, b, a, 1, false);
         else if (which == 1) hop<double>(I, b, a, 1, true);
         else if (which == 2) m5<double>(I, b, a, MAT_MEE_INV);
-        else op_hermop<double>(I, b, a, 1);
+        else if (which == 3) op_hermop<double>(I, b, a, 1);
+        else {
+          qdwf::launch_copy((const float4*)a, (float4*)b, 2 * n32(I), I.stream);
+          check_launch();
+        }
       } else {
         float4* a = V(I, V_R);
         float4* b = V(I, V_AP);
         if (which == 0) hop<float>(I, b, a, 1, false);
         else if (which == 1) hop<float>(I, b, a, 1, true);
         else if (which == 2) m5<float>(I, b, a, MAT_MEE_INV);
-        else op_hermop<float>(I, b, a, 1);
+        else if (which == 3) op_hermop<float>(I, b, a, 1);
+        else {
+          qdwf::launch_copy(a, b, n32(I), I.stream);
+          check_launch();
+        }
       }
     };
     // finite, non-trivial input values

@@ -1,14 +1,17 @@
 // Shared device-side definitions of the gfx950 domain-wall library.
 //
-// Device layouts (structure of arrays, lanes = consecutive half-sites):
-//   half spinor field, FP64: double2  f[(s*12 + k)*stride + h]   k = spin*3+colour
-//   half spinor field, FP32: float4   f[(s*6 + kp)*stride + h]   complex 2kp,2kp+1
+// Device layouts (blocked structure of arrays: lanes = 64 consecutive
+// device half-sites; hb = h / 64, lane = h % 64):
+//   half spinor field, FP64: double2  f[((hb*Ls + s)*12 + k)*64 + lane]   k = spin*3+colour
+//   half spinor field, FP32: float4   f[((hb*Ls + s)*6 + kp)*64 + lane]   complex 2kp, 2kp+1
 //   links, FP64 / FP32:      double2 / float2
-//                            g[((p*8 + d)*9 + e)*stride + h]
+//                            g[((p*nhb + hb)*72 + d*9 + e)*64 + lane]
 //     p = output parity slot (0 even, 1 odd), d = 0..3 forward U_mu(x),
-//     d = 4..7 backward U_mu(x - mu)^dagger, e = row*3+col.
-// `stride` is the half volume rounded up to a multiple of 256; pad entries
-// are zero and never written.
+//     d = 4..7 backward U_mu(x - mu)^dagger, e = row*3+col, nhb = stride/64.
+// All Ls values of a 64-site block are contiguous (72 KB in FP32), and so
+// are a block's 8 links (37 KB), which keeps the number of pages a wave
+// touches small.  `stride` is the half volume rounded up to a multiple of
+// 256; pad entries are zero and never written.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -66,33 +69,71 @@ __device__ __forceinline__ cplx<T> cneg(cplx<T> a)
 }
 
 // ---------------------------------------------------------------- geometry
-// Lattice extents and parity bookkeeping.  Site order is the reference's
-// (x fastest); a half-site index is full_index/2 (qlat core.h:232-261).
+// Lattice extents and the device site order.  The reference orders sites
+// lexicographically with x fastest and a half-site index is full_index/2
+// (qlat core.h:232-261).  On the device the half sites of one parity are
+// ordered as
+//     d = xh + X/2 * (yi + by * (z + Z * (t + T * yt))),  y = yt*by + yi,
+// i.e. the lattice is cut into tiles of `by` rows in y and each tile is
+// swept in t.  A t neighbour is then X/2*by*Z half sites away instead of
+// X/2*Y*Z, which keeps it in the L2 of the XCD that reads it.  by = Y is
+// the reference order.  Layout conversion at the boundary applies the
+// permutation (pack/unpack kernels).
 struct LatGeo {
   int n[4];       // X Y Z T
   int xh;         // X/2
+  int by;         // tile height in y (divides Y)
   int64_t vh;     // V/2
   int64_t stride; // vh rounded up to kStrideAlign
 };
 
-// Full coordinate of half-site h on parity eo (2 even, 1 odd).
-__device__ __forceinline__ void half_coord(const LatGeo& g, int64_t h, int eo,
-                                           int x[4])
+// Full coordinate of the device half-site d on parity eo (2 even, 1 odd).
+// 32-bit arithmetic: a half volume is < 2^31 for any lattice that fits in
+// device memory.
+__device__ __forceinline__ void half_coord(const LatGeo& g, int64_t d64, int eo, int x[4])
 {
-  int64_t r = h / g.xh;
-  const int xh = (int)(h - r * g.xh);
-  x[1] = (int)(r % g.n[1]);
-  r /= g.n[1];
-  x[2] = (int)(r % g.n[2]);
-  x[3] = (int)(r / g.n[2]);
+  const unsigned d = (unsigned)d64;
+  const unsigned xh = (unsigned)g.xh, by = (unsigned)g.by, nz = (unsigned)g.n[2], nt = (unsigned)g.n[3];
+  unsigned r = d / xh;
+  const unsigned xr = d - r * xh;
+  unsigned q = r / by;
+  const unsigned yi = r - q * by;
+  r = q / nz;
+  x[2] = (int)(q - r * nz);
+  q = r / nt;
+  x[3] = (int)(r - q * nt);
+  x[1] = (int)(q * by + yi);
   const int odd = (eo == 1) ? 1 : 0;
-  x[0] = 2 * xh + ((x[1] + x[2] + x[3] + odd) & 1);
+  x[0] = 2 * (int)xr + ((x[1] + x[2] + x[3] + odd) & 1);
 }
 
+// Device half-site index of full coordinate x (0 <= x[mu] < n[mu]).
 __device__ __forceinline__ int64_t half_index(const LatGeo& g, const int x[4])
 {
-  return (int64_t)(x[0] >> 1) +
-         (int64_t)g.xh * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
+  const unsigned by = (unsigned)g.by;
+  const unsigned y = (unsigned)x[1], yt = y / by, yi = y - yt * by;
+  return (int64_t)((unsigned)(x[0] >> 1) +
+                   (unsigned)g.xh * (yi + by * ((unsigned)x[2] + (unsigned)g.n[2] * ((unsigned)x[3] + (unsigned)g.n[3] * yt))));
+}
+
+// Slot index of (site h, fifth coordinate s, 16-byte slot) in a blocked
+// spinor field with NS slots per 5D site.
+template <int NS>
+__device__ __forceinline__ int64_t sidx(int ls, int s, int slot, int64_t h)
+{
+  return ((((h >> 6) * ls + s) * NS + slot) << 6) + (h & 63);
+}
+
+// Element index of link (p, d, e) of site h in the blocked link layout.
+__device__ __forceinline__ int64_t lidx(int64_t stride, int p, int de, int64_t h)
+{
+  return ((((int64_t)p * (stride >> 6) + (h >> 6)) * 72 + de) << 6) + (h & 63);
+}
+
+// Reference (lexicographic) full site index of coordinate x.
+__device__ __forceinline__ int64_t full_index(const LatGeo& g, const int x[4])
+{
+  return (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
 }
 
 __device__ __forceinline__ int wrap_inc(int v, int n) { return (v + 1 == n) ? 0 : v + 1; }
@@ -121,6 +162,33 @@ __device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t nblk)
   return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
+// Non-temporal (streaming) stores: outputs that the next kernel reads in a
+// different order should not evict this kernel's reusable inputs from L2.
+__device__ __forceinline__ void store_nt(float4* p, const float4& v)
+{
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  v4f t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, (v4f*)p);
+}
+__device__ __forceinline__ void store_nt(double2* p, const double2& v)
+{
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  v2d t = {v.x, v.y};
+  __builtin_nontemporal_store(t, (v2d*)p);
+}
+__device__ __forceinline__ float2 load_nt(const float2* p)
+{
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  const v2f t = __builtin_nontemporal_load((const v2f*)p);
+  return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ double2 load_nt(const double2* p)
+{
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  const v2d t = __builtin_nontemporal_load((const v2d*)p);
+  return make_double2(t.x, t.y);
+}
+
 // ---------------------------------------------------------------- spinors
 template <class T>
 struct SpinorIO;
@@ -129,20 +197,29 @@ template <>
 struct SpinorIO<double> {
   typedef double2 slot_t;
   static constexpr int kSlots = 12;
-  __device__ __forceinline__ static void load(const double2* __restrict__ f, int64_t stride,
-                                              int s, int64_t h, cplx<double> v[12])
+  __device__ __forceinline__ static void load(const double2* __restrict__ f, int ls, int s, int64_t h,
+                                              cplx<double> v[12])
   {
+    const double2* p = f + sidx<12>(ls, s, 0, h);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      const double2 a = f[((int64_t)s * 12 + k) * stride + h];
+      const double2 a = p[k * 64];
       v[k] = cmk<double>(a.x, a.y);
     }
   }
-  __device__ __forceinline__ static void store(double2* __restrict__ f, int64_t stride, int s,
-                                               int64_t h, const cplx<double> v[12])
+  __device__ __forceinline__ static void store(double2* __restrict__ f, int ls, int s, int64_t h,
+                                               const cplx<double> v[12])
   {
+    double2* p = f + sidx<12>(ls, s, 0, h);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) f[((int64_t)s * 12 + k) * stride + h] = make_double2(v[k].re, v[k].im);
+    for (int k = 0; k < 12; ++k) p[k * 64] = make_double2(v[k].re, v[k].im);
+  }
+  __device__ __forceinline__ static void store_stream(double2* __restrict__ f, int ls, int s, int64_t h,
+                                                      const cplx<double> v[12])
+  {
+    double2* p = f + sidx<12>(ls, s, 0, h);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) store_nt(&p[k * 64], make_double2(v[k].re, v[k].im));
   }
 };
 
@@ -150,23 +227,32 @@ template <>
 struct SpinorIO<float> {
   typedef float4 slot_t;
   static constexpr int kSlots = 6;
-  __device__ __forceinline__ static void load(const float4* __restrict__ f, int64_t stride,
-                                              int s, int64_t h, cplx<float> v[12])
+  __device__ __forceinline__ static void load(const float4* __restrict__ f, int ls, int s, int64_t h,
+                                              cplx<float> v[12])
   {
+    const float4* p = f + sidx<6>(ls, s, 0, h);
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      const float4 a = f[((int64_t)s * 6 + k) * stride + h];
+      const float4 a = p[k * 64];
       v[2 * k] = cmk<float>(a.x, a.y);
       v[2 * k + 1] = cmk<float>(a.z, a.w);
     }
   }
-  __device__ __forceinline__ static void store(float4* __restrict__ f, int64_t stride, int s,
-                                               int64_t h, const cplx<float> v[12])
+  __device__ __forceinline__ static void store(float4* __restrict__ f, int ls, int s, int64_t h,
+                                               const cplx<float> v[12])
   {
+    float4* p = f + sidx<6>(ls, s, 0, h);
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-      f[((int64_t)s * 6 + k) * stride + h] =
-          make_float4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im);
+      p[k * 64] = make_float4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im);
+  }
+  __device__ __forceinline__ static void store_stream(float4* __restrict__ f, int ls, int s, int64_t h,
+                                                      const cplx<float> v[12])
+  {
+    float4* p = f + sidx<6>(ls, s, 0, h);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      store_nt(&p[k * 64], make_float4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im));
   }
 };
 
@@ -176,12 +262,13 @@ struct LinkIO;
 template <>
 struct LinkIO<double> {
   typedef double2 elem_t;
-  __device__ __forceinline__ static void load(const double2* __restrict__ g, int64_t stride,
-                                              int pd, int64_t h, cplx<double> u[9])
+  __device__ __forceinline__ static void load(const double2* __restrict__ g, int64_t stride, int p, int d,
+                                              int64_t h, cplx<double> u[9])
   {
+    const double2* q = g + lidx(stride, p, d * 9, h);
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
-      const double2 a = g[((int64_t)pd * 9 + e) * stride + h];
+      const double2 a = q[e * 64];
       u[e] = cmk<double>(a.x, a.y);
     }
   }
@@ -190,12 +277,13 @@ struct LinkIO<double> {
 template <>
 struct LinkIO<float> {
   typedef float2 elem_t;
-  __device__ __forceinline__ static void load(const float2* __restrict__ g, int64_t stride,
-                                              int pd, int64_t h, cplx<float> u[9])
+  __device__ __forceinline__ static void load(const float2* __restrict__ g, int64_t stride, int p, int d,
+                                              int64_t h, cplx<float> u[9])
   {
+    const float2* q = g + lidx(stride, p, d * 9, h);
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
-      const float2 a = g[((int64_t)pd * 9 + e) * stride + h];
+      const float2 a = q[e * 64];
       u[e] = cmk<float>(a.x, a.y);
     }
   }

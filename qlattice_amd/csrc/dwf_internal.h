@@ -53,6 +53,7 @@ template <class S>
 void launch_cg_update(double a, const S* p, const S* ap, S* x, S* r, int64_t n, double* part, double* out,
                       hipStream_t st);
 
+void launch_copy(const float4* x, float4* y, int64_t n, hipStream_t st);
 void launch_d2f(const double2* in, float4* out, int ls, int64_t st, hipStream_t s);
 void launch_f2d(const float4* in, double2* out, int ls, int64_t st, int acc, hipStream_t s);
 

@@ -46,12 +46,12 @@ __global__ void __launch_bounds__(kHopBlock) k_hop(HopArgs<T> a)
 #define QDWF_HOP_DIR(D, MU, SG)                                          \
   {                                                                      \
     cplx<T> W[9];                                                        \
-    LinkIO<T>::load(links, a.geo.stride, pslot * 8 + (D), h, W);         \
+    LinkIO<T>::load(links, a.geo.stride, pslot, (D), h, W);              \
     _Pragma("unroll") for (int j = 0; j < SPT; ++j)                      \
     {                                                                    \
       if (SPT == 1 || s0 + j < a.ls) {                                   \
         cplx<T> psi[12];                                                 \
-        SpinorIO<T>::load(in, a.geo.stride, s0 + j, nb[D], psi);         \
+        SpinorIO<T>::load(in, a.ls, s0 + j, nb[D], psi);                 \
         hop_term<MU, SG>(W, psi, acc[j]);                                \
       }                                                                  \
     }                                                                    \
@@ -77,13 +77,94 @@ __global__ void __launch_bounds__(kHopBlock) k_hop(HopArgs<T> a)
     for (int k = 0; k < 12; ++k) v[k] = cmk<T>(scale * acc[j][k].re, scale * acc[j][k].im);
     if (XPAY) {
       cplx<T> xv[12];
-      SpinorIO<T>::load((const slot_t*)a.x, a.geo.stride, s, h, xv);
+      SpinorIO<T>::load((const slot_t*)a.x, a.ls, s, h, xv);
       const cplx<T> b = a.beta[s];
 #pragma unroll
       for (int k = 0; k < 12; ++k) v[k] = cfma(v[k], b, xv[k]);
     }
-    SpinorIO<T>::store(out, a.geo.stride, s, h, v);
+    SpinorIO<T>::store(out, a.ls, s, h, v);
   }
+}
+
+// Variant with the block's links staged in LDS: a block covers 64
+// consecutive output half-sites; its NW waves load the 64 x 8 links once,
+// then wave w computes s = w, w+NW, ... for all 64 sites, reading the links
+// from LDS.  One link load from HBM per site per launch, independent of Ls.
+template <class T, int NW, bool DAG, bool XPAY, bool NT>
+__global__ void __launch_bounds__(64 * NW) k_hop_lds(HopArgs<T> a)
+{
+  typedef typename SpinorIO<T>::slot_t slot_t;
+  typedef typename LinkIO<T>::elem_t link_t;
+  __shared__ link_t sl[72][64];
+  const int64_t lb = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t h0 = lb * 64;
+  const int pslot = (a.eo_out == 2) ? 0 : 1;
+  // the block's 72 x 64 link elements are contiguous (pads are zero)
+  const link_t* __restrict__ links = (const link_t*)a.links + lidx(a.geo.stride, pslot, 0, h0);
+  for (int i = threadIdx.x; i < 72 * 64; i += 64 * NW) sl[i >> 6][i & 63] = NT ? load_nt(&links[i]) : links[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t h = h0 + lane;
+  if (h >= a.geo.vh) return;
+  int x[4];
+  half_coord(a.geo, h, a.eo_out, x);
+  int64_t nb[8];
+  neighbours(a.geo, x, nb);
+  const slot_t* __restrict__ in = (const slot_t*)a.in;
+  slot_t* __restrict__ out = (slot_t*)a.out;
+  constexpr int SF = DAG ? 1 : -1;
+  constexpr int SB = -SF;
+  const T scale = (T)(-0.5) * a.alpha;
+  for (int s = w; s < a.ls; s += NW) {
+    cplx<T> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = cmk<T>(0, 0);
+#define QDWF_HOP_LDS_DIR(D, MU, SG)                                \
+  {                                                                \
+    cplx<T> W[9];                                                  \
+    _Pragma("unroll") for (int e = 0; e < 9; ++e)                  \
+    {                                                              \
+      const link_t q = sl[(D) * 9 + e][lane];                      \
+      W[e] = cmk<T>(q.x, q.y);                                     \
+    }                                                              \
+    cplx<T> psi[12];                                               \
+    SpinorIO<T>::load(in, a.ls, s, nb[D], psi);                    \
+    hop_term<MU, SG>(W, psi, acc);                                 \
+  }
+    QDWF_HOP_LDS_DIR(0, 0, SF)
+    QDWF_HOP_LDS_DIR(1, 1, SF)
+    QDWF_HOP_LDS_DIR(2, 2, SF)
+    QDWF_HOP_LDS_DIR(3, 3, SF)
+    QDWF_HOP_LDS_DIR(4, 0, SB)
+    QDWF_HOP_LDS_DIR(5, 1, SB)
+    QDWF_HOP_LDS_DIR(6, 2, SB)
+    QDWF_HOP_LDS_DIR(7, 3, SB)
+#undef QDWF_HOP_LDS_DIR
+    cplx<T> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cmk<T>(scale * acc[k].re, scale * acc[k].im);
+    if (XPAY) {
+      cplx<T> xv[12];
+      SpinorIO<T>::load((const slot_t*)a.x, a.ls, s, h, xv);
+      const cplx<T> b = a.beta[s];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v[k] = cfma(v[k], b, xv[k]);
+    }
+    if (NT) SpinorIO<T>::store_stream(out, a.ls, s, h, v);
+    else SpinorIO<T>::store(out, a.ls, s, h, v);
+  }
+}
+
+template <class T, int NW, bool NT>
+static void launch_hop_lds(const HopArgs<T>& a, bool dag, hipStream_t st)
+{
+  const dim3 grid((unsigned)((a.geo.vh + 63) / 64));
+  const dim3 blk(64 * NW);
+  const bool xpay = a.x != nullptr;
+  if (!dag && !xpay) hipLaunchKernelGGL((k_hop_lds<T, NW, false, false, NT>), grid, blk, 0, st, a);
+  if (!dag && xpay) hipLaunchKernelGGL((k_hop_lds<T, NW, false, true, NT>), grid, blk, 0, st, a);
+  if (dag && !xpay) hipLaunchKernelGGL((k_hop_lds<T, NW, true, false, NT>), grid, blk, 0, st, a);
+  if (dag && xpay) hipLaunchKernelGGL((k_hop_lds<T, NW, true, true, NT>), grid, blk, 0, st, a);
 }
 
 template <class T, int SPT>
@@ -100,10 +181,17 @@ static void launch_hop_spt(const HopArgs<T>& a0, bool dag, hipStream_t st)
   if (dag && xpay) hipLaunchKernelGGL((k_hop<T, SPT, true, true>), grid, dim3(kHopBlock), 0, st, a);
 }
 
+// spt > 0: one thread per site and `spt` s values (k_hop<.., spt>);
+// spt = -2 / -4: LDS-staged links, 2 or 4 waves per block (k_hop_lds);
+// spt = -102 / -104: the same with streaming output stores and link loads.
 template <class T>
 void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st)
 {
   switch (spt) {
+    case -2: launch_hop_lds<T, 2, false>(a, dag, st); break;
+    case -4: launch_hop_lds<T, 4, false>(a, dag, st); break;
+    case -102: launch_hop_lds<T, 2, true>(a, dag, st); break;
+    case -104: launch_hop_lds<T, 4, true>(a, dag, st); break;
     case 1: launch_hop_spt<T, 1>(a, dag, st); break;
     case 2: launch_hop_spt<T, 2>(a, dag, st); break;
     case 3: launch_hop_spt<T, 3>(a, dag, st); break;
@@ -137,14 +225,13 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
   const cplx<T>* __restrict__ M = a.mat + (int64_t)chi * ls * ls;
   const slot_t* __restrict__ in = (const slot_t*)a.in;
   slot_t* __restrict__ out = (slot_t*)a.out;
-  const int64_t st = a.geo.stride;
   // a slot holds one (FP64) or two (FP32) complex numbers
   constexpr int NC = (NS == 12) ? 1 : 2;
   cplx<T> v[LSMAX][NC];
 #pragma unroll
   for (int t = 0; t < LSMAX; ++t) {
     if (t < ls) {
-      const slot_t q = in[((int64_t)t * NS + slot) * st + h];
+      const slot_t q = in[sidx<NS>(ls, t, slot, h)];
       if constexpr (NC == 1) {
         v[t][0] = cmk<T>(q.x, q.y);
       } else {
@@ -167,7 +254,7 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) r[c] = cmk<T>(a.alpha * r[c].re, a.alpha * r[c].im);
-    const int64_t off = ((int64_t)s * NS + slot) * st + h;
+    const int64_t off = sidx<NS>(ls, s, slot, h);
     if (XPAY) {
       const slot_t q = ((const slot_t*)a.x)[off];
       if constexpr (NC == 1) {
@@ -353,29 +440,32 @@ __global__ void k_reduce_final(const double* __restrict__ part, int nblk, int nv
   }
 }
 
-// FP64 [s][12][st] double2  <->  FP32 [s][6][st] float4
+// FP64 blocked field (12 double2 slots per 5D site) <-> FP32 blocked field
+// (6 float4 slots per 5D site); i enumerates FP32 slots.
 __global__ void __launch_bounds__(kBlasBlock) k_d2f(const double2* __restrict__ in, float4* __restrict__ out,
-                                                    int64_t nslot32, int64_t st)
+                                                    int64_t nslot32)
 {
   for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < nslot32; i += (int64_t)gridDim.x * kBlasBlock) {
-    const int64_t h = i % st, q = i / st;  // q = s*6 + kp
-    const int64_t s = q / 6, kp = q - s * 6;
-    const double2 a = in[(s * 12 + 2 * kp) * st + h];
-    const double2 b = in[(s * 12 + 2 * kp + 1) * st + h];
+    const int64_t lane = i & 63, q = i >> 6;  // q = (hb*ls + s)*6 + kp
+    const int64_t hs = q / 6, kp = q - hs * 6;
+    const int64_t o = ((hs * 12 + 2 * kp) << 6) + lane;
+    const double2 a = in[o];
+    const double2 b = in[o + 64];
     out[i] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
   }
 }
 
 // out64 (+)= in32 : acc=0 overwrite, acc=1 add
 __global__ void __launch_bounds__(kBlasBlock) k_f2d(const float4* __restrict__ in, double2* __restrict__ out,
-                                                    int64_t nslot32, int64_t st, int acc)
+                                                    int64_t nslot32, int acc)
 {
   for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < nslot32; i += (int64_t)gridDim.x * kBlasBlock) {
-    const int64_t h = i % st, q = i / st;
-    const int64_t s = q / 6, kp = q - s * 6;
+    const int64_t lane = i & 63, q = i >> 6;
+    const int64_t hs = q / 6, kp = q - hs * 6;
+    const int64_t o = ((hs * 12 + 2 * kp) << 6) + lane;
     const float4 a = in[i];
-    double2* o0 = &out[(s * 12 + 2 * kp) * st + h];
-    double2* o1 = &out[(s * 12 + 2 * kp + 1) * st + h];
+    double2* o0 = &out[o];
+    double2* o1 = &out[o + 64];
     if (acc) {
       const double2 p0 = *o0, p1 = *o1;
       *o0 = make_double2(p0.x + a.x, p0.y + a.y);
@@ -385,6 +475,15 @@ __global__ void __launch_bounds__(kBlasBlock) k_f2d(const float4* __restrict__ i
       *o1 = make_double2(a.z, a.w);
     }
   }
+}
+
+// Plain copy (reads n slots, writes n slots): the reference point for
+// converting HBM traffic counters into bytes.
+template <class S>
+__global__ void __launch_bounds__(kBlasBlock) k_copy(const S* __restrict__ x, S* __restrict__ y, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock)
+    y[i] = x[i];
 }
 
 static int blas_grid(int64_t n)
@@ -428,6 +527,11 @@ void launch_cg_update(double a, const S* p, const S* ap, S* x, S* r, int64_t n, 
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, part, g, 1, out);
 }
 
+void launch_copy(const float4* x, float4* y, int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_copy<float4>, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, x, y, n);
+}
+
 template void launch_norm2<double2>(const double2*, int64_t, double*, double*, hipStream_t);
 template void launch_norm2<float4>(const float4*, int64_t, double*, double*, hipStream_t);
 template void launch_dot<double2>(const double2*, const double2*, int64_t, double*, double*, hipStream_t);
@@ -444,12 +548,12 @@ template void launch_cg_update<float4>(double, const float4*, const float4*, flo
 void launch_d2f(const double2* in, float4* out, int ls, int64_t st, hipStream_t s)
 {
   const int64_t n = (int64_t)ls * 6 * st;
-  hipLaunchKernelGGL(k_d2f, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n, st);
+  hipLaunchKernelGGL(k_d2f, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n);
 }
 void launch_f2d(const float4* in, double2* out, int ls, int64_t st, int acc, hipStream_t s)
 {
   const int64_t n = (int64_t)ls * 6 * st;
-  hipLaunchKernelGGL(k_f2d, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n, st, acc);
+  hipLaunchKernelGGL(k_f2d, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n, acc);
 }
 
 // ============================================================ layouts
@@ -464,13 +568,11 @@ __global__ void __launch_bounds__(kBlasBlock) k_pack5(const double2* __restrict_
     // i enumerates (s, k, h) with h fastest
     const int64_t h = i % g.vh, q = i / g.vh;
     const int64_t s = q / 12, k = q - s * 12;
-    int64_t site = h;
-    if (full) {
-      int x[4];
-      half_coord(g, h, eo, x);
-      site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
-    }
-    soa[(s * 12 + k) * g.stride + h] = aos[(site * ls + s) * 12 + k];
+    // full field: reference site index; half field: reference half index
+    int x[4];
+    half_coord(g, h, eo, x);
+    const int64_t site = full ? full_index(g, x) : full_index(g, x) / 2;
+    soa[sidx<12>(ls, (int)s, (int)k, h)] = aos[(site * ls + s) * 12 + k];
   }
 }
 
@@ -481,13 +583,10 @@ __global__ void __launch_bounds__(kBlasBlock) k_unpack5(const double2* __restric
   for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
     const int64_t h = i % g.vh, q = i / g.vh;
     const int64_t s = q / 12, k = q - s * 12;
-    int64_t site = h;
-    if (full) {
-      int x[4];
-      half_coord(g, h, eo, x);
-      site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
-    }
-    aos[(site * ls + s) * 12 + k] = soa[(s * 12 + k) * g.stride + h];
+    int x[4];
+    half_coord(g, h, eo, x);
+    const int64_t site = full ? full_index(g, x) : full_index(g, x) / 2;
+    aos[(site * ls + s) * 12 + k] = soa[sidx<12>(ls, (int)s, (int)k, h)];
   }
 }
 
@@ -519,10 +618,10 @@ __global__ void __launch_bounds__(kBlasBlock) k_prop_col_to_5d(const double2* __
     if (take) {
       int x[4];
       half_coord(g, h, eo, x);
-      const int64_t site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
+      const int64_t site = full_index(g, x);
       v = (col >= 0) ? src[(site * 12 + k) * 12 + col] : src[site * 12 + k];
     }
-    soa[((int64_t)s * 12 + k) * g.stride + h] = v;
+    soa[sidx<12>(ls, s, k, h)] = v;
   }
 }
 
@@ -538,8 +637,8 @@ __global__ void __launch_bounds__(kBlasBlock) k_5d_to_prop_col(const double2* __
     const int s = (k < 6) ? ls - 1 : 0;
     int x[4];
     half_coord(g, h, eo, x);
-    const int64_t site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
-    const double2 v = soa[((int64_t)s * 12 + k) * g.stride + h];
+    const int64_t site = full_index(g, x);
+    const double2 v = soa[sidx<12>(ls, s, k, h)];
     if (col >= 0) dst[(site * 12 + k) * 12 + col] = v;
     else dst[site * 12 + k] = v;
   }
@@ -571,12 +670,12 @@ __global__ void __launch_bounds__(kBlasBlock) k_pack_gauge(const double2* __rest
     half_coord(g, h, eo, x);
     const int mu = d & 3;
     if (d >= 4) x[mu] = wrap_dec(x[mu], g.n[mu]);
-    const int64_t site = (int64_t)x[0] + (int64_t)g.n[0] * ((int64_t)x[1] + (int64_t)g.n[1] * ((int64_t)x[2] + (int64_t)g.n[2] * x[3]));
+    const int64_t site = full_index(g, x);
     const double2* U = &gauge[(site * 4 + mu) * 9];
     for (int r = 0; r < 3; ++r)
       for (int c = 0; c < 3; ++c) {
         double2 v = (d < 4) ? U[r * 3 + c] : make_double2(U[c * 3 + r].x, -U[c * 3 + r].y);
-        const int64_t off = (((int64_t)p * 8 + d) * 9 + r * 3 + c) * g.stride + h;
+        const int64_t off = lidx(g.stride, p, d * 9 + r * 3 + c, h);
         l64[off] = v;
         l32[off] = make_float2((float)v.x, (float)v.y);
       }

@@ -253,3 +253,29 @@ def test_mixed_precision_solver_reaches_tolerance():
     assert st["true_residual"] <= 1e-8
     assert st["hermop_low"] > 0 and st["reliable_updates"] > 0
     assert rel(xmx.data, exact) < 10 * e64 + 1e-9
+
+
+def test_tiled_device_order_matches_oracle(monkeypatch):
+    """The device site order with y tiles (QLAT_DWF_TILE_Y) is a layout
+    choice only: every operator and the solve must be unchanged."""
+    site = (4, 8, 4, 8)
+    U = rand_gauge(site, 0.4, 31)
+    for by in ("2", "4"):
+        monkeypatch.setenv("QLAT_DWF_TILE_Y", by)
+        for spt in ("1", "4", "-4", "-104"):
+            monkeypatch.setenv("QLAT_DWF_HOP_SPT32", spt)
+            monkeypatch.setenv("QLAT_DWF_HOP_SPT64", spt if spt != "4" else "2")
+            geo, inv, orc = make_pair(site, U, mass=0.1, ls=6, m5=1.8, mobius_scale=2.0)
+            x = rand_field((geo.total_volume() // 2, 6, 12), 400)
+            for name, fn, _ in HALF_OPS:
+                got = fn(None, half(geo, 6, 2, x), inv)
+                assert rel(got.data, orc.apply_half(name, x, 2)) < 1e-12, (by, spt, name)
+            y = rand_field((geo.total_volume(), 6, 12), 401)
+            assert rel(q.multiply_m(None, full(geo, 6, y), inv).data, orc.apply_full("m", y)) < 1e-12
+            inv.set_solver("mixed")
+            src = q.FermionField4d(geo)
+            src.data = rand_field((geo.total_volume(), 12), 402)
+            sol = inv * src
+            ref, _ = orc.invert_4d(src.data)
+            assert inv.stats()["true_residual"] <= 1e-8
+            assert rel(sol.data, ref) < 1e-6, (by, spt)
