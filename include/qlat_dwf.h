@@ -134,6 +134,12 @@ int qlat_dwf_multiply_mpcdag_sym2(qlat_dwf_inverter* inv, double* out, const dou
 /* multiply_hermop_sym2 (dslash.h:282-283, :323-324; dslash.cpp:1140-1149) */
 int qlat_dwf_multiply_hermop_sym2(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
 
+/* multiply_hermop_sym2 evaluated in FP32 (input rounded to FP32, result
+ * widened to FP64).  No counterpart in the reference: it is the operator
+ * the inner loop of the mixed-precision solver applies, exposed so that
+ * callers (and tests) can check it at FP32 accuracy. */
+int qlat_dwf_multiply_hermop_sym2_fp32(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
+
 /* ------------------------------------------- operators on 5D full fields */
 
 /* multiply_m (dslash.h:270-271, :311-312; dslash.cpp:1062-1080) */

@@ -34,6 +34,7 @@ from .inverter import (
     invert,
     multiply_d_minus,
     multiply_hermop_sym2,
+    multiply_hermop_sym2_fp32,
     multiply_m,
     multiply_m_e_e,
     multiply_m_e_e_inv,

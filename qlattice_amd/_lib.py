@@ -38,6 +38,7 @@ EXPORTED = [
     "qlat_dwf_multiply_mpc_sym2",
     "qlat_dwf_multiply_mpcdag_sym2",
     "qlat_dwf_multiply_hermop_sym2",
+    "qlat_dwf_multiply_hermop_sym2_fp32",
     "qlat_dwf_multiply_m",
     "qlat_dwf_multiply_mdag",
     "qlat_dwf_multiply_d_minus",
@@ -122,7 +123,8 @@ def load():
     for name in ("qlat_dwf_multiply_wilson_d_e_o", "qlat_dwf_multiply_wilson_ddag_e_o",
                  "qlat_dwf_multiply_m_e_o", "qlat_dwf_multiply_mdag_e_o",
                  "qlat_dwf_multiply_mpc_sym2", "qlat_dwf_multiply_mpcdag_sym2",
-                 "qlat_dwf_multiply_hermop_sym2", "qlat_dwf_multiply_hermop_sym2_device"):
+                 "qlat_dwf_multiply_hermop_sym2", "qlat_dwf_multiply_hermop_sym2_device",
+                 "qlat_dwf_multiply_hermop_sym2_fp32"):
         getattr(L, name).argtypes = [_vp, _dp, _dp, ctypes.c_int]
     L.qlat_dwf_dot_product.argtypes = [_vp, _dp, _dp, ctypes.c_long, _dp, _dp]
     L.qlat_dwf_qnorm.argtypes = [_vp, _dp, ctypes.c_long, _dp]

@@ -14,6 +14,7 @@
 // Ls x Ls complex matrix per chirality, built here in FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -22,6 +23,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/qlat_dwf.h"
@@ -142,9 +144,9 @@ Mat diag_plus(const std::vector<cd>& d, const std::vector<cd>& e, const Mat& S, 
 // ---------------------------------------------------------------- state
 enum Work64 {
   W_IN_E, W_IN_O, W_DM_E, W_DM_O, W_OUT_EP, W_IN_OP, W_OUT_OP, W_TMP, W_ITMP, W_OUT_E, W_OUT_O,
-  W_CG_R, W_CG_P, W_CG_AP, W_CG_X, W_OP_T1, W_OP_T2, W_OP_Y, W_COUNT64
+  W_CG_R, W_CG_P, W_CG_AP, W_CG_X, W_CG_T, W_OP_T1, W_OP_T2, W_OP_T3, W_OP_Y, W_COUNT64
 };
-enum Work32 { V_R, V_P, V_AP, V_X, V_T1, V_T2, V_Y, W_COUNT32 };
+enum Work32 { V_R, V_P, V_AP, V_X, V_T, V_T1, V_T2, V_T3, V_Y, W_COUNT32 };
 
 }  // namespace
 
@@ -163,6 +165,8 @@ struct qlat_dwf_inverter {
   double delta = 0.1;
   // hop launch shapes (launch_hop): LDS-staged links; FP64 with streaming stores
   int hop_spt64 = -102, hop_spt32 = -4;
+  bool fused32 = true;  // FP32 hop + s-matrix fusion (QLAT_DWF_FUSED=0 disables)
+  bool real_mat[16] = {};  // s-matrix MatId has only real entries (Moebius)
 
   double2* l64 = nullptr;
   float2* l32 = nullptr;
@@ -256,17 +260,21 @@ struct Prec<float> {
   static int64_t n(const Inv& I) { return n32(I); }
 };
 
-// out = alpha * Mat in + beta * x
+// out = alpha * Mat in + beta * x; with dotp (and x) also block partials of
+// Re <dotp, out> into I.part (m5_dot_blocks of them)
 template <class T>
 void m5(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int mat, T alpha = 1,
-        const typename Prec<T>::slot* x = nullptr, T beta = 0)
+        const typename Prec<T>::slot* x = nullptr, T beta = 0, const typename Prec<T>::slot* dotp = nullptr)
 {
   qdwf::M5Args<T> a;
+  a.dotp = dotp;
+  a.part = I.part;
   a.geo = I.geo;
   a.ls = I.ls;
   a.alpha = alpha;
   a.beta = beta;
   a.mat = Prec<T>::mat(I, mat);
+  a.real_mat = I.real_mat[mat];
   a.in = in;
   a.out = out;
   a.x = x;
@@ -301,12 +309,14 @@ template <>
 struct Tmp<double> {
   static double2* t1(Inv& I) { return W(I, W_OP_T1); }
   static double2* t2(Inv& I) { return W(I, W_OP_T2); }
+  static double2* t3(Inv& I) { return W(I, W_OP_T3); }
   static double2* y(Inv& I) { return W(I, W_OP_Y); }
 };
 template <>
 struct Tmp<float> {
   static float4* t1(Inv& I) { return V(I, V_T1); }
   static float4* t2(Inv& I) { return V(I, V_T2); }
+  static float4* t3(Inv& I) { return V(I, V_T3); }
   static float4* y(Inv& I) { return V(I, V_Y); }
 };
 
@@ -328,6 +338,31 @@ void op_mdag_e_o(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::sl
   m5<T>(I, out, t, MAT_P_DAG);
 }
 
+// FP32 fused out = alpha * Mat (H in) + beta * x (false if not available)
+bool hop_m5_32(Inv& I, float4* out, const float4* in, int eo_out, bool dag, int mat, float alpha = 1.f,
+               const float4* x = nullptr, float beta = 0.f, const float4* dotp = nullptr)
+{
+  if (!I.fused32) return false;
+  if (dotp && !(dag && x)) return false;
+  qdwf::HopM5Args a;
+  a.dotp = dotp;
+  a.part = I.part;
+  a.geo = I.geo;
+  a.ls = I.ls;
+  a.eo_out = eo_out;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.in = in;
+  a.out = out;
+  a.x = x;
+  a.links = I.l32;
+  a.mat = Prec<float>::mat(I, mat);
+  a.real_mat = I.real_mat[mat];
+  if (!qdwf::launch_hop_m5(a, dag, I.stream)) return false;
+  check_launch();
+  return true;
+}
+
 // Mpc = 1 - H K H K
 template <class T>
 void op_mpc(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo)
@@ -335,6 +370,12 @@ void op_mpc(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* i
   auto* t1 = Tmp<T>::t1(I);
   auto* t2 = Tmp<T>::t2(I);
   m5<T>(I, t1, in, MAT_K);
+  if constexpr (std::is_same<T, float>::value) {
+    if (hop_m5_32(I, t2, t1, 3 - eo, false, MAT_K)) {
+      hop<T>(I, out, t2, eo, false, (T)-1, in, 0);
+      return;
+    }
+  }
   hop<T>(I, t2, t1, 3 - eo, false);
   m5<T>(I, t1, t2, MAT_K);
   hop<T>(I, out, t1, eo, false, (T)-1, in, 0);
@@ -346,6 +387,12 @@ void op_mpcdag(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot
 {
   auto* t1 = Tmp<T>::t1(I);
   auto* t2 = Tmp<T>::t2(I);
+  if constexpr (std::is_same<T, float>::value) {
+    if (hop_m5_32(I, t2, in, 3 - eo, true, MAT_K_DAG)) {
+      hop_m5_32(I, out, t2, eo, true, MAT_K_DAG, -1.f, in, 1.f);
+      return;
+    }
+  }
   hop<T>(I, t1, in, 3 - eo, true);
   m5<T>(I, t2, t1, MAT_K_DAG);
   hop<T>(I, t1, t2, eo, true);
@@ -359,6 +406,63 @@ void op_hermop(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot
   auto* y = Tmp<T>::y(I);
   op_mpc<T>(I, y, in, eo);
   op_mpcdag<T>(I, out, y, eo);
+}
+
+// CG form of Mpc^+ Mpc: ap = A p given t1 = K p (computed by the previous
+// CG step), with Re <p, ap> reduced in the same pass.  Returns that dot.
+template <class T>
+double hermop_t1_dot(Inv& I, typename Prec<T>::slot* ap, const typename Prec<T>::slot* p,
+                     const typename Prec<T>::slot* t1, int eo)
+{
+  auto* A = Tmp<T>::t2(I);
+  auto* B = Tmp<T>::t3(I);
+  auto* Y = Tmp<T>::y(I);
+  int64_t nparts = 0;
+  bool done = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (hop_m5_32(I, B, t1, 3 - eo, false, MAT_K)) {
+      hop<T>(I, Y, B, eo, false, (T)-1, p, 0);                   // y = p - H K H K p
+      hop_m5_32(I, B, Y, 3 - eo, true, MAT_K_DAG);               // u = K^+ H^+ y
+      hop_m5_32(I, ap, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, p);  // ap = y - K^+ H^+ u
+      nparts = (I.geo.vh + 63) / 64;
+      done = true;
+    }
+  }
+  if (!done) {
+    hop<T>(I, A, t1, 3 - eo, false);
+    m5<T>(I, B, A, MAT_K);
+    hop<T>(I, Y, B, eo, false, (T)-1, p, 0);
+    hop<T>(I, A, Y, 3 - eo, true);
+    m5<T>(I, B, A, MAT_K_DAG);
+    hop<T>(I, A, B, eo, true);
+    m5<T>(I, ap, A, MAT_K_DAG, (T)-1, Y, (T)1, p);
+    nparts = (I.geo.vh + qdwf::kM5Block - 1) / qdwf::kM5Block * qdwf::SpinorIO<T>::kSlots;
+  }
+  qdwf::launch_reduce(I.part, (int)nparts, 1, I.red, I.stream);
+  check_launch();
+  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, sizeof(double), hipMemcpyDeviceToHost, I.stream));
+  QDWF_HIP(hipStreamSynchronize(I.stream));
+  return I.red_host[0];
+}
+
+// x += alpha p (x may be null); p = r + beta p; t = K p
+template <class T>
+void cg_step(Inv& I, typename Prec<T>::slot* x, typename Prec<T>::slot* p, const typename Prec<T>::slot* r,
+             typename Prec<T>::slot* t, double alpha, double beta)
+{
+  qdwf::CgStepArgs<T> a;
+  a.geo = I.geo;
+  a.ls = I.ls;
+  a.alpha = (T)alpha;
+  a.beta = (T)beta;
+  a.mat = Prec<T>::mat(I, MAT_K);
+  a.real_mat = I.real_mat[MAT_K];
+  a.x = x;
+  a.p = p;
+  a.r = r;
+  a.t = t;
+  qdwf::launch_cg_step<T>(a, I.stream);
+  check_launch();
 }
 
 // ------------------------------------------------------------ BLAS-1
@@ -443,6 +547,7 @@ long cg_fp64(Inv& I, double2* x, const double2* b, double stop_rsd, long max_num
   double2* r = W(I, W_CG_R);
   double2* p = W(I, W_CG_P);
   double2* ap = W(I, W_CG_AP);
+  double2* t = W(I, W_CG_T);
   const int64_t n = n64(I);
   if (guess_zero) {
     copy64(I, r, b);
@@ -454,18 +559,28 @@ long cg_fp64(Inv& I, double2* x, const double2* b, double stop_rsd, long max_num
     check_launch();
   }
   copy64(I, p, r);
+  m5<double>(I, t, p, MAT_K);
   const double qnorm_in = dev_norm2(I, b);
   double qnorm_r = dev_norm2(I, r);
   for (long iter = 1; iter <= max_num_iter; ++iter) {
-    op_hermop<double>(I, ap, p, 1);
+    // alpha = |r|^2 / Re <p, A p> ; r -= alpha A p (dslash.h:365-374)
+    const double pap = hermop_t1_dot<double>(I, ap, p, t, 1);
     I.stats.hermop_fp64 += 1;
-    const double alpha = qnorm_r / dev_dot<double2>(I, p, ap, n).real();
-    const double new_qnorm_r = dev_cg_update<double2>(I, alpha, p, ap, x, r, n);
-    if (!std::isfinite(new_qnorm_r)) throw std::runtime_error("CG breakdown: non-finite residual (FP64)");
-    if (new_qnorm_r <= qnorm_in * stop_rsd * stop_rsd) return iter;
-    const double beta = new_qnorm_r / qnorm_r;
-    qdwf::launch_xpay<double2>(r, beta, p, n, I.stream);
+    const double alpha = qnorm_r / pap;
+    qdwf::launch_axpy_norm<double2>(alpha, ap, r, n, I.part, I.red, I.stream);
     check_launch();
+    QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, sizeof(double), hipMemcpyDeviceToHost, I.stream));
+    sync(I);
+    const double new_qnorm_r = I.red_host[0];
+    if (!std::isfinite(new_qnorm_r)) throw std::runtime_error("CG breakdown: non-finite residual (FP64)");
+    if (new_qnorm_r <= qnorm_in * stop_rsd * stop_rsd) {
+      qdwf::launch_axpy<double2>(alpha, p, x, n, I.stream);
+      check_launch();
+      return iter;
+    }
+    // x += alpha p ; p = r + beta p ; t = K p (dslash.h:369-392)
+    const double beta = new_qnorm_r / qnorm_r;
+    cg_step<double>(I, x, p, r, t, alpha, beta);
     qnorm_r = new_qnorm_r;
   }
   return max_num_iter + 1;
@@ -484,16 +599,19 @@ long cg_mixed(Inv& I, double2* x64, const double2* b64, double stop_rsd, long ma
   float4* p = V(I, V_P);
   float4* ap = V(I, V_AP);
   float4* x = V(I, V_X);
+  float4* t = V(I, V_T);
   const int64_t n = n32(I), nd = n64(I);
   const double b2 = dev_norm2(I, b64);
   const double target = stop_rsd * stop_rsd * b2;
-  // r64 = b - A x64
-  op_hermop<double>(I, ax64, x64, 1);
-  I.stats.hermop_fp64 += 1;
-  copy64(I, r64, b64);
-  qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
-  check_launch();
-  double rr = dev_norm2(I, r64);
+  auto true_residual = [&]() {
+    op_hermop<double>(I, ax64, x64, 1);
+    I.stats.hermop_fp64 += 1;
+    copy64(I, r64, b64);
+    qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
+    check_launch();
+    return dev_norm2(I, r64);
+  };
+  double rr = true_residual();
   if (rr <= target) {
     *true_rel = std::sqrt(rr / b2);
     return 0;
@@ -502,28 +620,28 @@ long cg_mixed(Inv& I, double2* x64, const double2* b64, double stop_rsd, long ma
   check_launch();
   QDWF_HIP(hipMemcpyAsync(p, r, field32_bytes(I), hipMemcpyDeviceToDevice, I.stream));
   zero32(I, x);
+  m5<float>(I, t, p, MAT_K);
   double maxrr = rr;
   const double d2 = I.delta * I.delta;
   long iter = 0;
   while (iter < max_total_iter) {
-    op_hermop<float>(I, ap, p, 1);
+    const double pap = hermop_t1_dot<float>(I, ap, p, t, 1);
     I.stats.hermop_low += 1;
-    const double pap = dev_dot<float4>(I, p, ap, n).real();
     const double alpha = rr / pap;
-    double rr_new = dev_cg_update<float4>(I, alpha, p, ap, x, r, n);
+    qdwf::launch_axpy_norm<float4>(alpha, ap, r, n, I.part, I.red, I.stream);
+    check_launch();
+    QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, sizeof(double), hipMemcpyDeviceToHost, I.stream));
+    sync(I);
+    double rr_new = I.red_host[0];
     ++iter;
     if (!std::isfinite(rr_new)) throw std::runtime_error("CG breakdown: non-finite residual (FP32)");
     if (rr_new <= d2 * maxrr || rr_new <= target) {
-      // reliable update: x64 += x32, r64 = b - A x64 (FP64)
+      // reliable update: x64 += x32 + alpha p, r64 = b - A x64 in FP64
+      qdwf::launch_axpy<float4>(alpha, p, x, n, I.stream);
       qdwf::launch_f2d(x, x64, I.ls, I.geo.stride, 1, I.stream);
       check_launch();
       zero32(I, x);
-      op_hermop<double>(I, ax64, x64, 1);
-      I.stats.hermop_fp64 += 1;
-      copy64(I, r64, b64);
-      qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
-      check_launch();
-      const double rr_true = dev_norm2(I, r64);
+      const double rr_true = true_residual();
       I.stats.reliable_updates += 1;
       if (rr_true <= target) {
         *true_rel = std::sqrt(rr_true / b2);
@@ -533,21 +651,16 @@ long cg_mixed(Inv& I, double2* x64, const double2* b64, double stop_rsd, long ma
       check_launch();
       rr_new = rr_true;
       maxrr = rr_true;
+      cg_step<float>(I, nullptr, p, r, t, 0.0, rr_new / rr);
+    } else {
+      cg_step<float>(I, x, p, r, t, alpha, rr_new / rr);
     }
-    const double beta = rr_new / rr;
-    qdwf::launch_xpay<float4>(r, beta, p, n, I.stream);
-    check_launch();
     rr = rr_new;
   }
   // not converged: fold in what we have
   qdwf::launch_f2d(x, x64, I.ls, I.geo.stride, 1, I.stream);
   check_launch();
-  op_hermop<double>(I, ax64, x64, 1);
-  I.stats.hermop_fp64 += 1;
-  copy64(I, r64, b64);
-  qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
-  check_launch();
-  *true_rel = std::sqrt(dev_norm2(I, r64) / b2);
+  *true_rel = std::sqrt(true_residual() / b2);
   return max_total_iter + 1;
 }
 
@@ -830,6 +943,12 @@ void build_matrices(Inv& I)
     for (int m = 0; m < MAT_COUNT; ++m)
       std::memcpy(&all[((size_t)m * 2 + chi) * ls * ls], list[m].data(), sizeof(cd) * ls * ls);
   }
+  for (int m = 0; m < MAT_COUNT; ++m) {
+    bool real = true;
+    for (size_t i = (size_t)m * 2 * ls * ls; i < (size_t)(m + 1) * 2 * ls * ls; ++i)
+      if (all[i].imag() != 0.0) real = false;
+    I.real_mat[m] = real;
+  }
   std::vector<cplx<double>> h64(all.size());
   std::vector<cplx<float>> h32(all.size());
   for (size_t i = 0; i < all.size(); ++i) {
@@ -962,7 +1081,15 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     // hop launch shape: QLAT_DWF_HOP_SPT32 / QLAT_DWF_HOP_SPT64 (see launch_hop)
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT32")) I->hop_spt32 = std::atoi(e);
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT64")) I->hop_spt64 = std::atoi(e);
-    I->part = (double*)dev_alloc(qdwf::kMaxBlasGrid * 4 * sizeof(double));
+    if (const char* e = std::getenv("QLAT_DWF_FUSED")) I->fused32 = std::atoi(e) != 0;
+    {
+      // partial sums: BLAS grids, one per 64-site block (fused kernels), or
+      // one per (256-site block, slot) (s-matrix kernel)
+      int64_t np = qdwf::kMaxBlasGrid * 4;
+      np = std::max<int64_t>(np, (I->geo.vh + 63) / 64 + 16);
+      np = std::max<int64_t>(np, (I->geo.vh + qdwf::kM5Block - 1) / qdwf::kM5Block * 12 + 16);
+      I->part = (double*)dev_alloc(np * sizeof(double));
+    }
     I->red = (double*)dev_alloc(8 * sizeof(double));
     QDWF_HIP(hipHostMalloc((void**)&I->red_host, 8 * sizeof(double), hipHostMallocDefault));
     *out = I;
@@ -1214,6 +1341,29 @@ int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const doub
     }
     sync(*inv);
     *result = ratio;
+  });
+}
+
+int qlat_dwf_multiply_hermop_sym2_fp32(qlat_dwf_inverter* inv, double* out, const double* in, int eo)
+{
+  return guarded([&] {
+    check_inv(inv);
+    check_eo(eo);
+    QDWF_REQUIRE(out && in, "null field pointer");
+    Inv& I = *inv;
+    const size_t bytes = half_aos_bytes(I);
+    const double2* d_in = upload(I, 0, in, bytes);
+    double2* a = W(I, W_IN_E);
+    double2* b = W(I, W_OUT_E);
+    qdwf::launch_pack5(d_in, a, I.geo, I.ls, eo, 0, I.stream);
+    qdwf::launch_d2f(a, V(I, V_R), I.ls, I.geo.stride, I.stream);
+    check_launch();
+    op_hermop<float>(I, V(I, V_AP), V(I, V_R), eo);
+    qdwf::launch_f2d(V(I, V_AP), b, I.ls, I.geo.stride, 0, I.stream);
+    double2* d_out = stage(I, 1, bytes);
+    qdwf::launch_unpack5(b, d_out, I.geo, I.ls, eo, 0, I.stream);
+    check_launch();
+    download(I, out, d_out, bytes);
   });
 }
 

@@ -31,15 +31,56 @@ struct M5Args {
   T alpha;            // out = alpha * Mat in + beta * x
   T beta;
   const cplx<T>* mat; // [2][ls][ls] (device)
+  bool real_mat;      // all entries real: skip the imaginary parts
   const void* in;
   void* out;
   const void* x;      // optional
+  const void* dotp;   // optional (with x): part[block] = Re <dotp, out>
+  double* part;
+};
+
+// x += alpha p (x optional); p = r + beta p; t = Mat p
+template <class T>
+struct CgStepArgs {
+  LatGeo geo;
+  int ls;
+  T alpha, beta;
+  const cplx<T>* mat;
+  bool real_mat;
+  void* x;
+  void* p;
+  const void* r;
+  void* t;
+};
+
+// FP32 fused hop + s-matrix: out = alpha * Mat (H in) + beta * x
+struct HopM5Args {
+  LatGeo geo;
+  int ls;
+  int eo_out;
+  float alpha;
+  float beta;
+  const void* in;
+  void* out;
+  const void* x;  // optional
+  const void* links;
+  const cplx<float>* mat;  // [2][ls][ls]
+  bool real_mat;
+  const void* dotp;  // optional (dag xpay form): part[block] = Re <dotp, out>
+  double* part;
 };
 
 template <class T>
 void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st);
+// returns false (nothing launched) when ls is too large for the fused kernel
+bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st);
 template <class T>
 void launch_m5(const M5Args<T>& a, hipStream_t st);
+template <class T>
+void launch_cg_step(const CgStepArgs<T>& a, hipStream_t st);
+template <class S>
+void launch_axpy_norm(double a, const S* ap, S* r, int64_t n, double* part, double* out, hipStream_t st);
+void launch_reduce(const double* part, int nblk, int nv, double* out, hipStream_t st);
 
 template <class S>
 void launch_norm2(const S* a, int64_t n, double* part, double* out, hipStream_t st);

@@ -204,6 +204,158 @@ void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st)
 template void launch_hop<double>(const HopArgs<double>&, bool, int, hipStream_t);
 template void launch_hop<float>(const HopArgs<float>&, bool, int, hipStream_t);
 
+// ============================================================ hop + M5
+// Fused out = alpha * Mat (H in) + beta * x for FP32 fields, where Mat is a
+// dense Ls x Ls per-chirality s-matrix (the K = P M_ee^-1 of the Schur
+// complement, or its adjoint).  Block = 64 output half-sites x Ls waves:
+// phase 1, wave s computes (H in)(., s) with the block's links staged in
+// LDS and parks it in LDS; phase 2, thread (site, slot) applies Mat along s
+// and writes the result (one pass over the output instead of two).
+template <int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT>
+__global__ void __launch_bounds__(64 * LSMAX) k_hop_m5(HopM5Args a)
+{
+  __shared__ float2 sl[72][64];
+  __shared__ float4 hr[LSMAX][6][64];
+  const int64_t lb = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t h0 = lb * 64;
+  const int pslot = (a.eo_out == 2) ? 0 : 1;
+  const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
+  for (int i = threadIdx.x; i < 72 * 64; i += blockDim.x) sl[i >> 6][i & 63] = links[i];
+  __syncthreads();
+  const int ls = a.ls;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t h = h0 + lane;
+  const bool active = h < a.geo.vh;
+  if (active && w < ls) {
+    int x[4];
+    half_coord(a.geo, h, a.eo_out, x);
+    int64_t nb[8];
+    neighbours(a.geo, x, nb);
+    const float4* __restrict__ in = (const float4*)a.in;
+    constexpr int SF = DAG ? 1 : -1;
+    constexpr int SB = -SF;
+    cplx<float> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = cmk<float>(0, 0);
+#define QDWF_HOPM5_DIR(D, MU, SG)                                   \
+  {                                                                 \
+    cplx<float> W[9];                                               \
+    _Pragma("unroll") for (int e = 0; e < 9; ++e)                   \
+    {                                                               \
+      const float2 q = sl[(D) * 9 + e][lane];                       \
+      W[e] = cmk<float>(q.x, q.y);                                  \
+    }                                                               \
+    cplx<float> psi[12];                                            \
+    SpinorIO<float>::load(in, ls, w, nb[D], psi);                   \
+    hop_term<MU, SG>(W, psi, acc);                                  \
+  }
+    QDWF_HOPM5_DIR(0, 0, SF)
+    QDWF_HOPM5_DIR(1, 1, SF)
+    QDWF_HOPM5_DIR(2, 2, SF)
+    QDWF_HOPM5_DIR(3, 3, SF)
+    QDWF_HOPM5_DIR(4, 0, SB)
+    QDWF_HOPM5_DIR(5, 1, SB)
+    QDWF_HOPM5_DIR(6, 2, SB)
+    QDWF_HOPM5_DIR(7, 3, SB)
+#undef QDWF_HOPM5_DIR
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      hr[w][k][lane] = make_float4(-0.5f * acc[2 * k].re, -0.5f * acc[2 * k].im, -0.5f * acc[2 * k + 1].re,
+                                   -0.5f * acc[2 * k + 1].im);
+  }
+  __syncthreads();
+  double dot = 0.0;  // Re <dotp, out> over this thread's outputs (DOT)
+  for (int kp = w; active && kp < 6; kp += ls) {  // wave-uniform slot
+    const int chi = (kp < 3) ? 0 : 1;
+    const cplx<float>* __restrict__ M = a.mat + chi * ls * ls;
+    float4 v[LSMAX];
+#pragma unroll
+    for (int t = 0; t < LSMAX; ++t)
+      if (t < ls) v[t] = hr[t][kp][lane];
+    for (int s = 0; s < ls; ++s) {
+      float r0 = 0.f, i0 = 0.f, r1 = 0.f, i1 = 0.f;
+#pragma unroll
+      for (int t = 0; t < LSMAX; ++t) {
+        if (t < ls) {
+          const cplx<float> m = M[s * ls + t];
+          r0 += m.re * v[t].x;
+          i0 += m.re * v[t].y;
+          r1 += m.re * v[t].z;
+          i1 += m.re * v[t].w;
+          if (!REAL) {
+            r0 -= m.im * v[t].y;
+            i0 += m.im * v[t].x;
+            r1 -= m.im * v[t].w;
+            i1 += m.im * v[t].z;
+          }
+        }
+      }
+      const int64_t o = sidx<6>(ls, s, kp, h);
+      float4 res = make_float4(a.alpha * r0, a.alpha * i0, a.alpha * r1, a.alpha * i1);
+      if (XPAY) {
+        const float4 xv = ((const float4*)a.x)[o];
+        res.x += a.beta * xv.x;
+        res.y += a.beta * xv.y;
+        res.z += a.beta * xv.z;
+        res.w += a.beta * xv.w;
+      }
+      store_nt(&((float4*)a.out)[o], res);
+      if (DOT) {
+        const float4 pv = ((const float4*)a.dotp)[o];
+        dot += (double)pv.x * res.x + (double)pv.y * res.y + (double)pv.z * res.z + (double)pv.w * res.w;
+      }
+    }
+  }
+  if (DOT) {
+    __shared__ double red[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
+    if (lane == 0) red[w] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+      a.part[lb] = t;
+    }
+  }
+}
+
+template <int LSMAX>
+static void launch_hop_m5_ls(const HopM5Args& a, bool dag, hipStream_t st)
+{
+  const dim3 grid((unsigned)((a.geo.vh + 63) / 64));
+  const dim3 blk(64 * a.ls);
+  const bool xpay = a.x != nullptr;
+#define QDWF_L(D, X, R) hipLaunchKernelGGL((k_hop_m5<LSMAX, D, X, R, false>), grid, blk, 0, st, a)
+  if (a.dotp) {
+    // only the x - Mat H in form carries the CG dot product
+    if (dag && xpay && a.real_mat) hipLaunchKernelGGL((k_hop_m5<LSMAX, true, true, true, true>), grid, blk, 0, st, a);
+    else if (dag && xpay) hipLaunchKernelGGL((k_hop_m5<LSMAX, true, true, false, true>), grid, blk, 0, st, a);
+    return;
+  }
+  if (a.real_mat) {
+    if (!dag && !xpay) QDWF_L(false, false, true);
+    if (!dag && xpay) QDWF_L(false, true, true);
+    if (dag && !xpay) QDWF_L(true, false, true);
+    if (dag && xpay) QDWF_L(true, true, true);
+  } else {
+    if (!dag && !xpay) QDWF_L(false, false, false);
+    if (!dag && xpay) QDWF_L(false, true, false);
+    if (dag && !xpay) QDWF_L(true, false, false);
+    if (dag && xpay) QDWF_L(true, true, false);
+  }
+#undef QDWF_L
+}
+
+bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st)
+{
+  if (a.ls > 16) return false;
+  if (a.ls <= 8) launch_hop_m5_ls<8>(a, dag, st);
+  else if (a.ls <= 12) launch_hop_m5_ls<12>(a, dag, st);
+  else launch_hop_m5_ls<16>(a, dag, st);
+  return true;
+}
+
 // ================================================================== M5
 // out = alpha * (Mat_chi in) + beta * x, where Mat_chi is a dense Ls x Ls
 // complex matrix per chirality (chi = 0: spins 0,1; chi = 1: spins 2,3).
@@ -212,13 +364,14 @@ template void launch_hop<float>(const HopArgs<float>&, bool, int, hipStream_t);
 // M_eo (dslash.cpp:961-1003) and its adjoint (dslash.cpp:1005-1060), and
 // their products.  Thread = (half-site, 16-byte slot); the slot index is
 // blockIdx.y so chirality and matrix reads are wave-uniform.
-template <class T, int LSMAX, bool XPAY>
+template <class T, int LSMAX, bool XPAY, bool REAL, bool DOT>
 __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
 {
   typedef typename SpinorIO<T>::slot_t slot_t;
   const int64_t h = (int64_t)blockIdx.x * kM5Block + threadIdx.x;
-  if (h >= a.geo.vh) return;
   const int slot = blockIdx.y;
+  double dot = 0.0;
+  if (h < a.geo.vh) {
   constexpr int NS = SpinorIO<T>::kSlots;
   const int chi = (slot < NS / 2) ? 0 : 1;
   const int ls = a.ls;
@@ -249,7 +402,14 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
       if (t < ls) {
         const cplx<T> m = M[s * ls + t];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) r[c] = cfma(r[c], m, v[t][c]);
+        for (int c = 0; c < NC; ++c) {
+          if (REAL) {
+            r[c].re += m.re * v[t][c].re;
+            r[c].im += m.re * v[t][c].im;
+          } else {
+            r[c] = cfma(r[c], m, v[t][c]);
+          }
+        }
       }
     }
 #pragma unroll
@@ -278,15 +438,152 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
       o.w = r[1].im;
     }
     out[off] = o;
+    if (DOT) {
+      const slot_t q = ((const slot_t*)a.dotp)[off];
+      if constexpr (NC == 1) {
+        dot += (double)q.x * o.x + (double)q.y * o.y;
+      } else {
+        dot += (double)q.x * o.x + (double)q.y * o.y + (double)q.z * o.z + (double)q.w * o.w;
+      }
+    }
+  }
+  }
+  if (DOT) {
+    double v[1] = {dot};
+    // block_reduce_store writes part[blockIdx.x]; offset rows by blockIdx.y
+    __shared__ double sh[kM5Block / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[0] += __shfl_down(v[0], o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v[0];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int i = 0; i < kM5Block / 64; ++i) t += sh[i];
+      a.part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
   }
 }
+
+// CG direction update fused with the first s-matrix of the next operator
+// application:  x += alpha p (if UPDX);  p = r + beta p;  t = Mat p.
+// Thread = (half-site, slot) looping over s, like k_m5.
+template <class T, int LSMAX, bool UPDX, bool REAL>
+__global__ void __launch_bounds__(kM5Block) k_cg_step(CgStepArgs<T> a)
+{
+  typedef typename SpinorIO<T>::slot_t slot_t;
+  const int64_t h = (int64_t)blockIdx.x * kM5Block + threadIdx.x;
+  if (h >= a.geo.vh) return;
+  const int slot = blockIdx.y;
+  constexpr int NS = SpinorIO<T>::kSlots;
+  constexpr int NC = (NS == 12) ? 1 : 2;
+  const int chi = (slot < NS / 2) ? 0 : 1;
+  const int ls = a.ls;
+  const cplx<T>* __restrict__ M = a.mat + (int64_t)chi * ls * ls;
+  slot_t* __restrict__ x = (slot_t*)a.x;
+  slot_t* __restrict__ p = (slot_t*)a.p;
+  const slot_t* __restrict__ r = (const slot_t*)a.r;
+  const T al = a.alpha, be = a.beta;
+  cplx<T> v[LSMAX][NC];
+#pragma unroll
+  for (int t = 0; t < LSMAX; ++t) {
+    if (t < ls) {
+      const int64_t o = sidx<NS>(ls, t, slot, h);
+      const slot_t pv = p[o];
+      const slot_t rv = r[o];
+      if (UPDX) {
+        slot_t xv = x[o];
+        xv.x += al * pv.x;
+        xv.y += al * pv.y;
+        if constexpr (NC == 2) {
+          xv.z += al * pv.z;
+          xv.w += al * pv.w;
+        }
+        x[o] = xv;
+      }
+      slot_t pn;
+      pn.x = be * pv.x + rv.x;
+      pn.y = be * pv.y + rv.y;
+      if constexpr (NC == 2) {
+        pn.z = be * pv.z + rv.z;
+        pn.w = be * pv.w + rv.w;
+      }
+      p[o] = pn;
+      v[t][0] = cmk<T>(pn.x, pn.y);
+      if constexpr (NC == 2) v[t][NC - 1] = cmk<T>(pn.z, pn.w);
+    }
+  }
+  slot_t* __restrict__ out = (slot_t*)a.t;
+  for (int s = 0; s < ls; ++s) {
+    cplx<T> res[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) res[c] = cmk<T>(0, 0);
+#pragma unroll
+    for (int t = 0; t < LSMAX; ++t) {
+      if (t < ls) {
+        const cplx<T> m = M[s * ls + t];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          if (REAL) {
+            res[c].re += m.re * v[t][c].re;
+            res[c].im += m.re * v[t][c].im;
+          } else {
+            res[c] = cfma(res[c], m, v[t][c]);
+          }
+        }
+      }
+    }
+    slot_t o;
+    o.x = res[0].re;
+    o.y = res[0].im;
+    if constexpr (NC == 2) {
+      o.z = res[NC - 1].re;
+      o.w = res[NC - 1].im;
+    }
+    out[sidx<NS>(ls, s, slot, h)] = o;
+  }
+}
+
+template <class T, int LSMAX>
+static void launch_cg_step_ls(const CgStepArgs<T>& a, hipStream_t st)
+{
+  const dim3 grid((unsigned)((a.geo.vh + kM5Block - 1) / kM5Block), SpinorIO<T>::kSlots);
+  const bool updx = a.x != nullptr;
+  if (a.real_mat) {
+    if (updx) hipLaunchKernelGGL((k_cg_step<T, LSMAX, true, true>), grid, dim3(kM5Block), 0, st, a);
+    else hipLaunchKernelGGL((k_cg_step<T, LSMAX, false, true>), grid, dim3(kM5Block), 0, st, a);
+  } else {
+    if (updx) hipLaunchKernelGGL((k_cg_step<T, LSMAX, true, false>), grid, dim3(kM5Block), 0, st, a);
+    else hipLaunchKernelGGL((k_cg_step<T, LSMAX, false, false>), grid, dim3(kM5Block), 0, st, a);
+  }
+}
+
+template <class T>
+void launch_cg_step(const CgStepArgs<T>& a, hipStream_t st)
+{
+  if (a.ls <= 8) launch_cg_step_ls<T, 8>(a, st);
+  else if (a.ls <= 12) launch_cg_step_ls<T, 12>(a, st);
+  else if (a.ls <= 16) launch_cg_step_ls<T, 16>(a, st);
+  else if (a.ls <= 24) launch_cg_step_ls<T, 24>(a, st);
+  else launch_cg_step_ls<T, 32>(a, st);
+}
+
+template void launch_cg_step<double>(const CgStepArgs<double>&, hipStream_t);
+template void launch_cg_step<float>(const CgStepArgs<float>&, hipStream_t);
 
 template <class T, int LSMAX>
 static void launch_m5_ls(const M5Args<T>& a, hipStream_t st)
 {
   const dim3 grid((unsigned)((a.geo.vh + kM5Block - 1) / kM5Block), SpinorIO<T>::kSlots);
-  if (a.x) hipLaunchKernelGGL((k_m5<T, LSMAX, true>), grid, dim3(kM5Block), 0, st, a);
-  else hipLaunchKernelGGL((k_m5<T, LSMAX, false>), grid, dim3(kM5Block), 0, st, a);
+  if (a.dotp) {  // x-pay form with the CG dot product
+    if (a.real_mat) hipLaunchKernelGGL((k_m5<T, LSMAX, true, true, true>), grid, dim3(kM5Block), 0, st, a);
+    else hipLaunchKernelGGL((k_m5<T, LSMAX, true, false, true>), grid, dim3(kM5Block), 0, st, a);
+  } else if (a.real_mat) {
+    if (a.x) hipLaunchKernelGGL((k_m5<T, LSMAX, true, true, false>), grid, dim3(kM5Block), 0, st, a);
+    else hipLaunchKernelGGL((k_m5<T, LSMAX, false, true, false>), grid, dim3(kM5Block), 0, st, a);
+  } else {
+    if (a.x) hipLaunchKernelGGL((k_m5<T, LSMAX, true, false, false>), grid, dim3(kM5Block), 0, st, a);
+    else hipLaunchKernelGGL((k_m5<T, LSMAX, false, false, false>), grid, dim3(kM5Block), 0, st, a);
+  }
 }
 
 template <class T>
@@ -429,15 +726,37 @@ __global__ void __launch_bounds__(kBlasBlock) k_cg_update(double a, const S* __r
   block_reduce_store<1>(v, part);
 }
 
-__global__ void k_reduce_final(const double* __restrict__ part, int nblk, int nv, double* out)
+// Sum nblk partial rows of nv values (nv <= 2) with one 256-thread block;
+// the summation order depends only on nblk, so results are reproducible.
+// r -= a ap ; partial |r|^2
+template <class S>
+__global__ void __launch_bounds__(kBlasBlock) k_axpy_norm(double a, const S* __restrict__ ap, S* __restrict__ r,
+                                                          int64_t n, double* part)
 {
-  // one thread per value; partial sums added in block order (deterministic)
-  const int i = threadIdx.x;
-  if (i < nv) {
-    double t = 0;
-    for (int b = 0; b < nblk; ++b) t += part[(int64_t)b * nv + i];
-    out[i] = t;
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
+    const S rn = slot_axpy(-a, ap[i], r[i]);
+    r[i] = rn;
+    v[0] += slot_norm2(rn);
   }
+  block_reduce_store<1>(v, part);
+}
+
+__global__ void __launch_bounds__(256) k_reduce_final(const double* __restrict__ part, int nblk, int nv, double* out)
+{
+  __shared__ double sh[4][2];
+  double v[2] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < nblk; b += 256)
+    for (int i = 0; i < nv; ++i) v[i] += part[(int64_t)b * nv + i];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int i = 0; i < 2; ++i) {
+    double t = v[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+    if (lane == 0) sh[wid][i] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) out[threadIdx.x] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
 }
 
 // FP64 blocked field (12 double2 slots per 5D site) <-> FP32 blocked field
@@ -499,14 +818,14 @@ void launch_norm2(const S* a, int64_t n, double* part, double* out, hipStream_t 
 {
   const int g = blas_grid(n);
   hipLaunchKernelGGL(k_norm2<S>, dim3(g), dim3(kBlasBlock), 0, st, a, n, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, part, g, 1, out);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, g, 1, out);
 }
 template <class S>
 void launch_dot(const S* a, const S* b, int64_t n, double* part, double* out, hipStream_t st)
 {
   const int g = blas_grid(n);
   hipLaunchKernelGGL(k_dot<S>, dim3(g), dim3(kBlasBlock), 0, st, a, b, n, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, part, g, 2, out);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, g, 2, out);
 }
 template <class S>
 void launch_axpy(double a, const S* x, S* y, int64_t n, hipStream_t st)
@@ -524,13 +843,27 @@ void launch_cg_update(double a, const S* p, const S* ap, S* x, S* r, int64_t n, 
 {
   const int g = blas_grid(n);
   hipLaunchKernelGGL(k_cg_update<S>, dim3(g), dim3(kBlasBlock), 0, st, a, p, ap, x, r, n, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, part, g, 1, out);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, g, 1, out);
 }
 
 void launch_copy(const float4* x, float4* y, int64_t n, hipStream_t st)
 {
   hipLaunchKernelGGL(k_copy<float4>, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, x, y, n);
 }
+
+template <class S>
+void launch_axpy_norm(double a, const S* ap, S* r, int64_t n, double* part, double* out, hipStream_t st)
+{
+  const int g = blas_grid(n);
+  hipLaunchKernelGGL(k_axpy_norm<S>, dim3(g), dim3(kBlasBlock), 0, st, a, ap, r, n, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, g, 1, out);
+}
+void launch_reduce(const double* part, int nblk, int nv, double* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, nblk, nv, out);
+}
+template void launch_axpy_norm<double2>(double, const double2*, double2*, int64_t, double*, double*, hipStream_t);
+template void launch_axpy_norm<float4>(double, const float4*, float4*, int64_t, double*, double*, hipStream_t);
 
 template void launch_norm2<double2>(const double2*, int64_t, double*, double*, hipStream_t);
 template void launch_norm2<float4>(const float4*, int64_t, double*, double*, hipStream_t);

@@ -225,6 +225,11 @@ def multiply_hermop_sym2(out, in_, inv):
     return _apply_half("qlat_dwf_multiply_hermop_sym2", out, in_, inv, in_.eo, in_.eo)
 
 
+def multiply_hermop_sym2_fp32(out, in_, inv):
+    """Mpc^+ Mpc evaluated in FP32 (the mixed solver's inner operator)."""
+    return _apply_half("qlat_dwf_multiply_hermop_sym2_fp32", out, in_, inv, in_.eo, in_.eo)
+
+
 def _apply_full(name, out, in_, inv):
     assert isinstance(in_, FermionField5d) and in_.eo == 0
     src = np.ascontiguousarray(in_.data, dtype=np.complex128)

@@ -279,3 +279,19 @@ def test_tiled_device_order_matches_oracle(monkeypatch):
             ref, _ = orc.invert_4d(src.data)
             assert inv.stats()["true_residual"] <= 1e-8
             assert rel(sol.data, ref) < 1e-6, (by, spt)
+
+
+@pytest.mark.parametrize("fused", ["1", "0"])
+@pytest.mark.parametrize("ls,omega", [(8, None), (12, "z"), (16, None)])
+def test_fp32_hermop_matches_oracle(monkeypatch, fused, ls, omega):
+    """The FP32 operator of the mixed solver (fused hop + s-matrix kernel and
+    the unfused kernels) agrees with the FP64 oracle to FP32 accuracy."""
+    monkeypatch.setenv("QLAT_DWF_FUSED", fused)
+    site = (4, 4, 4, 8)
+    U = rand_gauge(site, 0.4, 41)
+    om = ZMOBIUS_OMEGA if omega else None
+    geo, inv, orc = make_pair(site, U, mass=0.05, ls=ls, m5=1.8, mobius_scale=2.0, omega=om)
+    for eo in (1, 2):
+        x = rand_field((geo.total_volume() // 2, ls, 12), 500 + eo)
+        got = q.multiply_hermop_sym2_fp32(None, half(geo, ls, eo, x), inv)
+        assert rel(got.data, orc.apply_half("hermop_sym2", x, eo)) < 2e-6
