@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--kernel-iters", type=int, default=50)
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the kernels and stop (for profiler runs)")
+    ap.add_argument("--profile-column", action="store_true",
+                    help="solve one propagator column with each solver and stop (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -140,6 +142,17 @@ def main():
         print(json.dumps({"hop_fp32_ms": hop32_ms, "hop_fp32_gbps": gbs32, "hop_fp64_ms": hop64_ms,
                           "hop_fp64_gbps": gbs64, "m5_fp32_ms": m5_32_ms, "hermop_fp32_ms": herm32_ms,
                           "hermop_fp64_ms": herm64_ms}), flush=True)
+        return
+
+    if args.profile_column:
+        src4 = q.FermionField4d(geo)
+        src4.data[0, 0] = 1.0
+        for solver in ("mixed", "fp64"):
+            inv.set_solver(solver)
+            t0 = time.perf_counter()
+            inv * src4
+            st = inv.stats()
+            log(f"{solver}: one column {time.perf_counter() - t0:.2f} s, iters {st['total_iter']}")
         return
 
     # point-source propagator, resident on the device

@@ -424,7 +424,7 @@ double hermop_t1_dot(Inv& I, typename Prec<T>::slot* ap, const typename Prec<T>:
       hop<T>(I, Y, B, eo, false, (T)-1, p, 0);                   // y = p - H K H K p
       hop_m5_32(I, B, Y, 3 - eo, true, MAT_K_DAG);               // u = K^+ H^+ y
       hop_m5_32(I, ap, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, p);  // ap = y - K^+ H^+ u
-      nparts = (I.geo.vh + 63) / 64;
+      nparts = qdwf::hop_m5_blocks(I.geo);
       done = true;
     }
   }
@@ -1086,7 +1086,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
       // partial sums: BLAS grids, one per 64-site block (fused kernels), or
       // one per (256-site block, slot) (s-matrix kernel)
       int64_t np = qdwf::kMaxBlasGrid * 4;
-      np = std::max<int64_t>(np, (I->geo.vh + 63) / 64 + 16);
+      np = std::max<int64_t>(np, (I->geo.vh + 31) / 32 + 16);
       np = std::max<int64_t>(np, (I->geo.vh + qdwf::kM5Block - 1) / qdwf::kM5Block * 12 + 16);
       I->part = (double*)dev_alloc(np * sizeof(double));
     }

@@ -9,6 +9,7 @@ constexpr int kHopBlock = 256;
 constexpr int kM5Block = 256;
 constexpr int kBlasBlock = 256;
 constexpr int kMaxBlasGrid = 2048;
+constexpr int kHopM5Sites = 32;  // output half-sites per fused hop + s-matrix block
 
 template <class T>
 struct HopArgs {
@@ -74,6 +75,7 @@ template <class T>
 void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st);
 // returns false (nothing launched) when ls is too large for the fused kernel
 bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st);
+int64_t hop_m5_blocks(const LatGeo& g);
 template <class T>
 void launch_m5(const M5Args<T>& a, hipStream_t st);
 template <class T>

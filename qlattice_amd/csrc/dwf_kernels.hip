@@ -207,33 +207,38 @@ template void launch_hop<float>(const HopArgs<float>&, bool, int, hipStream_t);
 // ============================================================ hop + M5
 // Fused out = alpha * Mat (H in) + beta * x for FP32 fields, where Mat is a
 // dense Ls x Ls per-chirality s-matrix (the K = P M_ee^-1 of the Schur
-// complement, or its adjoint).  Block = 64 output half-sites x Ls waves:
-// phase 1, wave s computes (H in)(., s) with the block's links staged in
-// LDS and parks it in LDS; phase 2, thread (site, slot) applies Mat along s
-// and writes the result (one pass over the output instead of two).
-template <int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT>
-__global__ void __launch_bounds__(64 * LSMAX) k_hop_m5(HopM5Args a)
+// complement, or its adjoint).  Block = SB output half-sites x Ls:
+// phase 1, thread (site, s) computes (H in)(site, s) with the block's links
+// staged in LDS and parks it in LDS; phase 2, thread (site, slot) applies Mat
+// along s, adds beta*x, writes the result once and, with DOT, accumulates
+// Re <dotp, out> for the CG.  SB = 32 keeps the LDS at 55 KB (Ls = 12) so
+// two blocks share a CU and one block's s-matrix phase overlaps the other's
+// memory-bound hop phase.
+template <int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT>
+__global__ void __launch_bounds__(SB * LSMAX) __attribute__((amdgpu_waves_per_eu(3)))
+k_hop_m5(HopM5Args a)
 {
-  __shared__ float2 sl[72][64];
-  __shared__ float4 hr[LSMAX][6][64];
+  __shared__ float2 sl[72][SB];
+  __shared__ float4 hr[LSMAX][6][SB];
   const int64_t lb = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
-  const int64_t h0 = lb * 64;
+  const int64_t h0 = lb * SB;
   const int pslot = (a.eo_out == 2) ? 0 : 1;
+  // the link block holding these SB sites (64-site link blocks)
   const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
-  for (int i = threadIdx.x; i < 72 * 64; i += blockDim.x) sl[i >> 6][i & 63] = links[i];
+  for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) sl[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
   __syncthreads();
   const int ls = a.ls;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t h = h0 + lane;
+  const int site = threadIdx.x % SB, row = threadIdx.x / SB;
+  const int64_t h = h0 + site;
   const bool active = h < a.geo.vh;
-  if (active && w < ls) {
+  if (active && row < ls) {
     int x[4];
     half_coord(a.geo, h, a.eo_out, x);
     int64_t nb[8];
     neighbours(a.geo, x, nb);
     const float4* __restrict__ in = (const float4*)a.in;
     constexpr int SF = DAG ? 1 : -1;
-    constexpr int SB = -SF;
+    constexpr int SB_ = -SF;
     cplx<float> acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = cmk<float>(0, 0);
@@ -242,117 +247,130 @@ __global__ void __launch_bounds__(64 * LSMAX) k_hop_m5(HopM5Args a)
     cplx<float> W[9];                                               \
     _Pragma("unroll") for (int e = 0; e < 9; ++e)                   \
     {                                                               \
-      const float2 q = sl[(D) * 9 + e][lane];                       \
+      const float2 q = sl[(D) * 9 + e][site];                       \
       W[e] = cmk<float>(q.x, q.y);                                  \
     }                                                               \
     cplx<float> psi[12];                                            \
-    SpinorIO<float>::load(in, ls, w, nb[D], psi);                   \
+    SpinorIO<float>::load(in, ls, row, nb[D], psi);                 \
     hop_term<MU, SG>(W, psi, acc);                                  \
   }
     QDWF_HOPM5_DIR(0, 0, SF)
     QDWF_HOPM5_DIR(1, 1, SF)
     QDWF_HOPM5_DIR(2, 2, SF)
     QDWF_HOPM5_DIR(3, 3, SF)
-    QDWF_HOPM5_DIR(4, 0, SB)
-    QDWF_HOPM5_DIR(5, 1, SB)
-    QDWF_HOPM5_DIR(6, 2, SB)
-    QDWF_HOPM5_DIR(7, 3, SB)
+    QDWF_HOPM5_DIR(4, 0, SB_)
+    QDWF_HOPM5_DIR(5, 1, SB_)
+    QDWF_HOPM5_DIR(6, 2, SB_)
+    QDWF_HOPM5_DIR(7, 3, SB_)
 #undef QDWF_HOPM5_DIR
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-      hr[w][k][lane] = make_float4(-0.5f * acc[2 * k].re, -0.5f * acc[2 * k].im, -0.5f * acc[2 * k + 1].re,
-                                   -0.5f * acc[2 * k + 1].im);
+      hr[row][k][site] = make_float4(-0.5f * acc[2 * k].re, -0.5f * acc[2 * k].im, -0.5f * acc[2 * k + 1].re,
+                                     -0.5f * acc[2 * k + 1].im);
   }
   __syncthreads();
   double dot = 0.0;  // Re <dotp, out> over this thread's outputs (DOT)
-  for (int kp = w; active && kp < 6; kp += ls) {  // wave-uniform slot
+  const int kp = row;  // phase 2: thread (site, slot)
+  if (active && kp < 6) {
     const int chi = (kp < 3) ? 0 : 1;
     const cplx<float>* __restrict__ M = a.mat + chi * ls * ls;
     float4 v[LSMAX];
+    float4 xv[XPAY ? LSMAX : 1];
+    float4 pv[DOT ? LSMAX : 1];
 #pragma unroll
-    for (int t = 0; t < LSMAX; ++t)
-      if (t < ls) v[t] = hr[t][kp][lane];
-    for (int s = 0; s < ls; ++s) {
-      float r0 = 0.f, i0 = 0.f, r1 = 0.f, i1 = 0.f;
+    for (int t = 0; t < LSMAX; ++t) {
+      if (t < ls) {
+        v[t] = hr[t][kp][site];
+        const int64_t o = sidx<6>(ls, t, kp, h);
+        if (XPAY) xv[t] = ((const float4*)a.x)[o];
+        if (DOT) pv[t] = ((const float4*)a.dotp)[o];
+      }
+    }
 #pragma unroll
-      for (int t = 0; t < LSMAX; ++t) {
-        if (t < ls) {
-          const cplx<float> m = M[s * ls + t];
-          r0 += m.re * v[t].x;
-          i0 += m.re * v[t].y;
-          r1 += m.re * v[t].z;
-          i1 += m.re * v[t].w;
-          if (!REAL) {
-            r0 -= m.im * v[t].y;
-            i0 += m.im * v[t].x;
-            r1 -= m.im * v[t].w;
-            i1 += m.im * v[t].z;
+    for (int s = 0; s < LSMAX; ++s) {
+      if (s < ls) {
+        float r0 = 0.f, i0 = 0.f, r1 = 0.f, i1 = 0.f;
+#pragma unroll
+        for (int t = 0; t < LSMAX; ++t) {
+          if (t < ls) {
+            const cplx<float> m = M[s * ls + t];
+            r0 += m.re * v[t].x;
+            i0 += m.re * v[t].y;
+            r1 += m.re * v[t].z;
+            i1 += m.re * v[t].w;
+            if (!REAL) {
+              r0 -= m.im * v[t].y;
+              i0 += m.im * v[t].x;
+              r1 -= m.im * v[t].w;
+              i1 += m.im * v[t].z;
+            }
           }
         }
-      }
-      const int64_t o = sidx<6>(ls, s, kp, h);
-      float4 res = make_float4(a.alpha * r0, a.alpha * i0, a.alpha * r1, a.alpha * i1);
-      if (XPAY) {
-        const float4 xv = ((const float4*)a.x)[o];
-        res.x += a.beta * xv.x;
-        res.y += a.beta * xv.y;
-        res.z += a.beta * xv.z;
-        res.w += a.beta * xv.w;
-      }
-      store_nt(&((float4*)a.out)[o], res);
-      if (DOT) {
-        const float4 pv = ((const float4*)a.dotp)[o];
-        dot += (double)pv.x * res.x + (double)pv.y * res.y + (double)pv.z * res.z + (double)pv.w * res.w;
+        float4 res = make_float4(a.alpha * r0, a.alpha * i0, a.alpha * r1, a.alpha * i1);
+        if (XPAY) {
+          res.x += a.beta * xv[s].x;
+          res.y += a.beta * xv[s].y;
+          res.z += a.beta * xv[s].z;
+          res.w += a.beta * xv[s].w;
+        }
+        store_nt(&((float4*)a.out)[sidx<6>(ls, s, kp, h)], res);
+        if (DOT) dot += (double)pv[s].x * res.x + (double)pv[s].y * res.y + (double)pv[s].z * res.z + (double)pv[s].w * res.w;
       }
     }
   }
   if (DOT) {
-    __shared__ double red[16];
+    __shared__ double red[(SB * LSMAX + 63) / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
-    if (lane == 0) red[w] = dot;
+    if (lane == 0) red[wv] = dot;
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
-      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+      for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) t += red[i];
       a.part[lb] = t;
     }
   }
 }
 
-template <int LSMAX>
+template <int SB, int LSMAX>
 static void launch_hop_m5_ls(const HopM5Args& a, bool dag, hipStream_t st)
 {
-  const dim3 grid((unsigned)((a.geo.vh + 63) / 64));
-  const dim3 blk(64 * a.ls);
+  const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB));
+  const dim3 blk(SB * a.ls);
   const bool xpay = a.x != nullptr;
-#define QDWF_L(D, X, R) hipLaunchKernelGGL((k_hop_m5<LSMAX, D, X, R, false>), grid, blk, 0, st, a)
+#define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5<SB, LSMAX, D, X, R, DT>), grid, blk, 0, st, a)
   if (a.dotp) {
-    // only the x - Mat H in form carries the CG dot product
-    if (dag && xpay && a.real_mat) hipLaunchKernelGGL((k_hop_m5<LSMAX, true, true, true, true>), grid, blk, 0, st, a);
-    else if (dag && xpay) hipLaunchKernelGGL((k_hop_m5<LSMAX, true, true, false, true>), grid, blk, 0, st, a);
+    // only the x - Mat H^+ in form carries the CG dot product
+    if (dag && xpay && a.real_mat) QDWF_L(true, true, true, true);
+    else if (dag && xpay) QDWF_L(true, true, false, true);
     return;
   }
   if (a.real_mat) {
-    if (!dag && !xpay) QDWF_L(false, false, true);
-    if (!dag && xpay) QDWF_L(false, true, true);
-    if (dag && !xpay) QDWF_L(true, false, true);
-    if (dag && xpay) QDWF_L(true, true, true);
+    if (!dag && !xpay) QDWF_L(false, false, true, false);
+    if (!dag && xpay) QDWF_L(false, true, true, false);
+    if (dag && !xpay) QDWF_L(true, false, true, false);
+    if (dag && xpay) QDWF_L(true, true, true, false);
   } else {
-    if (!dag && !xpay) QDWF_L(false, false, false);
-    if (!dag && xpay) QDWF_L(false, true, false);
-    if (dag && !xpay) QDWF_L(true, false, false);
-    if (dag && xpay) QDWF_L(true, true, false);
+    if (!dag && !xpay) QDWF_L(false, false, false, false);
+    if (!dag && xpay) QDWF_L(false, true, false, false);
+    if (dag && !xpay) QDWF_L(true, false, false, false);
+    if (dag && xpay) QDWF_L(true, true, false, false);
   }
 #undef QDWF_L
 }
 
+// Number of partial sums a DOT launch writes (one per block).
+int64_t hop_m5_blocks(const LatGeo& g) { return (g.vh + kHopM5Sites - 1) / kHopM5Sites; }
+
 bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st)
 {
-  if (a.ls > 16) return false;
-  if (a.ls <= 8) launch_hop_m5_ls<8>(a, dag, st);
-  else if (a.ls <= 12) launch_hop_m5_ls<12>(a, dag, st);
-  else launch_hop_m5_ls<16>(a, dag, st);
+  // phase 2 needs >= 6 rows; whole waves (SB * ls multiple of 64) for the
+  // block reduction
+  if (a.ls < 6 || a.ls > 16 || (kHopM5Sites * a.ls) % 64 != 0) return false;
+  if (a.ls <= 8) launch_hop_m5_ls<kHopM5Sites, 8>(a, dag, st);
+  else if (a.ls <= 12) launch_hop_m5_ls<kHopM5Sites, 12>(a, dag, st);
+  else launch_hop_m5_ls<kHopM5Sites, 16>(a, dag, st);
   return true;
 }
 
