@@ -182,6 +182,12 @@ struct qlat_dwf_inverter {
   double* red = nullptr;     // device reduction results
   double* red_host = nullptr;  // pinned
 
+  // multi-RHS: FP32 stacks of `mcap` fields and per-column FP64 fields
+  int rhs_tile = 12;  // columns per multi-RHS solve (QLAT_DWF_RHS_TILE)
+  int mcap = 0;
+  float4* m32[8] = {};
+  std::vector<double2*> col64;  // 3 per column: b, x, out_e_p
+
   qlat_dwf_solve_stats stats{};
 };
 
@@ -221,6 +227,39 @@ float4* V(Inv& I, int id)
   }
   return I.w32[id];
 }
+enum Multi32 { M_R, M_P, M_AP, M_X, M_T, M_A, M_B, M_Y };
+
+// FP32 stack buffer `id` with room for `ncol` fields.
+float4* MV(Inv& I, int id, int ncol)
+{
+  if (ncol > I.mcap) {
+    QDWF_HIP(hipStreamSynchronize(I.stream));
+    for (auto*& p : I.m32)
+      if (p) {
+        QDWF_HIP(hipFree(p));
+        p = nullptr;
+      }
+    I.mcap = ncol;
+  }
+  if (!I.m32[id]) {
+    I.m32[id] = (float4*)dev_alloc(field32_bytes(I) * I.mcap);
+    QDWF_HIP(hipMemsetAsync(I.m32[id], 0, field32_bytes(I) * I.mcap, I.stream));
+  }
+  return I.m32[id];
+}
+
+// Per-column FP64 field: kind 0 = odd-prec source b, 1 = solution x, 2 = out_e_p.
+double2* C64(Inv& I, int col, int kind)
+{
+  const size_t k = (size_t)col * 3 + kind;
+  if (I.col64.size() <= k) I.col64.resize(k + 1, nullptr);
+  if (!I.col64[k]) {
+    I.col64[k] = (double2*)dev_alloc(field64_bytes(I));
+    QDWF_HIP(hipMemsetAsync(I.col64[k], 0, field64_bytes(I), I.stream));
+  }
+  return I.col64[k];
+}
+
 size_t stage_size(const Inv& I);
 
 // Two device staging buffers for host-pointer entry points, each large
@@ -260,15 +299,26 @@ struct Prec<float> {
   static int64_t n(const Inv& I) { return n32(I); }
 };
 
+qdwf::RhsSet one_col()
+{
+  qdwf::RhsSet c;
+  c.n = 1;
+  c.fstride = 0;
+  for (int i = 0; i < qdwf::kMaxRhs; ++i) c.rhs[i] = 0;
+  return c;
+}
+
 // out = alpha * Mat in + beta * x; with dotp (and x) also block partials of
 // Re <dotp, out> into I.part (m5_dot_blocks of them)
 template <class T>
 void m5(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int mat, T alpha = 1,
-        const typename Prec<T>::slot* x = nullptr, T beta = 0, const typename Prec<T>::slot* dotp = nullptr)
+        const typename Prec<T>::slot* x = nullptr, T beta = 0, const typename Prec<T>::slot* dotp = nullptr,
+        const qdwf::RhsSet* cols = nullptr)
 {
   qdwf::M5Args<T> a;
   a.dotp = dotp;
   a.part = I.part;
+  a.cols = cols ? *cols : one_col();
   a.geo = I.geo;
   a.ls = I.ls;
   a.alpha = alpha;
@@ -285,9 +335,10 @@ void m5(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, i
 // out(parity eo_out) = alpha * H in + beta_s * x   (beta: 0 ones, 1 c~)
 template <class T>
 void hop(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo_out, bool dag,
-         T alpha = 1, const typename Prec<T>::slot* x = nullptr, int beta = 0)
+         T alpha = 1, const typename Prec<T>::slot* x = nullptr, int beta = 0, const qdwf::RhsSet* cols = nullptr)
 {
   qdwf::HopArgs<T> a;
+  a.cols = cols ? *cols : one_col();
   a.geo = I.geo;
   a.ls = I.ls;
   a.eo_out = eo_out;
@@ -340,13 +391,15 @@ void op_mdag_e_o(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::sl
 
 // FP32 fused out = alpha * Mat (H in) + beta * x (false if not available)
 bool hop_m5_32(Inv& I, float4* out, const float4* in, int eo_out, bool dag, int mat, float alpha = 1.f,
-               const float4* x = nullptr, float beta = 0.f, const float4* dotp = nullptr)
+               const float4* x = nullptr, float beta = 0.f, const float4* dotp = nullptr,
+               const qdwf::RhsSet* cols = nullptr)
 {
   if (!I.fused32) return false;
   if (dotp && !(dag && x)) return false;
   qdwf::HopM5Args a;
   a.dotp = dotp;
   a.part = I.part;
+  a.cols = cols ? *cols : one_col();
   a.geo = I.geo;
   a.ls = I.ls;
   a.eo_out = eo_out;
@@ -409,52 +462,65 @@ void op_hermop(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot
 }
 
 // CG form of Mpc^+ Mpc: ap = A p given t1 = K p (computed by the previous
-// CG step), with Re <p, ap> reduced in the same pass.  Returns that dot.
+// CG step), with Re <p, ap> reduced in the same pass, for every column of
+// `cols` (stacks of fields).  dots[i] receives column i's value.
 template <class T>
-double hermop_t1_dot(Inv& I, typename Prec<T>::slot* ap, const typename Prec<T>::slot* p,
-                     const typename Prec<T>::slot* t1, int eo)
+void hermop_t1_dot(Inv& I, typename Prec<T>::slot* ap, const typename Prec<T>::slot* p,
+                   const typename Prec<T>::slot* t1, int eo, const qdwf::RhsSet& cols,
+                   typename Prec<T>::slot* A, typename Prec<T>::slot* B, typename Prec<T>::slot* Y, double* dots)
 {
-  auto* A = Tmp<T>::t2(I);
-  auto* B = Tmp<T>::t3(I);
-  auto* Y = Tmp<T>::y(I);
   int64_t nparts = 0;
   bool done = false;
   if constexpr (std::is_same<T, float>::value) {
-    if (hop_m5_32(I, B, t1, 3 - eo, false, MAT_K)) {
-      hop<T>(I, Y, B, eo, false, (T)-1, p, 0);                   // y = p - H K H K p
-      hop_m5_32(I, B, Y, 3 - eo, true, MAT_K_DAG);               // u = K^+ H^+ y
-      hop_m5_32(I, ap, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, p);  // ap = y - K^+ H^+ u
+    if (hop_m5_32(I, B, t1, 3 - eo, false, MAT_K, 1.f, nullptr, 0.f, nullptr, &cols)) {
+      hop<T>(I, Y, B, eo, false, (T)-1, p, 0, &cols);                         // y = p - H K H K p
+      hop_m5_32(I, B, Y, 3 - eo, true, MAT_K_DAG, 1.f, nullptr, 0.f, nullptr, &cols);  // u = K^+ H^+ y
+      hop_m5_32(I, ap, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, p, &cols);      // ap = y - K^+ H^+ u
       nparts = qdwf::hop_m5_blocks(I.geo);
       done = true;
     }
   }
   if (!done) {
-    hop<T>(I, A, t1, 3 - eo, false);
-    m5<T>(I, B, A, MAT_K);
-    hop<T>(I, Y, B, eo, false, (T)-1, p, 0);
-    hop<T>(I, A, Y, 3 - eo, true);
-    m5<T>(I, B, A, MAT_K_DAG);
-    hop<T>(I, A, B, eo, true);
-    m5<T>(I, ap, A, MAT_K_DAG, (T)-1, Y, (T)1, p);
+    hop<T>(I, A, t1, 3 - eo, false, (T)1, nullptr, 0, &cols);
+    m5<T>(I, B, A, MAT_K, (T)1, nullptr, (T)0, nullptr, &cols);
+    hop<T>(I, Y, B, eo, false, (T)-1, p, 0, &cols);
+    hop<T>(I, A, Y, 3 - eo, true, (T)1, nullptr, 0, &cols);
+    m5<T>(I, B, A, MAT_K_DAG, (T)1, nullptr, (T)0, nullptr, &cols);
+    hop<T>(I, A, B, eo, true, (T)1, nullptr, 0, &cols);
+    m5<T>(I, ap, A, MAT_K_DAG, (T)-1, Y, (T)1, p, &cols);
     nparts = (I.geo.vh + qdwf::kM5Block - 1) / qdwf::kM5Block * qdwf::SpinorIO<T>::kSlots;
   }
-  qdwf::launch_reduce(I.part, (int)nparts, 1, I.red, I.stream);
+  qdwf::launch_reduce_rows(I.part, (int)nparts, cols.n, I.red, I.stream);
   check_launch();
-  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, sizeof(double), hipMemcpyDeviceToHost, I.stream));
+  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, cols.n * sizeof(double), hipMemcpyDeviceToHost, I.stream));
   QDWF_HIP(hipStreamSynchronize(I.stream));
-  return I.red_host[0];
+  for (int i = 0; i < cols.n; ++i) dots[i] = I.red_host[i];
 }
 
-// x += alpha p (x may be null); p = r + beta p; t = K p
+// single-field form
+template <class T>
+double hermop_t1_dot(Inv& I, typename Prec<T>::slot* ap, const typename Prec<T>::slot* p,
+                     const typename Prec<T>::slot* t1, int eo)
+{
+  double d = 0;
+  const qdwf::RhsSet c = one_col();
+  hermop_t1_dot<T>(I, ap, p, t1, eo, c, Tmp<T>::t2(I), Tmp<T>::t3(I), Tmp<T>::y(I), &d);
+  return d;
+}
+
+// x_i += alpha_i p_i (x may be null); p_i = r_i + beta_i p_i; t_i = K p_i
 template <class T>
 void cg_step(Inv& I, typename Prec<T>::slot* x, typename Prec<T>::slot* p, const typename Prec<T>::slot* r,
-             typename Prec<T>::slot* t, double alpha, double beta)
+             typename Prec<T>::slot* t, const double* alpha, const double* beta, const qdwf::RhsSet& cols)
 {
   qdwf::CgStepArgs<T> a;
   a.geo = I.geo;
   a.ls = I.ls;
-  a.alpha = (T)alpha;
-  a.beta = (T)beta;
+  for (int i = 0; i < qdwf::kMaxRhs; ++i) {
+    a.alpha[i] = i < cols.n ? alpha[i] : 0.0;
+    a.beta[i] = i < cols.n ? beta[i] : 0.0;
+  }
+  a.cols = cols;
   a.mat = Prec<T>::mat(I, MAT_K);
   a.real_mat = I.real_mat[MAT_K];
   a.x = x;
@@ -463,6 +529,13 @@ void cg_step(Inv& I, typename Prec<T>::slot* x, typename Prec<T>::slot* p, const
   a.t = t;
   qdwf::launch_cg_step<T>(a, I.stream);
   check_launch();
+}
+
+template <class T>
+void cg_step(Inv& I, typename Prec<T>::slot* x, typename Prec<T>::slot* p, const typename Prec<T>::slot* r,
+             typename Prec<T>::slot* t, double alpha, double beta)
+{
+  cg_step<T>(I, x, p, r, t, &alpha, &beta, one_col());
 }
 
 // ------------------------------------------------------------ BLAS-1
@@ -662,6 +735,133 @@ long cg_mixed(Inv& I, double2* x64, const double2* b64, double stop_rsd, long ma
   check_launch();
   *true_rel = std::sqrt(true_residual() / b2);
   return max_total_iter + 1;
+}
+
+// Multi-RHS mixed-precision CG: A x_c = b_c for the columns c < ncol
+// (ncol <= kMaxRhs) held in C64(c, 0) (b) and C64(c, 1) (x, zero on entry).
+// All columns advance together through FP32 kernels that stage each link
+// block once for every column; each column keeps its own alpha, beta,
+// FP64 reliable updates (residual recomputed as b - A x in FP64 when its
+// iterated residual dropped by delta, and at convergence) and retires when
+// its FP64 true residual reaches stop_rsd.  Returns the summed iterations.
+long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
+{
+  const int64_t fs = n32(I), nd = n64(I);
+  float4* R = MV(I, M_R, ncol);
+  float4* P = MV(I, M_P, ncol);
+  float4* AP = MV(I, M_AP, ncol);
+  float4* X = MV(I, M_X, ncol);
+  float4* T = MV(I, M_T, ncol);
+  float4* B = MV(I, M_B, ncol);
+  float4* Y = MV(I, M_Y, ncol);
+  float4* A = I.fused32 ? B : MV(I, M_A, ncol);  // only the unfused path needs A
+  double2* r64 = W(I, W_CG_R);
+  double2* ax64 = W(I, W_CG_AP);
+  std::vector<double> b2(ncol), target(ncol), rr(ncol), maxrr(ncol);
+  std::vector<long> iters(ncol, 0);
+  std::vector<int> active;
+  for (int c = 0; c < ncol; ++c) {
+    b2[c] = dev_norm2(I, C64(I, c, 0));
+    target[c] = stop_rsd * stop_rsd * b2[c];
+    zero64(I, C64(I, c, 1));
+    // x = 0: r = b exactly
+    qdwf::launch_d2f(C64(I, c, 0), R + c * fs, I.ls, I.geo.stride, I.stream);
+    check_launch();
+    rr[c] = maxrr[c] = b2[c];
+    if (rr[c] > target[c]) active.push_back(c);
+  }
+  QDWF_HIP(hipMemcpyAsync(P, R, field32_bytes(I) * ncol, hipMemcpyDeviceToDevice, I.stream));
+  QDWF_HIP(hipMemsetAsync(X, 0, field32_bytes(I) * ncol, I.stream));
+  auto colset = [&](const std::vector<int>& act) {
+    qdwf::RhsSet cs;
+    cs.n = (int)act.size();
+    cs.fstride = fs;
+    for (int i = 0; i < qdwf::kMaxRhs; ++i) cs.rhs[i] = i < cs.n ? act[i] : 0;
+    return cs;
+  };
+  {
+    const qdwf::RhsSet all = colset(active);
+    if (all.n) m5<float>(I, T, P, MAT_K, 1.f, nullptr, 0.f, nullptr, &all);
+  }
+  const double d2 = I.delta * I.delta;
+  auto true_residual = [&](int c) {
+    op_hermop<double>(I, ax64, C64(I, c, 1), 1);
+    I.stats.hermop_fp64 += 1;
+    copy64(I, r64, C64(I, c, 0));
+    qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
+    check_launch();
+    return dev_norm2(I, r64);
+  };
+  double pap[qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs], beta[qdwf::kMaxRhs],
+      step_alpha[qdwf::kMaxRhs];
+  long it = 0;
+  while (!active.empty() && it < cap) {
+    const qdwf::RhsSet cs = colset(active);
+    hermop_t1_dot<float>(I, AP, P, T, 1, cs, A, B, Y, pap);
+    I.stats.hermop_low += cs.n;
+    qdwf::AxpyNormArgs<float4> an;
+    an.ap = AP;
+    an.r = R;
+    an.n = fs;
+    an.cols = cs;
+    an.part = I.part;
+    for (int i = 0; i < qdwf::kMaxRhs; ++i) {
+      alpha[i] = i < cs.n ? rr[active[i]] / pap[i] : 0.0;
+      an.alpha[i] = alpha[i];
+    }
+    qdwf::launch_axpy_norm_multi<float4>(an, I.red, I.stream);
+    check_launch();
+    QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, cs.n * sizeof(double), hipMemcpyDeviceToHost, I.stream));
+    sync(I);
+    for (int i = 0; i < cs.n; ++i) rr_new[i] = I.red_host[i];
+    ++it;
+    std::vector<int> keep;
+    std::vector<double> ka, kb;
+    for (int i = 0; i < cs.n; ++i) {
+      const int c = active[i];
+      iters[c] += 1;
+      if (!std::isfinite(rr_new[i])) throw std::runtime_error("CG breakdown: non-finite residual (FP32, multi)");
+      step_alpha[i] = alpha[i];
+      if (rr_new[i] <= d2 * maxrr[c] || rr_new[i] <= target[c]) {
+        float4* xc = X + c * fs;
+        qdwf::launch_axpy<float4>(alpha[i], P + c * fs, xc, fs, I.stream);
+        qdwf::launch_f2d(xc, C64(I, c, 1), I.ls, I.geo.stride, 1, I.stream);
+        check_launch();
+        zero32(I, xc);
+        const double rr_true = true_residual(c);
+        I.stats.reliable_updates += 1;
+        if (rr_true <= target[c]) {
+          const double rel = std::sqrt(rr_true / b2[c]);
+          if (rel > I.stats.true_residual) I.stats.true_residual = rel;
+          continue;  // column retired
+        }
+        qdwf::launch_d2f(r64, R + c * fs, I.ls, I.geo.stride, I.stream);
+        check_launch();
+        rr_new[i] = rr_true;
+        maxrr[c] = rr_true;
+        step_alpha[i] = 0.0;
+      }
+      keep.push_back(c);
+      ka.push_back(step_alpha[i]);
+      kb.push_back(rr_new[i] / rr[c]);
+      rr[c] = rr_new[i];
+    }
+    if (!keep.empty()) {
+      const qdwf::RhsSet ks = colset(keep);
+      cg_step<float>(I, X, P, R, T, ka.data(), kb.data(), ks);
+    }
+    active.swap(keep);
+  }
+  long total = 0;
+  for (int c = 0; c < ncol; ++c) total += iters[c];
+  for (int c : active) {  // cap reached: fold in what we have
+    qdwf::launch_f2d(X + c * fs, C64(I, c, 1), I.ls, I.geo.stride, 1, I.stream);
+    check_launch();
+    const double rel = std::sqrt(true_residual(c) / b2[c]);
+    if (rel > I.stats.true_residual) I.stats.true_residual = rel;
+    total += 1;  // report non-convergence like cg_with_f (max_num_iter + 1)
+  }
+  return total;
 }
 
 double true_residual_fp64(Inv& I, const double2* x, const double2* b)
@@ -895,6 +1095,42 @@ long invert_5d_dev(Inv& I, double2* out_aos, const double2* in_aos)
   return it;
 }
 
+// D_- and the sym2 odd preconditioning of one 5D source (FP64 halves):
+// b = odd-prec source, oep = out_e_p (set_odd_prec_field_sym2, dslash.h:401-417).
+// Returns |D_- in|^2.
+double prepare_column(Inv& I, const double2* in_e, const double2* in_o, double2* b, double2* oep)
+{
+  double2* dm_e = W(I, W_DM_E);
+  double2* dm_o = W(I, W_DM_O);
+  if (I.dminus) {
+    op_d_minus(I, dm_e, dm_o, in_e, in_o);
+  } else {
+    copy64(I, dm_e, in_e);
+    copy64(I, dm_o, in_o);
+  }
+  const double q = dev_norm2(I, dm_e) + dev_norm2(I, dm_o);
+  double2* tmp = W(I, W_TMP);
+  m5<double>(I, oep, dm_e, MAT_MEE_INV);
+  op_m_e_o<double>(I, tmp, oep, 2);
+  copy64(I, b, dm_o);
+  qdwf::launch_axpy<double2>(-1.0, tmp, b, n64(I), I.stream);
+  check_launch();
+  op_mpcdag<double>(I, b, b, 1);
+  return q;
+}
+
+// restore_field_from_odd_prec_sym2 (dslash.h:419-438)
+void restore_column(Inv& I, const double2* x_o, const double2* oep, double2* out_e, double2* out_o)
+{
+  double2* tmp = W(I, W_TMP);
+  m5<double>(I, out_o, x_o, MAT_MEE_INV);
+  op_m_e_o<double>(I, tmp, out_o, 1);
+  m5<double>(I, W(I, W_ITMP), tmp, MAT_MEE_INV);
+  copy64(I, out_e, oep);
+  qdwf::launch_axpy<double2>(-1.0, W(I, W_ITMP), out_e, n64(I), I.stream);
+  check_launch();
+}
+
 // Propagator (or 4D fermion, ncol = 1 with col = -1) on device.
 long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
 {
@@ -904,6 +1140,34 @@ long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
   double2* oo = W(I, W_OUT_O);
   long total = 0;
   const int ncol = is_prop ? 12 : 1;
+  const int tile = std::min(I.rhs_tile, qdwf::kMaxRhs);
+  if (I.solver == QLAT_DWF_SOLVER_MIXED_FP32 && is_prop && tile > 1) {
+    // multi-RHS: columns in tiles solved as one block
+    for (int c0 = 0; c0 < ncol; c0 += tile) {
+      const int w = std::min(tile, ncol - c0);
+      std::vector<int> nonzero;
+      for (int j = 0; j < w; ++j) {
+        qdwf::launch_prop_col_to_5d(src, ie, I.geo, I.ls, 2, c0 + j, I.stream);
+        qdwf::launch_prop_col_to_5d(src, io, I.geo, I.ls, 1, c0 + j, I.stream);
+        check_launch();
+        const double q = prepare_column(I, ie, io, C64(I, j, 0), C64(I, j, 2));
+        if (q == 0.0) {
+          // dm_in == 0: solution zero (dslash.h:477-482)
+          zero64(I, C64(I, j, 0));
+        }
+      }
+      total += cg_mixed_multi(I, w, I.stop_rsd, I.max_num_iter * I.max_cycle);
+      I.stats.cycles += 1;
+      for (int j = 0; j < w; ++j) {
+        restore_column(I, C64(I, j, 1), C64(I, j, 2), oe, oo);
+        qdwf::launch_5d_to_prop_col(oe, sol, I.geo, I.ls, 2, c0 + j, I.stream);
+        qdwf::launch_5d_to_prop_col(oo, sol, I.geo, I.ls, 1, c0 + j, I.stream);
+        check_launch();
+      }
+    }
+    I.stats.total_iter += total;
+    return total;
+  }
   for (int j = 0; j < ncol; ++j) {
     const int col = is_prop ? j : -1;
     qdwf::launch_prop_col_to_5d(src, ie, I.geo, I.ls, 2, col, I.stream);
@@ -996,6 +1260,8 @@ void free_all(Inv* I)
   for (auto* p : I->w64) f(p);
   for (auto* p : I->w32) f(p);
   for (auto* p : I->stage) f(p);
+  for (auto* p : I->m32) f(p);
+  for (auto* p : I->col64) f(p);
   f(I->part);
   f(I->red);
   if (I->red_host) (void)hipHostFree(I->red_host);
@@ -1082,16 +1348,17 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT32")) I->hop_spt32 = std::atoi(e);
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT64")) I->hop_spt64 = std::atoi(e);
     if (const char* e = std::getenv("QLAT_DWF_FUSED")) I->fused32 = std::atoi(e) != 0;
+    if (const char* e = std::getenv("QLAT_DWF_RHS_TILE")) I->rhs_tile = std::max(1, std::min(qdwf::kMaxRhs, std::atoi(e)));
     {
       // partial sums: BLAS grids, one per 64-site block (fused kernels), or
       // one per (256-site block, slot) (s-matrix kernel)
       int64_t np = qdwf::kMaxBlasGrid * 4;
       np = std::max<int64_t>(np, (I->geo.vh + 31) / 32 + 16);
       np = std::max<int64_t>(np, (I->geo.vh + qdwf::kM5Block - 1) / qdwf::kM5Block * 12 + 16);
-      I->part = (double*)dev_alloc(np * sizeof(double));
+      I->part = (double*)dev_alloc(np * qdwf::kMaxRhs * sizeof(double));
     }
-    I->red = (double*)dev_alloc(8 * sizeof(double));
-    QDWF_HIP(hipHostMalloc((void**)&I->red_host, 8 * sizeof(double), hipHostMallocDefault));
+    I->red = (double*)dev_alloc(2 * qdwf::kMaxRhs * sizeof(double));
+    QDWF_HIP(hipHostMalloc((void**)&I->red_host, 2 * qdwf::kMaxRhs * sizeof(double), hipHostMallocDefault));
     *out = I;
   });
   if (rc != QLAT_DWF_OK && I) {

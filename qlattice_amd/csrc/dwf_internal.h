@@ -10,6 +10,15 @@ constexpr int kM5Block = 256;
 constexpr int kBlasBlock = 256;
 constexpr int kMaxBlasGrid = 2048;
 constexpr int kHopM5Sites = 32;  // output half-sites per fused hop + s-matrix block
+constexpr int kMaxRhs = 16;      // right-hand sides per multi-RHS launch (tile width)
+
+// The columns a launch works on: column rhs[i] of a stack of fields spaced
+// fstride slots apart, i < n.  A single field is {1, fstride, {0}}.
+struct RhsSet {
+  int n;
+  int64_t fstride;
+  int rhs[kMaxRhs];
+};
 
 template <class T>
 struct HopArgs {
@@ -23,6 +32,7 @@ struct HopArgs {
   const void* x;       // optional xpay operand (parity eo_out) or nullptr
   const cplx<T>* beta; // per-s xpay coefficients (device), used with x
   const void* links;   // 8-link layout (dwf_common.h)
+  RhsSet cols;         // in/out/x are stacks of fields
 };
 
 template <class T>
@@ -36,8 +46,9 @@ struct M5Args {
   const void* in;
   void* out;
   const void* x;      // optional
-  const void* dotp;   // optional (with x): part[block] = Re <dotp, out>
+  const void* dotp;   // optional (with x): part[col*nblk + block] = Re <dotp, out>
   double* part;
+  RhsSet cols;        // blockIdx.z = column index i
 };
 
 // x += alpha p (x optional); p = r + beta p; t = Mat p
@@ -45,7 +56,8 @@ template <class T>
 struct CgStepArgs {
   LatGeo geo;
   int ls;
-  T alpha, beta;
+  double alpha[kMaxRhs], beta[kMaxRhs];  // per column i
+  RhsSet cols;                           // blockIdx.z = column index i
   const cplx<T>* mat;
   bool real_mat;
   void* x;
@@ -67,7 +79,19 @@ struct HopM5Args {
   const void* links;
   const cplx<float>* mat;  // [2][ls][ls]
   bool real_mat;
-  const void* dotp;  // optional (dag xpay form): part[block] = Re <dotp, out>
+  const void* dotp;  // optional (dag xpay form): part[i*nblk + block] = Re <dotp, out>
+  double* part;
+  RhsSet cols;       // in/out/x/dotp are stacks of fields
+};
+
+// r_i -= alpha_i ap_i with per-column partial |r_i|^2 (part[i*nblk + block])
+template <class S>
+struct AxpyNormArgs {
+  const S* ap;
+  S* r;
+  int64_t n;  // slots per field
+  double alpha[kMaxRhs];
+  RhsSet cols;
   double* part;
 };
 
@@ -83,6 +107,10 @@ void launch_cg_step(const CgStepArgs<T>& a, hipStream_t st);
 template <class S>
 void launch_axpy_norm(double a, const S* ap, S* r, int64_t n, double* part, double* out, hipStream_t st);
 void launch_reduce(const double* part, int nblk, int nv, double* out, hipStream_t st);
+// out[i] = sum_b part[i*nblk + b], i < nrows
+void launch_reduce_rows(const double* part, int nblk, int nrows, double* out, hipStream_t st);
+template <class S>
+int launch_axpy_norm_multi(const AxpyNormArgs<S>& a, double* out, hipStream_t st);
 
 template <class S>
 void launch_norm2(const S* a, int64_t n, double* part, double* out, hipStream_t st);

@@ -110,12 +110,15 @@ __global__ void __launch_bounds__(64 * NW) k_hop_lds(HopArgs<T> a)
   half_coord(a.geo, h, a.eo_out, x);
   int64_t nb[8];
   neighbours(a.geo, x, nb);
-  const slot_t* __restrict__ in = (const slot_t*)a.in;
-  slot_t* __restrict__ out = (slot_t*)a.out;
   constexpr int SF = DAG ? 1 : -1;
   constexpr int SB = -SF;
   const T scale = (T)(-0.5) * a.alpha;
-  for (int s = w; s < a.ls; s += NW) {
+  const int nrs = a.cols.n * a.ls;
+  for (int rs = w; rs < nrs; rs += NW) {
+    const int ci = rs / a.ls, s = rs - ci * a.ls;
+    const int64_t coff = (int64_t)a.cols.rhs[ci] * a.cols.fstride;
+    const slot_t* __restrict__ in = (const slot_t*)a.in + coff;
+    slot_t* __restrict__ out = (slot_t*)a.out + coff;
     cplx<T> acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = cmk<T>(0, 0);
@@ -145,7 +148,7 @@ __global__ void __launch_bounds__(64 * NW) k_hop_lds(HopArgs<T> a)
     for (int k = 0; k < 12; ++k) v[k] = cmk<T>(scale * acc[k].re, scale * acc[k].im);
     if (XPAY) {
       cplx<T> xv[12];
-      SpinorIO<T>::load((const slot_t*)a.x, a.ls, s, h, xv);
+      SpinorIO<T>::load((const slot_t*)a.x + coff, a.ls, s, h, xv);
       const cplx<T> b = a.beta[s];
 #pragma unroll
       for (int k = 0; k < 12; ++k) v[k] = cfma(v[k], b, xv[k]);
@@ -170,6 +173,20 @@ static void launch_hop_lds(const HopArgs<T>& a, bool dag, hipStream_t st)
 template <class T, int SPT>
 static void launch_hop_spt(const HopArgs<T>& a0, bool dag, hipStream_t st)
 {
+  typedef typename SpinorIO<T>::slot_t slot_t;
+  if (a0.cols.n != 1 || a0.cols.rhs[0] != 0) {  // one launch per column
+    for (int i = 0; i < a0.cols.n; ++i) {
+      HopArgs<T> b = a0;
+      const int64_t off = (int64_t)a0.cols.rhs[i] * a0.cols.fstride;
+      b.in = (const slot_t*)a0.in + off;
+      b.out = (slot_t*)a0.out + off;
+      if (a0.x) b.x = (const slot_t*)a0.x + off;
+      b.cols.n = 1;
+      b.cols.rhs[0] = 0;
+      launch_hop_spt<T, SPT>(b, dag, st);
+    }
+    return;
+  }
   HopArgs<T> a = a0;
   a.nchunk = (a.ls + SPT - 1) / SPT;
   const int64_t hblocks = (a.geo.vh + kHopBlock - 1) / kHopBlock;
@@ -231,12 +248,17 @@ k_hop_m5(HopM5Args a)
   const int site = threadIdx.x % SB, row = threadIdx.x / SB;
   const int64_t h = h0 + site;
   const bool active = h < a.geo.vh;
-  if (active && row < ls) {
+  int64_t nb[8];
+  if (active) {
     int x[4];
     half_coord(a.geo, h, a.eo_out, x);
-    int64_t nb[8];
     neighbours(a.geo, x, nb);
-    const float4* __restrict__ in = (const float4*)a.in;
+  }
+  const int64_t nblk = (int64_t)gridDim.x;
+  for (int ci = 0; ci < a.cols.n; ++ci) {
+  const int64_t coff = (int64_t)a.cols.rhs[ci] * a.cols.fstride;
+  if (active && row < ls) {
+    const float4* __restrict__ in = (const float4*)a.in + coff;
     constexpr int SF = DAG ? 1 : -1;
     constexpr int SB_ = -SF;
     cplx<float> acc[12];
@@ -282,8 +304,8 @@ k_hop_m5(HopM5Args a)
       if (t < ls) {
         v[t] = hr[t][kp][site];
         const int64_t o = sidx<6>(ls, t, kp, h);
-        if (XPAY) xv[t] = ((const float4*)a.x)[o];
-        if (DOT) pv[t] = ((const float4*)a.dotp)[o];
+        if (XPAY) xv[t] = ((const float4*)a.x)[coff + o];
+        if (DOT) pv[t] = ((const float4*)a.dotp)[coff + o];
       }
     }
 #pragma unroll
@@ -313,7 +335,7 @@ k_hop_m5(HopM5Args a)
           res.z += a.beta * xv[s].z;
           res.w += a.beta * xv[s].w;
         }
-        store_nt(&((float4*)a.out)[sidx<6>(ls, s, kp, h)], res);
+        store_nt(&((float4*)a.out)[coff + sidx<6>(ls, s, kp, h)], res);
         if (DOT) dot += (double)pv[s].x * res.x + (double)pv[s].y * res.y + (double)pv[s].z * res.z + (double)pv[s].w * res.w;
       }
     }
@@ -328,8 +350,10 @@ k_hop_m5(HopM5Args a)
     if (threadIdx.x == 0) {
       double t = 0.0;
       for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) t += red[i];
-      a.part[lb] = t;
+      a.part[ci * nblk + lb] = t;
     }
+  }
+  __syncthreads();  // hr (and red) are reused by the next column
   }
 }
 
@@ -394,8 +418,9 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
   const int chi = (slot < NS / 2) ? 0 : 1;
   const int ls = a.ls;
   const cplx<T>* __restrict__ M = a.mat + (int64_t)chi * ls * ls;
-  const slot_t* __restrict__ in = (const slot_t*)a.in;
-  slot_t* __restrict__ out = (slot_t*)a.out;
+  const int64_t coff = (int64_t)a.cols.rhs[blockIdx.z] * a.cols.fstride;
+  const slot_t* __restrict__ in = (const slot_t*)a.in + coff;
+  slot_t* __restrict__ out = (slot_t*)a.out + coff;
   // a slot holds one (FP64) or two (FP32) complex numbers
   constexpr int NC = (NS == 12) ? 1 : 2;
   cplx<T> v[LSMAX][NC];
@@ -434,7 +459,7 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
     for (int c = 0; c < NC; ++c) r[c] = cmk<T>(a.alpha * r[c].re, a.alpha * r[c].im);
     const int64_t off = sidx<NS>(ls, s, slot, h);
     if (XPAY) {
-      const slot_t q = ((const slot_t*)a.x)[off];
+      const slot_t q = ((const slot_t*)a.x)[coff + off];
       if constexpr (NC == 1) {
         r[0].re += a.beta * q.x;
         r[0].im += a.beta * q.y;
@@ -457,7 +482,7 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
     }
     out[off] = o;
     if (DOT) {
-      const slot_t q = ((const slot_t*)a.dotp)[off];
+      const slot_t q = ((const slot_t*)a.dotp)[coff + off];
       if constexpr (NC == 1) {
         dot += (double)q.x * o.x + (double)q.y * o.y;
       } else {
@@ -477,7 +502,7 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
     if (threadIdx.x == 0) {
       double t = 0.0;
       for (int i = 0; i < kM5Block / 64; ++i) t += sh[i];
-      a.part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+      a.part[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
     }
   }
 }
@@ -497,10 +522,12 @@ __global__ void __launch_bounds__(kM5Block) k_cg_step(CgStepArgs<T> a)
   const int chi = (slot < NS / 2) ? 0 : 1;
   const int ls = a.ls;
   const cplx<T>* __restrict__ M = a.mat + (int64_t)chi * ls * ls;
-  slot_t* __restrict__ x = (slot_t*)a.x;
-  slot_t* __restrict__ p = (slot_t*)a.p;
-  const slot_t* __restrict__ r = (const slot_t*)a.r;
-  const T al = a.alpha, be = a.beta;
+  const int ci = blockIdx.z;
+  const int64_t coff = (int64_t)a.cols.rhs[ci] * a.cols.fstride;
+  slot_t* __restrict__ x = (slot_t*)a.x + coff;
+  slot_t* __restrict__ p = (slot_t*)a.p + coff;
+  const slot_t* __restrict__ r = (const slot_t*)a.r + coff;
+  const T al = (T)a.alpha[ci], be = (T)a.beta[ci];
   cplx<T> v[LSMAX][NC];
 #pragma unroll
   for (int t = 0; t < LSMAX; ++t) {
@@ -530,7 +557,7 @@ __global__ void __launch_bounds__(kM5Block) k_cg_step(CgStepArgs<T> a)
       if constexpr (NC == 2) v[t][NC - 1] = cmk<T>(pn.z, pn.w);
     }
   }
-  slot_t* __restrict__ out = (slot_t*)a.t;
+  slot_t* __restrict__ out = (slot_t*)a.t + coff;
   for (int s = 0; s < ls; ++s) {
     cplx<T> res[NC];
 #pragma unroll
@@ -564,7 +591,7 @@ __global__ void __launch_bounds__(kM5Block) k_cg_step(CgStepArgs<T> a)
 template <class T, int LSMAX>
 static void launch_cg_step_ls(const CgStepArgs<T>& a, hipStream_t st)
 {
-  const dim3 grid((unsigned)((a.geo.vh + kM5Block - 1) / kM5Block), SpinorIO<T>::kSlots);
+  const dim3 grid((unsigned)((a.geo.vh + kM5Block - 1) / kM5Block), SpinorIO<T>::kSlots, a.cols.n);
   const bool updx = a.x != nullptr;
   if (a.real_mat) {
     if (updx) hipLaunchKernelGGL((k_cg_step<T, LSMAX, true, true>), grid, dim3(kM5Block), 0, st, a);
@@ -591,7 +618,7 @@ template void launch_cg_step<float>(const CgStepArgs<float>&, hipStream_t);
 template <class T, int LSMAX>
 static void launch_m5_ls(const M5Args<T>& a, hipStream_t st)
 {
-  const dim3 grid((unsigned)((a.geo.vh + kM5Block - 1) / kM5Block), SpinorIO<T>::kSlots);
+  const dim3 grid((unsigned)((a.geo.vh + kM5Block - 1) / kM5Block), SpinorIO<T>::kSlots, a.cols.n);
   if (a.dotp) {  // x-pay form with the CG dot product
     if (a.real_mat) hipLaunchKernelGGL((k_m5<T, LSMAX, true, true, true>), grid, dim3(kM5Block), 0, st, a);
     else hipLaunchKernelGGL((k_m5<T, LSMAX, true, false, true>), grid, dim3(kM5Block), 0, st, a);
@@ -760,6 +787,46 @@ __global__ void __launch_bounds__(kBlasBlock) k_axpy_norm(double a, const S* __r
   block_reduce_store<1>(v, part);
 }
 
+template <class S>
+__global__ void __launch_bounds__(kBlasBlock) k_axpy_norm_multi(AxpyNormArgs<S> a)
+{
+  const int ci = blockIdx.y;
+  const int64_t coff = (int64_t)a.cols.rhs[ci] * a.cols.fstride;
+  const S* __restrict__ ap = a.ap + coff;
+  S* __restrict__ r = a.r + coff;
+  const double al = a.alpha[ci];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * kBlasBlock) {
+    const S rn = slot_axpy(-al, ap[i], r[i]);
+    r[i] = rn;
+    v += slot_norm2(rn);
+  }
+  __shared__ double sh[kBlasBlock / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kBlasBlock / 64; ++i) t += sh[i];
+    a.part[(int64_t)ci * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// out[row] = sum of part[row*nblk + 0 .. nblk)
+__global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ part, int nblk, double* out)
+{
+  __shared__ double sh[4];
+  const double* p = part + (int64_t)blockIdx.x * nblk;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) v += p[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
 __global__ void __launch_bounds__(256) k_reduce_final(const double* __restrict__ part, int nblk, int nv, double* out)
 {
   __shared__ double sh[4][2];
@@ -876,6 +943,21 @@ void launch_axpy_norm(double a, const S* ap, S* r, int64_t n, double* part, doub
   hipLaunchKernelGGL(k_axpy_norm<S>, dim3(g), dim3(kBlasBlock), 0, st, a, ap, r, n, part);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, g, 1, out);
 }
+void launch_reduce_rows(const double* part, int nblk, int nrows, double* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(256), 0, st, part, nblk, out);
+}
+template <class S>
+int launch_axpy_norm_multi(const AxpyNormArgs<S>& a, double* out, hipStream_t st)
+{
+  const int g = blas_grid(a.n);
+  hipLaunchKernelGGL(k_axpy_norm_multi<S>, dim3(g, a.cols.n), dim3(kBlasBlock), 0, st, a);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(a.cols.n), dim3(256), 0, st, a.part, g, out);
+  return g;
+}
+template int launch_axpy_norm_multi<double2>(const AxpyNormArgs<double2>&, double*, hipStream_t);
+template int launch_axpy_norm_multi<float4>(const AxpyNormArgs<float4>&, double*, hipStream_t);
+
 void launch_reduce(const double* part, int nblk, int nv, double* out, hipStream_t st)
 {
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, nblk, nv, out);
