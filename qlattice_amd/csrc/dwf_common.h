@@ -162,6 +162,18 @@ __device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t nblk)
   return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
+// Element i of a small by-value array (kernel arguments) without dynamic
+// indexing, which would copy the array to scratch memory; i is wave-uniform.
+template <class V, int N>
+__device__ __forceinline__ V pick(const V (&arr)[N], int i)
+{
+  V r = arr[0];
+#pragma unroll
+  for (int j = 1; j < N; ++j)
+    if (j == i) r = arr[j];
+  return r;
+}
+
 // Non-temporal (streaming) stores: outputs that the next kernel reads in a
 // different order should not evict this kernel's reusable inputs from L2.
 __device__ __forceinline__ void store_nt(float4* p, const float4& v)

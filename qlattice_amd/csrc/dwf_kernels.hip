@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(64 * NW) k_hop_lds(HopArgs<T> a)
   const int nrs = a.cols.n * a.ls;
   for (int rs = w; rs < nrs; rs += NW) {
     const int ci = rs / a.ls, s = rs - ci * a.ls;
-    const int64_t coff = (int64_t)a.cols.rhs[ci] * a.cols.fstride;
+    const int64_t coff = (int64_t)pick(a.cols.rhs, ci) * a.cols.fstride;
     const slot_t* __restrict__ in = (const slot_t*)a.in + coff;
     slot_t* __restrict__ out = (slot_t*)a.out + coff;
     cplx<T> acc[12];
@@ -237,7 +237,13 @@ k_hop_m5(HopM5Args a)
 {
   __shared__ float2 sl[72][SB];
   __shared__ float4 hr[LSMAX][6][SB];
-  const int64_t lb = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
+  // logical block -> (site block, column), column fastest: the blocks that
+  // share a site block (and its links) run back to back on one XCD
+  const int64_t lbc = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
+  const int ncol = a.cols.n;
+  const int64_t lb = lbc / ncol;
+  const int ci = (int)(lbc - lb * ncol);
+  const int64_t nblk = (int64_t)gridDim.x / ncol;
   const int64_t h0 = lb * SB;
   const int pslot = (a.eo_out == 2) ? 0 : 1;
   // the link block holding these SB sites (64-site link blocks)
@@ -248,16 +254,13 @@ k_hop_m5(HopM5Args a)
   const int site = threadIdx.x % SB, row = threadIdx.x / SB;
   const int64_t h = h0 + site;
   const bool active = h < a.geo.vh;
-  int64_t nb[8];
-  if (active) {
+  const int64_t coff = (int64_t)pick(a.cols.rhs, ci) * a.cols.fstride;
+  if (active && row < ls) {
+    // neighbours recomputed per column: cheaper than keeping them live
     int x[4];
     half_coord(a.geo, h, a.eo_out, x);
+    int64_t nb[8];
     neighbours(a.geo, x, nb);
-  }
-  const int64_t nblk = (int64_t)gridDim.x;
-  for (int ci = 0; ci < a.cols.n; ++ci) {
-  const int64_t coff = (int64_t)a.cols.rhs[ci] * a.cols.fstride;
-  if (active && row < ls) {
     const float4* __restrict__ in = (const float4*)a.in + coff;
     constexpr int SF = DAG ? 1 : -1;
     constexpr int SB_ = -SF;
@@ -296,16 +299,20 @@ k_hop_m5(HopM5Args a)
   if (active && kp < 6) {
     const int chi = (kp < 3) ? 0 : 1;
     const cplx<float>* __restrict__ M = a.mat + chi * ls * ls;
+    // x and dotp are prefetched for all s when that fits in registers
+    constexpr bool PREF = LSMAX <= 12;
     float4 v[LSMAX];
-    float4 xv[XPAY ? LSMAX : 1];
-    float4 pv[DOT ? LSMAX : 1];
+    float4 xv[XPAY && PREF ? LSMAX : 1];
+    float4 pv[DOT && PREF ? LSMAX : 1];
 #pragma unroll
     for (int t = 0; t < LSMAX; ++t) {
       if (t < ls) {
         v[t] = hr[t][kp][site];
-        const int64_t o = sidx<6>(ls, t, kp, h);
-        if (XPAY) xv[t] = ((const float4*)a.x)[coff + o];
-        if (DOT) pv[t] = ((const float4*)a.dotp)[coff + o];
+        if (PREF) {
+          const int64_t o = sidx<6>(ls, t, kp, h);
+          if (XPAY) xv[t] = ((const float4*)a.x)[coff + o];
+          if (DOT) pv[t] = ((const float4*)a.dotp)[coff + o];
+        }
       }
     }
 #pragma unroll
@@ -329,14 +336,19 @@ k_hop_m5(HopM5Args a)
           }
         }
         float4 res = make_float4(a.alpha * r0, a.alpha * i0, a.alpha * r1, a.alpha * i1);
+        const int64_t o = coff + sidx<6>(ls, s, kp, h);
         if (XPAY) {
-          res.x += a.beta * xv[s].x;
-          res.y += a.beta * xv[s].y;
-          res.z += a.beta * xv[s].z;
-          res.w += a.beta * xv[s].w;
+          const float4 xs = PREF ? xv[PREF ? s : 0] : ((const float4*)a.x)[o];
+          res.x += a.beta * xs.x;
+          res.y += a.beta * xs.y;
+          res.z += a.beta * xs.z;
+          res.w += a.beta * xs.w;
         }
-        store_nt(&((float4*)a.out)[coff + sidx<6>(ls, s, kp, h)], res);
-        if (DOT) dot += (double)pv[s].x * res.x + (double)pv[s].y * res.y + (double)pv[s].z * res.z + (double)pv[s].w * res.w;
+        store_nt(&((float4*)a.out)[o], res);
+        if (DOT) {
+          const float4 ps = PREF ? pv[PREF ? s : 0] : ((const float4*)a.dotp)[o];
+          dot += (double)ps.x * res.x + (double)ps.y * res.y + (double)ps.z * res.z + (double)ps.w * res.w;
+        }
       }
     }
   }
@@ -353,14 +365,12 @@ k_hop_m5(HopM5Args a)
       a.part[ci * nblk + lb] = t;
     }
   }
-  __syncthreads();  // hr (and red) are reused by the next column
-  }
 }
 
 template <int SB, int LSMAX>
 static void launch_hop_m5_ls(const HopM5Args& a, bool dag, hipStream_t st)
 {
-  const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB));
+  const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB * a.cols.n));
   const dim3 blk(SB * a.ls);
   const bool xpay = a.x != nullptr;
 #define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5<SB, LSMAX, D, X, R, DT>), grid, blk, 0, st, a)
@@ -418,7 +428,7 @@ __global__ void __launch_bounds__(kM5Block) k_m5(M5Args<T> a)
   const int chi = (slot < NS / 2) ? 0 : 1;
   const int ls = a.ls;
   const cplx<T>* __restrict__ M = a.mat + (int64_t)chi * ls * ls;
-  const int64_t coff = (int64_t)a.cols.rhs[blockIdx.z] * a.cols.fstride;
+  const int64_t coff = (int64_t)pick(a.cols.rhs, (int)blockIdx.z) * a.cols.fstride;
   const slot_t* __restrict__ in = (const slot_t*)a.in + coff;
   slot_t* __restrict__ out = (slot_t*)a.out + coff;
   // a slot holds one (FP64) or two (FP32) complex numbers
@@ -523,11 +533,11 @@ __global__ void __launch_bounds__(kM5Block) k_cg_step(CgStepArgs<T> a)
   const int ls = a.ls;
   const cplx<T>* __restrict__ M = a.mat + (int64_t)chi * ls * ls;
   const int ci = blockIdx.z;
-  const int64_t coff = (int64_t)a.cols.rhs[ci] * a.cols.fstride;
+  const int64_t coff = (int64_t)pick(a.cols.rhs, ci) * a.cols.fstride;
   slot_t* __restrict__ x = (slot_t*)a.x + coff;
   slot_t* __restrict__ p = (slot_t*)a.p + coff;
   const slot_t* __restrict__ r = (const slot_t*)a.r + coff;
-  const T al = (T)a.alpha[ci], be = (T)a.beta[ci];
+  const T al = (T)pick(a.alpha, ci), be = (T)pick(a.beta, ci);
   cplx<T> v[LSMAX][NC];
 #pragma unroll
   for (int t = 0; t < LSMAX; ++t) {
@@ -791,10 +801,10 @@ template <class S>
 __global__ void __launch_bounds__(kBlasBlock) k_axpy_norm_multi(AxpyNormArgs<S> a)
 {
   const int ci = blockIdx.y;
-  const int64_t coff = (int64_t)a.cols.rhs[ci] * a.cols.fstride;
+  const int64_t coff = (int64_t)pick(a.cols.rhs, ci) * a.cols.fstride;
   const S* __restrict__ ap = a.ap + coff;
   S* __restrict__ r = a.r + coff;
-  const double al = a.alpha[ci];
+  const double al = pick(a.alpha, ci);
   double v = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * kBlasBlock) {
     const S rn = slot_axpy(-al, ap[i], r[i]);
