@@ -184,6 +184,30 @@ int qlat_dwf_invert_prop(qlat_dwf_inverter* inv, double* sol, const double* src,
 int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,
                                               double* result);
 
+/* ---------------------------------------------------------------- force
+ * No counterpart in the reference (it has no fermion force); the
+ * conventions are those of its gauge force (hmc.cpp:64-75,
+ * matrix-hmc.h:202-213).
+ *
+ * S = phi^+ (Mpc^+ Mpc)^-1 phi for an odd half field phi ([V/2][Ls][12]);
+ * X = (Mpc^+ Mpc)^-1 phi is solved with the FP64 CG to stop_rsd.  Outputs
+ * (all [V][4][3][3] complex128):
+ *   force[x][mu] = T_A(U_mu(x) G_mu(x)),  T_A(M) = (M - M^+)/2 - Tr(M - M^+)/6,
+ *   grad[x][mu]  = G_mu(x)^+  (optional, may be NULL),
+ * where dS = sum Re Tr[G_mu(x) dU_mu(x)].  For U -> exp(eps T) U with T in
+ * su(3), dS/deps = sum Tr[force T]; grad is the derivative with respect to
+ * the complex link entries in the conjugate-Wirtinger convention
+ * (dS = Re sum conj(grad) dU).  action receives S, iter the CG iterations. */
+int qlat_dwf_fermion_force(qlat_dwf_inverter* inv, const double* phi, double stop_rsd, double* force, double* grad,
+                           double* action, long* iter);
+int qlat_dwf_fermion_force_device(qlat_dwf_inverter* inv, const double* phi, double stop_rsd, double* force,
+                                  double* grad, double* action, long* iter);
+
+/* Replace the inverter's gauge field (InverterDomainWall::setup(gf, fa)
+ * again, dslash.h:154-162) without rebuilding anything else; gauge is
+ * [V][4][3][3] complex128 in host (on_device = 0) or device memory. */
+int qlat_dwf_inverter_set_gauge(qlat_dwf_inverter* inv, const double* gauge, int on_device);
+
 /* ------------------------------------------ device-resident variants
  * Same semantics as above; all field pointers are device memory. */
 int qlat_dwf_multiply_hermop_sym2_device(qlat_dwf_inverter* inv, double* out, const double* in, int eo);

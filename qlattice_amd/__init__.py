@@ -50,5 +50,6 @@ from .inverter import (
     qnorm,
 )
 from ._lib import QlatDwfError, load as load_library
+from .force import fermion_force, pseudofermion_action, set_gauge
 
 __all__ = [n for n in dir() if not n.startswith("_")]

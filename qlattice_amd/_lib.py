@@ -53,6 +53,9 @@ EXPORTED = [
     "qlat_dwf_invert_5d_device",
     "qlat_dwf_invert_prop_device",
     "qlat_dwf_bench_kernel",
+    "qlat_dwf_fermion_force",
+    "qlat_dwf_fermion_force_device",
+    "qlat_dwf_inverter_set_gauge",
 ]
 
 
@@ -134,6 +137,9 @@ def load():
         getattr(L, name).argtypes = [_vp, _dp, _dp, _lp]
     L.qlat_dwf_find_max_eigen_value_hermop_sym2.argtypes = [_vp, _dp, ctypes.c_long, _dp]
     L.qlat_dwf_bench_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    for name in ("qlat_dwf_fermion_force", "qlat_dwf_fermion_force_device"):
+        getattr(L, name).argtypes = [_vp, _dp, ctypes.c_double, _dp, _dp, _dp, _lp]
+    L.qlat_dwf_inverter_set_gauge.argtypes = [_vp, _dp, ctypes.c_int]
     for name in EXPORTED:
         if name not in ("qlat_dwf_last_error", "qlat_dwf_abi_version"):
             getattr(L, name).restype = ctypes.c_int

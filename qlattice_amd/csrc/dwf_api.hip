@@ -1181,6 +1181,70 @@ long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
   return total;
 }
 
+// Pack an AoS gauge field already in device memory into the link layouts.
+void pack_gauge_dev(Inv& I, const double2* g)
+{
+  const size_t lcount = (size_t)2 * 8 * 9 * I.geo.stride;
+  if (!I.l64) {
+    I.l64 = (double2*)dev_alloc(lcount * sizeof(double2));
+    I.l32 = (float2*)dev_alloc(lcount * sizeof(float2));
+    QDWF_HIP(hipMemsetAsync(I.l64, 0, lcount * sizeof(double2), I.stream));
+    QDWF_HIP(hipMemsetAsync(I.l32, 0, lcount * sizeof(float2), I.stream));
+  }
+  qdwf::launch_pack_gauge(g, I.l64, I.l32, I.geo, I.stream);
+  check_launch();
+}
+
+// Fermion force of S = phi^+ (Mpc^+ Mpc)^-1 phi, phi an odd half field in
+// reference layout (device).  X = (Mpc^+ Mpc)^-1 phi is recomputed here by
+// the FP64 CG (nothing is kept between calls); with
+//   a = K X, c = K H a, Y = X - H c = Mpc X, w = K^+ H^+ Y
+// dS = -X^+ d(Mpc^+ Mpc) X = 2 Re[Y^+ dH c + w^+ dH a], whose link
+// derivative k_force evaluates with A = (w even, Y odd), B = (c even, a odd).
+long fermion_force_dev(Inv& I, const double2* phi, double stop_rsd, double2* force, double2* grad, double* action)
+{
+  double2* b = W(I, W_IN_OP);
+  double2* X = W(I, W_OUT_OP);
+  qdwf::launch_pack5(phi, b, I.geo, I.ls, 1, 0, I.stream);
+  check_launch();
+  zero64(I, X);
+  const long iter = cg_fp64(I, X, b, stop_rsd, I.max_num_iter * I.max_cycle, true);
+  I.stats.total_iter += iter;
+  I.stats.true_residual = true_residual_fp64(I, X, b);
+  double2* a1 = W(I, W_IN_E);
+  double2* c1 = W(I, W_IN_O);
+  double2* Y = W(I, W_OUT_E);
+  double2* w = W(I, W_OUT_O);
+  double2* tmp = W(I, W_TMP);
+  m5<double>(I, a1, X, MAT_K);               // a = K X          (odd)
+  hop<double>(I, tmp, a1, 2, false);          // H a              (even)
+  m5<double>(I, c1, tmp, MAT_K);              // c = K H a        (even)
+  hop<double>(I, Y, c1, 1, false, -1.0, X);   // Y = X - H c      (odd)
+  hop<double>(I, tmp, Y, 2, true);            // H^+ Y            (even)
+  m5<double>(I, w, tmp, MAT_K_DAG);           // w = K^+ H^+ Y    (even)
+  // S = 2 Re <phi, X> - |Mpc X|^2 differs from phi^+ A^-1 phi only by
+  // (X - A^-1 phi)^+ A (X - A^-1 phi): second order in the solver error.
+  // The sum is compensated so that S(U + dU) - S(U - dU) keeps its digits.
+  qdwf::launch_action_dd(b, X, Y, n64(I), I.part, I.red, I.stream);
+  check_launch();
+  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, sizeof(double), hipMemcpyDeviceToHost, I.stream));
+  sync(I);
+  *action = I.red_host[0];
+  qdwf::ForceArgs fa;
+  fa.geo = I.geo;
+  fa.ls = I.ls;
+  fa.A[0] = w;
+  fa.A[1] = Y;
+  fa.B[0] = c1;
+  fa.B[1] = a1;
+  fa.links = I.l64;
+  fa.force = force;
+  fa.grad = grad;
+  qdwf::launch_force(fa, I.stream);
+  check_launch();
+  return iter;
+}
+
 void build_matrices(Inv& I)
 {
   const int ls = I.ls;
@@ -1334,13 +1398,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     const size_t gbytes = (size_t)V * 36 * sizeof(double2);
     double2* g = (double2*)dev_alloc(gbytes);
     QDWF_HIP(hipMemcpyAsync(g, gauge, gbytes, hipMemcpyHostToDevice, I->stream));
-    const size_t lcount = (size_t)2 * 8 * 9 * I->geo.stride;
-    I->l64 = (double2*)dev_alloc(lcount * sizeof(double2));
-    I->l32 = (float2*)dev_alloc(lcount * sizeof(float2));
-    QDWF_HIP(hipMemsetAsync(I->l64, 0, lcount * sizeof(double2), I->stream));
-    QDWF_HIP(hipMemsetAsync(I->l32, 0, lcount * sizeof(float2), I->stream));
-    qdwf::launch_pack_gauge(g, I->l64, I->l32, I->geo, I->stream);
-    check_launch();
+    pack_gauge_dev(*I, g);
     QDWF_HIP(hipStreamSynchronize(I->stream));
     QDWF_HIP(hipFree(g));
     build_matrices(*I);
@@ -1674,6 +1732,71 @@ int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const doubl
     *iter = invert_prop_dev(*inv, (double2*)sol, (const double2*)src, true);
     sync(*inv);
     inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_inverter_set_gauge(qlat_dwf_inverter* inv, const double* gauge, int on_device)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(gauge, "null gauge pointer");
+    Inv& I = *inv;
+    if (on_device) {
+      pack_gauge_dev(I, (const double2*)gauge);
+    } else {
+      const size_t gbytes = (size_t)I.geo.vh * 2 * 36 * sizeof(double2);
+      double2* g = (double2*)dev_alloc(gbytes);
+      QDWF_HIP(hipMemcpyAsync(g, gauge, gbytes, hipMemcpyHostToDevice, I.stream));
+      pack_gauge_dev(I, g);
+      sync(I);
+      QDWF_HIP(hipFree(g));
+    }
+    sync(I);
+  });
+}
+
+int qlat_dwf_fermion_force(qlat_dwf_inverter* inv, const double* phi, double stop_rsd, double* force, double* grad,
+                           double* action, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(phi && force && action && iter && stop_rsd > 0, "invalid argument");
+    Inv& I = *inv;
+    reset_stats(I);
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t hbytes = half_aos_bytes(I);
+    const size_t gbytes = (size_t)I.geo.vh * 2 * 36 * sizeof(double2);
+    const double2* d_phi = upload(I, 0, phi, hbytes);
+    double2* d_f = (double2*)dev_alloc(gbytes);
+    double2* d_g = grad ? (double2*)dev_alloc(gbytes) : nullptr;
+    try {
+      *iter = fermion_force_dev(I, d_phi, stop_rsd, d_f, d_g, action);
+      QDWF_HIP(hipMemcpyAsync(force, d_f, gbytes, hipMemcpyDeviceToHost, I.stream));
+      if (grad) QDWF_HIP(hipMemcpyAsync(grad, d_g, gbytes, hipMemcpyDeviceToHost, I.stream));
+      sync(I);
+    } catch (...) {
+      (void)hipFree(d_f);
+      if (d_g) (void)hipFree(d_g);
+      throw;
+    }
+    QDWF_HIP(hipFree(d_f));
+    if (d_g) QDWF_HIP(hipFree(d_g));
+    I.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_fermion_force_device(qlat_dwf_inverter* inv, const double* phi, double stop_rsd, double* force,
+                                  double* grad, double* action, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(phi && force && action && iter && stop_rsd > 0, "invalid argument");
+    Inv& I = *inv;
+    reset_stats(I);
+    const auto t0 = std::chrono::steady_clock::now();
+    *iter = fermion_force_dev(I, (const double2*)phi, stop_rsd, (double2*)force, (double2*)grad, action);
+    sync(I);
+    I.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   });
 }
 

@@ -95,6 +95,20 @@ struct AxpyNormArgs {
   double* part;
 };
 
+// Fermion force: A[p], B[p] are FP64 half fields by parity slot (0 even,
+// 1 odd); links is the FP64 8-link layout; force/grad are [V][4][3][3]
+// complex128 (grad optional).
+struct ForceArgs {
+  LatGeo geo;
+  int ls;
+  const void* A[2];
+  const void* B[2];
+  const void* links;
+  void* force;
+  void* grad;
+};
+void launch_force(const ForceArgs& a, hipStream_t st);
+
 template <class T>
 void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st);
 // returns false (nothing launched) when ls is too large for the fused kernel
@@ -116,6 +130,9 @@ template <class S>
 void launch_norm2(const S* a, int64_t n, double* part, double* out, hipStream_t st);
 template <class S>
 void launch_dot(const S* a, const S* b, int64_t n, double* part, double* out, hipStream_t st);
+// out[0] = sum 2 Re conj(phi) x - |y|^2 with a compensated (twice-FP64) sum
+void launch_action_dd(const double2* phi, const double2* x, const double2* y, int64_t n, double* part, double* out,
+                      hipStream_t st);
 template <class S>
 void launch_axpy(double a, const S* x, S* y, int64_t n, hipStream_t st);
 template <class S>

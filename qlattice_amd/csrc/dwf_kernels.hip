@@ -408,6 +408,136 @@ bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st)
   return true;
 }
 
+// ============================================================ force
+// Link derivative of S = phi^+ (Mpc^+ Mpc)^-1 phi.  With full-lattice
+// fields A (Y on odd, w on even) and B (a on odd, c on even) -- see the host
+// code for their definition -- dS = sum_{y,mu} Re Tr[G_mu(y) dU_mu(y)] with
+//   G_mu(y) = - sum_s [ (1 - g_mu) B(y+mu) (x) A(y)^+  +  A(y+mu) (x) ((1 + g_mu) B(y))^+ ]
+// (outer products in colour, summed over spin and s).  The kernel writes
+// F_mu(y) = T_A(U_mu(y) G_mu(y)), T_A(M) = (M - M^+)/2 - Tr(M - M^+)/6, the
+// su(3) projection of the reference's gauge force (matrix-hmc.h:202-213,
+// hmc.cpp:64-75), and optionally G^+ (the gradient w.r.t. the complex link
+// entries in the conjugate-Wirtinger convention).
+// (1 + sg g_mu) psi = (w, sg B^+ w), w = u + sg B l  (dwf_common.h)
+template <int MU, int SG>
+__device__ __forceinline__ void full_proj(const cplx<double> psi[12], cplx<double> out[12])
+{
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    cplx<double> w0, w1;
+    proj<MU, SG>(psi[c], psi[3 + c], psi[6 + c], psi[9 + c], w0, w1);
+    cplx<double> l0 = cmk<double>(0, 0), l1 = cmk<double>(0, 0);
+    recon_lower<MU, SG>(w0, w1, l0, l1);
+    out[c] = w0;
+    out[3 + c] = w1;
+    out[6 + c] = l0;
+    out[9 + c] = l1;
+  }
+}
+
+// G += sign * sum_spin a (x) b^*  (G[c][c'] += a[spin][c] conj(b[spin][c']))
+__device__ __forceinline__ void outer_acc(cplx<double> G[9], const cplx<double> a[12], const cplx<double> b[12])
+{
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int cp = 0; cp < 3; ++cp) {
+      double re = 0, im = 0;
+#pragma unroll
+      for (int sp = 0; sp < 4; ++sp) {
+        const cplx<double> x = a[sp * 3 + c], y = b[sp * 3 + cp];
+        re += x.re * y.re + x.im * y.im;
+        im += x.im * y.re - x.re * y.im;
+      }
+      G[c * 3 + cp].re += re;
+      G[c * 3 + cp].im += im;
+    }
+}
+
+template <int MU>
+__device__ __forceinline__ void force_mu(const ForceArgs& a, int p, int64_t h, const int x[4], double2* __restrict__ fout,
+                                         double2* __restrict__ gout)
+{
+  int xp[4] = {x[0], x[1], x[2], x[3]};
+  xp[MU] = wrap_inc(x[MU], a.geo.n[MU]);
+  const int64_t hp = half_index(a.geo, xp);
+  const int q = 1 - p;  // parity slot of y + mu
+  cplx<double> G[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) G[e] = cmk<double>(0, 0);
+  for (int s = 0; s < a.ls; ++s) {
+    cplx<double> v[12], w[12], pv[12];
+    // (1 - g) B(y+mu) (x) A(y)^+
+    SpinorIO<double>::load((const double2*)a.B[q], a.ls, s, hp, v);
+    full_proj<MU, -1>(v, pv);
+    SpinorIO<double>::load((const double2*)a.A[p], a.ls, s, h, w);
+    outer_acc(G, pv, w);
+    // A(y+mu) (x) ((1 + g) B(y))^+
+    SpinorIO<double>::load((const double2*)a.B[p], a.ls, s, h, v);
+    full_proj<MU, 1>(v, pv);
+    SpinorIO<double>::load((const double2*)a.A[q], a.ls, s, hp, w);
+    outer_acc(G, w, pv);
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) G[e] = cneg(G[e]);
+  cplx<double> U[9];
+  LinkIO<double>::load((const double2*)a.links, a.geo.stride, p, MU, h, U);
+  // M = U G
+  cplx<double> M[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cplx<double> t = cmul(U[i * 3 + 0], G[0 * 3 + j]);
+      t = cfma(t, U[i * 3 + 1], G[1 * 3 + j]);
+      t = cfma(t, U[i * 3 + 2], G[2 * 3 + j]);
+      M[i * 3 + j] = t;
+    }
+  // T_A(M): anti-hermitian part, then remove the trace
+  double F[18];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      F[2 * (i * 3 + j)] = 0.5 * (M[i * 3 + j].re - M[j * 3 + i].re);
+      F[2 * (i * 3 + j) + 1] = 0.5 * (M[i * 3 + j].im + M[j * 3 + i].im);
+    }
+  const double tr = (F[1] + F[9] + F[17]) / 3.0;
+  F[1] -= tr;
+  F[9] -= tr;
+  F[17] -= tr;
+  const int64_t site = full_index(a.geo, x);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) fout[(site * 4 + MU) * 9 + e] = make_double2(F[2 * e], F[2 * e + 1]);
+  if (gout) {
+    // G^+ : gradient w.r.t. the complex entries of U (conjugate Wirtinger)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gout[(site * 4 + MU) * 9 + i * 3 + j] = make_double2(G[j * 3 + i].re, -G[j * 3 + i].im);
+  }
+}
+
+__global__ void __launch_bounds__(128) k_force(ForceArgs a)
+{
+  const int64_t h = (int64_t)blockIdx.x * 128 + threadIdx.x;
+  const int p = blockIdx.y;  // parity slot of y: 0 even, 1 odd
+  if (h >= a.geo.vh) return;
+  int x[4];
+  half_coord(a.geo, h, p == 0 ? 2 : 1, x);
+  double2* fout = (double2*)a.force;
+  double2* gout = (double2*)a.grad;
+  force_mu<0>(a, p, h, x, fout, gout);
+  force_mu<1>(a, p, h, x, fout, gout);
+  force_mu<2>(a, p, h, x, fout, gout);
+  force_mu<3>(a, p, h, x, fout, gout);
+}
+
+void launch_force(const ForceArgs& a, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_force, dim3((unsigned)((a.geo.vh + 127) / 128), 2), dim3(128), 0, st, a);
+}
+
 // ================================================================== M5
 // out = alpha * (Mat_chi in) + beta * x, where Mat_chi is a dense Ls x Ls
 // complex matrix per chirality (chi = 0: spins 0,1; chi = 1: spins 2,3).
@@ -749,6 +879,81 @@ __global__ void __launch_bounds__(kBlasBlock) k_dot(const S* __restrict__ a, con
   block_reduce_store<2>(v, part);
 }
 
+// Compensated sums (Ogita, Rump and Oishi, "Accurate sum and dot product",
+// SIAM J. Sci. Comput. 26 (2005), algorithms TwoSum / TwoProduct / Dot2):
+// the pair (s, c) carries the sum to about twice FP64 precision.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+  s = a + b;
+  const double z = s - a;
+  e = (a - (s - z)) + (b - z);
+}
+__device__ __forceinline__ void dot2_add(double& s, double& c, double a, double b)
+{
+  const double p = a * b;
+  const double q = fma(a, b, -p);  // exact: a*b = p + q
+  double r;
+  two_sum(s, p, s, r);
+  c += q + r;
+}
+
+// part[blk] = (s, c) of sum_i 2 Re conj(phi_i) x_i - |y_i|^2
+__global__ void __launch_bounds__(kBlasBlock) k_action_dd(const double2* __restrict__ phi,
+                                                          const double2* __restrict__ x,
+                                                          const double2* __restrict__ y, int64_t n, double* part)
+{
+  double s = 0.0, c = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
+    const double2 f = phi[i], a = x[i], b = y[i];
+    dot2_add(s, c, 2.0 * f.x, a.x);
+    dot2_add(s, c, 2.0 * f.y, a.y);
+    dot2_add(s, c, -b.x, b.x);
+    dot2_add(s, c, -b.y, b.y);
+  }
+  __shared__ double sh[kBlasBlock];
+  __shared__ double shc[kBlasBlock];
+  sh[threadIdx.x] = s;
+  shc[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = kBlasBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      double r;
+      two_sum(sh[threadIdx.x], sh[threadIdx.x + h], sh[threadIdx.x], r);
+      shc[threadIdx.x] += shc[threadIdx.x + h] + r;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0];
+    part[2 * blockIdx.x + 1] = shc[0];
+  }
+}
+
+// out[0] = compensated sum of nblk (s, c) pairs (one 256-thread block)
+__global__ void __launch_bounds__(256) k_reduce_dd(const double* __restrict__ part, int nblk, double* out)
+{
+  double s = 0.0, c = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) {
+    double r;
+    two_sum(s, part[2 * b], s, r);
+    c += part[2 * b + 1] + r;
+  }
+  __shared__ double sh[256];
+  __shared__ double shc[256];
+  sh[threadIdx.x] = s;
+  shc[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      double r;
+      two_sum(sh[threadIdx.x], sh[threadIdx.x + h], sh[threadIdx.x], r);
+      shc[threadIdx.x] += shc[threadIdx.x + h] + r;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0] + shc[0];
+}
+
 // y = a*x + y
 template <class S>
 __global__ void __launch_bounds__(kBlasBlock) k_axpy(double a, const S* __restrict__ x, S* __restrict__ y, int64_t n)
@@ -781,8 +986,6 @@ __global__ void __launch_bounds__(kBlasBlock) k_cg_update(double a, const S* __r
   block_reduce_store<1>(v, part);
 }
 
-// Sum nblk partial rows of nv values (nv <= 2) with one 256-thread block;
-// the summation order depends only on nblk, so results are reproducible.
 // r -= a ap ; partial |r|^2
 template <class S>
 __global__ void __launch_bounds__(kBlasBlock) k_axpy_norm(double a, const S* __restrict__ ap, S* __restrict__ r,
@@ -837,6 +1040,8 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ 
   if (threadIdx.x == 0) out[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// Sum nblk partial rows of nv values (nv <= 2) with one 256-thread block;
+// the summation order depends only on nblk, so results are reproducible.
 __global__ void __launch_bounds__(256) k_reduce_final(const double* __restrict__ part, int nblk, int nv, double* out)
 {
   __shared__ double sh[4][2];
@@ -921,6 +1126,13 @@ void launch_dot(const S* a, const S* b, int64_t n, double* part, double* out, hi
   const int g = blas_grid(n);
   hipLaunchKernelGGL(k_dot<S>, dim3(g), dim3(kBlasBlock), 0, st, a, b, n, part);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, g, 2, out);
+}
+void launch_action_dd(const double2* phi, const double2* x, const double2* y, int64_t n, double* part, double* out,
+                      hipStream_t st)
+{
+  const int g = blas_grid(n);
+  hipLaunchKernelGGL(k_action_dd, dim3(g), dim3(kBlasBlock), 0, st, phi, x, y, n, part);
+  hipLaunchKernelGGL(k_reduce_dd, dim3(1), dim3(256), 0, st, part, g, out);
 }
 template <class S>
 void launch_axpy(double a, const S* x, S* y, int64_t n, hipStream_t st)

@@ -110,6 +110,12 @@ class InverterDomainWall(Inverter):
                    "qlat_dwf_inverter_set_solver")
         self.solver = solver
 
+    def set_gauge(self, gf):
+        """Replace the links (InverterDomainWall::setup, dslash.h:154-162)."""
+        from .force import set_gauge
+
+        set_gauge(self, gf)
+
     def stats(self):
         st = _lib.SolveStatsC()
         _lib.check(_lib.load().qlat_dwf_inverter_get_stats(self.cdata, ctypes.byref(st)),
