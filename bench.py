@@ -4,8 +4,10 @@ Workload (the metric's configuration): 24^3 x 64 lattice, links
 U = exp(i 0.3 H) (seeded, antiperiodic in time), Moebius Ls = 12, M5 = 1.8,
 b = 1.5, c = 0.5, mass = 0.01; a 12-column point-source propagator solved
 to a 1e-8 FP64 true residual of the preconditioned odd system.  One step is
-one full propagator solve with the mixed-precision solver (FP32 CG with
-FP64 reliable updates); the FP64 solver is timed once on the same inputs.
+one full propagator solve with the mixed-precision multi-RHS solver (the 12
+columns as one block; FP32 arithmetic, Krylov vectors stored in FP16 with a
+per-site scale by default, FP64 reliable updates); the FP64 solver is timed
+once on the same inputs.
 Inputs (gauge field, source propagator) are resident in device memory
 before the timed region; solutions stay in device memory.
 
@@ -55,6 +57,28 @@ def hop_flops(site, ls):
     return 1320 * (V // 2) * ls
 
 
+def hop_m5s_bytes(site, ls, ncol, site_bytes):
+    """Algorithmic bytes of one multi-RHS hop + s-matrix launch over ncol
+    columns: read and write every column's 5D half field once (site_bytes
+    per 5D site and field: 52 = 48 FP16 data + 4 scale, or 96 FP32), read
+    the 8 FP32 links of every output site once."""
+    V = 1
+    for n in site:
+        V *= n
+    N = V // 2 * ls
+    return ncol * N * site_bytes * 2 + V * 4 * 18 * 4
+
+
+def read_traffic(kernel_tag):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE, calibrated against a copy kernel) and kept in profiles/."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def cpu_baseline(cfg, U, iters_fp64, sample_hermops):
     """The oracle (C++ restatement of the reference CPU path, FP64, OpenMP)
     on this host: time `sample_hermops` applications of Mpc^+ Mpc at the
@@ -91,6 +115,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", default="B", choices=sorted(CONFIGS))
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--solver", default="mixed16", choices=["mixed16", "mixed"],
+                    help="mixed16: FP16 Krylov storage with per-site scale; mixed: FP32 storage")
     ap.add_argument("--cpu-hermops", type=int, default=3, help="oracle hermop applications (0 = skip)")
     ap.add_argument("--kernel-iters", type=int, default=50)
     ap.add_argument("--kernels-only", action="store_true",
@@ -131,6 +157,17 @@ def main():
     m5_32_ms = q.bench_kernel(inv, 2, 32, args.kernel_iters)
     herm32_ms = q.bench_kernel(inv, 3, 32, max(1, args.kernel_iters // 5))
     herm64_ms = q.bench_kernel(inv, 3, 64, max(1, args.kernel_iters // 5))
+    tile = int(os.environ.get("QLAT_DWF_RHS_TILE", "12"))
+    prec_tp = 16 if args.solver == "mixed16" else 32
+    hm5s_ms = q.bench_kernel(inv, 5, prec_tp, max(1, args.kernel_iters // 5))
+    step_ms = q.bench_kernel(inv, 6, prec_tp, max(1, args.kernel_iters // 5))
+    hermtp_ms = q.bench_kernel(inv, 7, prec_tp, max(1, args.kernel_iters // 10))
+    site_bytes = 52 if prec_tp == 16 else 96
+    bm5s = hop_m5s_bytes(site, ls, tile, site_bytes)
+    gbs_m5s = bm5s / (hm5s_ms * 1e-3) / 1e9
+    log(f"three-phase ({prec_tp}-bit storage, {tile} columns): hop+s-matrix {hm5s_ms:.3f} ms "
+        f"{gbs_m5s:.0f} GB/s | cg step {step_ms:.3f} ms | hermop+dot {hermtp_ms:.3f} ms "
+        f"({hermtp_ms / tile:.3f} ms per column)")
     b32 = hop_bytes(site, ls, 4)
     b64 = hop_bytes(site, ls, 8)
     gbs32 = b32 / (hop32_ms * 1e-3) / 1e9
@@ -147,7 +184,7 @@ def main():
     if args.profile_column:
         src4 = q.FermionField4d(geo)
         src4.data[0, 0] = 1.0
-        for solver in ("mixed", "fp64"):
+        for solver in (args.solver, "fp64"):
             inv.set_solver(solver)
             t0 = time.perf_counter()
             inv * src4
@@ -173,12 +210,12 @@ def main():
 
     for w in range(args.warmup):
         log(f"warmup {w}")
-        solve("mixed")
-    log(f"timed mixed solves: {args.steps}")
+        solve(args.solver)
+    log(f"timed {args.solver} solves: {args.steps}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        st_mixed = solve("mixed")
+        st_mixed = solve(args.solver)
         log(f"step {s}: iters {st_mixed['total_iter']} reliable {st_mixed['reliable_updates']} "
             f"true_res {st_mixed['true_residual']:.2e} solve {st_mixed['seconds']:.2f} s")
     torch.cuda.synchronize()
@@ -206,6 +243,8 @@ def main():
         cpu = cpu_baseline(cfg, U, iters_ref, args.cpu_hermops)
 
     N = V // 2 * ls
+    tr = read_traffic(f"hop_m5s_{'fp16' if prec_tp == 16 else 'fp32'}_{args.config}_tile{tile}")
+    traffic_bytes = None if tr is None else tr["hbm_bytes_per_launch"]
     line = {
         "metric": METRIC,
         "value": t_mixed,
@@ -217,16 +256,19 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (inner CG) + fp64 (reliable updates, solution)",
+        "dtype": "fp32",
         "data": "synthetic: seeded exp(i*0.3*H) SU(3) links, antiperiodic t, point sources",
         "config": {
-            "workload": f"12-column point-source propagator, {site[0]}^3x{site[3]}, Ls={ls}, Moebius b=1.5 c=0.5, "
-                        f"M5=1.8, mass=0.01, stop 1e-8 (FP64 true residual), single-RHS mixed solver",
-            "lattice": list(site), "ls": ls, "columns": ncol, "stop_rsd": 1e-8,
+            "workload": f"12-column point-source propagator, {site[0]}^{3}x{site[3]}, Ls={ls}, Moebius b=1.5 c=0.5, "
+                        f"M5=1.8, mass=0.01, stop 1e-8 (FP64 true residual); columns solved as one multi-RHS "
+                        f"block of {tile}, FP32 arithmetic, {prec_tp}-bit Krylov storage"
+                        + (" with per-site scale" if prec_tp == 16 else "") + ", FP64 reliable updates",
+            "lattice": list(site), "ls": ls, "columns": ncol, "stop_rsd": 1e-8, "rhs_tile": tile,
+            "solver": args.solver,
         },
         "mixed": {"seconds": t_mixed, "iters": st_mixed["total_iter"],
                   "reliable_updates": st_mixed["reliable_updates"],
-                  "true_residual": st_mixed["true_residual"]},
+                  "true_residual": st_mixed["true_residual"], "spinor_bits": st_mixed["spinor_bits"]},
         "fp64": None if st64 is None else {"seconds": t_fp64, "iters": st64["total_iter"],
                                            "cycles": st64["cycles"], "true_residual": st64["true_residual"]},
         "speedup_vs_fp64": None if t_fp64 is None else t_fp64 / t_mixed,
@@ -236,17 +278,21 @@ def main():
             "hop_fp64_ms": hop64_ms, "hop_fp64_gbps": gbs64,
             "m5_fp32_ms": m5_32_ms, "hermop_fp32_ms": herm32_ms, "hermop_fp64_ms": herm64_ms,
             "hermop_ref_flops_convention": 5500 * ls * V,
+            "three_phase": {"storage_bits": prec_tp, "columns": tile, "hop_m5_ms": hm5s_ms,
+                            "hop_m5_gbps": gbs_m5s, "cg_step_ms": step_ms, "hermop_dot_ms": hermtp_ms},
         },
         "roofline": {
-            "kernel": "k_hop<float> (Wilson hop, FP32, 18-real links)",
+            "kernel": f"k_hop_m5s<{'Half16' if prec_tp == 16 else 'float'}> (hop + s-matrix, {tile} columns "
+                      f"per launch, {prec_tp}-bit spinor storage, FP32 links)",
             "bound": "hbm",
-            "achieved": gbs32,
+            "achieved": gbs_m5s,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
-            "frac": gbs32 / PEAK_HBM_GBS,
-            "traffic": None,
-            "algorithmic_bytes_per_launch": b32,
-            "units_per_launch": N,
+            "frac": gbs_m5s / PEAK_HBM_GBS,
+            "traffic": traffic_bytes,
+            "algorithmic_bytes_per_launch": bm5s,
+            "units_per_launch": tile * N,
+            "launch_ms": hm5s_ms,
         },
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }

@@ -46,6 +46,9 @@ extern "C" {
 /* Solver of the odd-site normal equations (qlat_dwf_inverter_set_solver). */
 #define QLAT_DWF_SOLVER_FP64 0       /* the reference algorithm, FP64 throughout */
 #define QLAT_DWF_SOLVER_MIXED_FP32 1 /* FP32 CG with FP64 reliable updates      */
+#define QLAT_DWF_SOLVER_MIXED_FP16 2 /* as MIXED_FP32, Krylov vectors stored in FP16
+                                        with a per-5D-site scale (6 <= ls <= 16, ls
+                                        even; FP32 storage otherwise)              */
 
 typedef struct qlat_dwf_inverter qlat_dwf_inverter;
 
@@ -73,6 +76,8 @@ typedef struct {
   long hermop_low;        /* FP32 applications of Mpc^+ Mpc                      */
   double true_residual;   /* max over columns of |b' - A x| / |b'| in FP64 (odd) */
   double seconds;         /* wall clock of the solve call                        */
+  int spinor_bits;        /* storage of the low-precision Krylov vectors (16/32),
+                             0 for the FP64 solver                              */
 } qlat_dwf_solve_stats;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -197,11 +202,15 @@ int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const doub
  * where dS = sum Re Tr[G_mu(x) dU_mu(x)].  For U -> exp(eps T) U with T in
  * su(3), dS/deps = sum Tr[force T]; grad is the derivative with respect to
  * the complex link entries in the conjugate-Wirtinger convention
- * (dS = Re sum conj(grad) dU).  action receives S, iter the CG iterations. */
+ * (dS = Re sum conj(grad) dU).  action receives S, evaluated as
+ * 2 Re <phi, X> - |Mpc X|^2 (second order in the solver error) with a
+ * compensated sum; action_lo (optional, may be NULL) receives S - action, the
+ * low-order part of that sum, so that differences of nearby actions keep
+ * their digits.  iter receives the CG iterations. */
 int qlat_dwf_fermion_force(qlat_dwf_inverter* inv, const double* phi, double stop_rsd, double* force, double* grad,
-                           double* action, long* iter);
+                           double* action, double* action_lo, long* iter);
 int qlat_dwf_fermion_force_device(qlat_dwf_inverter* inv, const double* phi, double stop_rsd, double* force,
-                                  double* grad, double* action, long* iter);
+                                  double* grad, double* action, double* action_lo, long* iter);
 
 /* Replace the inverter's gauge field (InverterDomainWall::setup(gf, fa)
  * again, dslash.h:154-162) without rebuilding anything else; gauge is
@@ -218,7 +227,10 @@ int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const doubl
  * Average duration (ms) of one launch of a kernel, timed with HIP events
  * on the inverter's stream over `iters` back-to-back launches after one
  * warm-up launch.  which: 0 hop H, 1 hop H^+, 2 dense s-matrix (M_ee^-1),
- * 3 Mpc^+ Mpc (all its kernels), 4 plain copy of one field.  precision: 64 or 32.
+ * 3 Mpc^+ Mpc (all its kernels), 4 plain copy of one field (precision 64
+ * or 32); on the multi-RHS tile of the three-phase solver (QLAT_DWF_RHS_TILE
+ * columns, precision 16 = FP16 storage or 32): 5 hop + s-matrix (b = K H t),
+ * 6 CG step (x += a p, p = r + b p, t = K p), 7 Mpc^+ Mpc with the CG dot.
  * Has no counterpart in the reference (its Timer, timer.h:290, is host
  * wall clock); it exists to report kernel bandwidth. */
 int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int iters, double* avg_ms);

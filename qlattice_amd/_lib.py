@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libqlat_dwf_hip.so")
 
 QLAT_DWF_SOLVER_FP64 = 0
 QLAT_DWF_SOLVER_MIXED_FP32 = 1
+QLAT_DWF_SOLVER_MIXED_FP16 = 2
 
 # Every symbol declared in include/qlat_dwf.h.
 EXPORTED = [
@@ -82,6 +83,7 @@ class SolveStatsC(ctypes.Structure):
         ("hermop_low", ctypes.c_long),
         ("true_residual", ctypes.c_double),
         ("seconds", ctypes.c_double),
+        ("spinor_bits", ctypes.c_int),
     ]
 
 
@@ -138,7 +140,7 @@ def load():
     L.qlat_dwf_find_max_eigen_value_hermop_sym2.argtypes = [_vp, _dp, ctypes.c_long, _dp]
     L.qlat_dwf_bench_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     for name in ("qlat_dwf_fermion_force", "qlat_dwf_fermion_force_device"):
-        getattr(L, name).argtypes = [_vp, _dp, ctypes.c_double, _dp, _dp, _dp, _lp]
+        getattr(L, name).argtypes = [_vp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _lp]
     L.qlat_dwf_inverter_set_gauge.argtypes = [_vp, _dp, ctypes.c_int]
     for name in EXPORTED:
         if name not in ("qlat_dwf_last_error", "qlat_dwf_abi_version"):

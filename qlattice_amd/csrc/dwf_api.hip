@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <memory>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -166,6 +167,7 @@ struct qlat_dwf_inverter {
   // hop launch shapes (launch_hop): LDS-staged links; FP64 with streaming stores
   int hop_spt64 = -102, hop_spt32 = -4;
   bool fused32 = true;  // FP32 hop + s-matrix fusion (QLAT_DWF_FUSED=0 disables)
+  bool three_phase32 = false;  // FP32 storage on the three-phase kernels (QLAT_DWF_THREE_PHASE=1)
   bool real_mat[16] = {};  // s-matrix MatId has only real entries (Moebius)
 
   double2* l64 = nullptr;
@@ -186,6 +188,7 @@ struct qlat_dwf_inverter {
   int rhs_tile = 12;  // columns per multi-RHS solve (QLAT_DWF_RHS_TILE)
   int mcap = 0;
   float4* m32[8] = {};
+  uint4* mh[5] = {};            // FP16 (per-site scale) stacks of the three-phase solver
   std::vector<double2*> col64;  // 3 per column: b, x, out_e_p
 
   qlat_dwf_solve_stats stats{};
@@ -239,6 +242,11 @@ float4* MV(Inv& I, int id, int ncol)
         QDWF_HIP(hipFree(p));
         p = nullptr;
       }
+    for (auto*& p : I.mh)
+      if (p) {
+        QDWF_HIP(hipFree(p));
+        p = nullptr;
+      }
     I.mcap = ncol;
   }
   if (!I.m32[id]) {
@@ -246,6 +254,20 @@ float4* MV(Inv& I, int id, int ncol)
     QDWF_HIP(hipMemsetAsync(I.m32[id], 0, field32_bytes(I) * I.mcap, I.stream));
   }
   return I.m32[id];
+}
+
+// FP16 stacks (16-byte units per field: fieldh_units)
+enum MultiH { H_P, H_AP, H_T, H_B, H_Y };
+int64_t fieldh_units(const Inv& I) { return (I.geo.stride >> 6) * I.ls * 208; }
+uint4* MH(Inv& I, int id, int ncol)
+{
+  MV(I, M_R, ncol);  // settles mcap (frees every stack when it grows)
+  if (!I.mh[id]) {
+    const size_t bytes = (size_t)fieldh_units(I) * sizeof(uint4) * I.mcap;
+    I.mh[id] = (uint4*)dev_alloc(bytes);
+    QDWF_HIP(hipMemsetAsync(I.mh[id], 0, bytes, I.stream));
+  }
+  return I.mh[id];
 }
 
 // Per-column FP64 field: kind 0 = odd-prec source b, 1 = solution x, 2 = out_e_p.
@@ -864,6 +886,254 @@ long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
   return total;
 }
 
+// Krylov stacks and passes of the three-phase solver.  p, A p, t = K p and
+// the operator temporaries b, y are stored in FP16 with a per-5D-site scale
+// (half = true) or in FP32; r and the increment x are FP32 stacks.
+struct ThreePhase {
+  Inv& I;
+  bool half;
+  int64_t f32, fh;  // column strides of the FP32 and of the low-storage stacks
+  float4 *R, *X;
+  void *P, *AP, *T, *B, *Y;
+  ThreePhase(Inv& inv, int ncol, bool h) : I(inv), half(h)
+  {
+    if (!qdwf::three_phase_ok(I.ls)) throw ArgError("the three-phase kernels need 6 <= ls <= 16, ls even");
+    f32 = n32(I);
+    fh = half ? fieldh_units(I) : f32;
+    R = MV(I, M_R, ncol);
+    X = MV(I, M_X, ncol);
+    if (half) {
+      P = MH(I, H_P, ncol);
+      AP = MH(I, H_AP, ncol);
+      T = MH(I, H_T, ncol);
+      B = MH(I, H_B, ncol);
+      Y = MH(I, H_Y, ncol);
+    } else {
+      P = MV(I, M_P, ncol);
+      AP = MV(I, M_AP, ncol);
+      T = MV(I, M_T, ncol);
+      B = MV(I, M_B, ncol);
+      Y = MV(I, M_Y, ncol);
+    }
+  }
+  size_t slot_bytes() const { return half ? sizeof(uint4) : sizeof(float4); }
+  void* at(void* base, int c) const { return (void*)((char*)base + (size_t)c * fh * slot_bytes()); }
+  int64_t n5() const { return (int64_t)I.geo.stride * I.ls; }
+  qdwf::RhsSet colset(const std::vector<int>& act) const
+  {
+    qdwf::RhsSet cs;
+    cs.n = (int)act.size();
+    cs.fstride = fh;
+    for (int i = 0; i < qdwf::kMaxRhs; ++i) cs.rhs[i] = i < cs.n ? act[i] : 0;
+    return cs;
+  }
+  // x += alpha p (updx); p = r + beta p; t = K p
+  void step(const std::vector<int>& cols, const double* alpha, const double* beta, bool updx)
+  {
+    qdwf::CgStepArgs<float> a;
+    a.geo = I.geo;
+    a.ls = I.ls;
+    for (int i = 0; i < qdwf::kMaxRhs; ++i) {
+      a.alpha[i] = i < (int)cols.size() ? alpha[i] : 0.0;
+      a.beta[i] = i < (int)cols.size() ? beta[i] : 0.0;
+    }
+    a.cols = colset(cols);
+    a.xr_fstride = f32;
+    a.mat = Prec<float>::mat(I, MAT_K);
+    a.real_mat = I.real_mat[MAT_K];
+    a.x = updx ? X : nullptr;
+    a.p = P;
+    a.r = R;
+    a.t = T;
+    qdwf::launch_cg_step_s(a, half, I.stream);
+    check_launch();
+  }
+  void hopm5(void* out, const void* in, int eo_out, bool dag, int mat, float alpha, const void* x, float beta,
+             const void* dotp, const qdwf::RhsSet& cs)
+  {
+    qdwf::HopM5Args a;
+    a.dotp = dotp;
+    a.part = I.part;
+    a.cols = cs;
+    a.geo = I.geo;
+    a.ls = I.ls;
+    a.eo_out = eo_out;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.in = in;
+    a.out = out;
+    a.x = x;
+    a.links = I.l32;
+    a.mat = Prec<float>::mat(I, mat);
+    a.real_mat = I.real_mat[mat];
+    qdwf::launch_hop_m5s(a, dag, half, I.stream);
+    check_launch();
+  }
+  // ap = Mpc^+ Mpc p on parity eo given t = K p, as y = p - H K H t,
+  // ap = y - K^+ H^+ K^+ H^+ y; Re <p, ap> per column into I.red[i]
+  void hermop_dot(const qdwf::RhsSet& cs, int eo)
+  {
+    hopm5(B, T, 3 - eo, false, MAT_K, 1.f, nullptr, 0.f, nullptr, cs);
+    qdwf::HopArgs<float> a;
+    a.cols = cs;
+    a.geo = I.geo;
+    a.ls = I.ls;
+    a.eo_out = eo;
+    a.nchunk = 0;
+    a.alpha = -1.f;
+    a.in = B;
+    a.out = Y;
+    a.x = P;
+    a.beta = Prec<float>::beta(I, 0);
+    a.links = I.l32;
+    if (half) qdwf::launch_hop_h16(a, false, I.stream);
+    else qdwf::launch_hop<float>(a, false, I.hop_spt32, I.stream);
+    check_launch();
+    hopm5(B, Y, 3 - eo, true, MAT_K_DAG, 1.f, nullptr, 0.f, nullptr, cs);
+    hopm5(AP, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, P, cs);
+    qdwf::launch_reduce_rows(I.part, (int)qdwf::hop_m5_blocks(I.geo), cs.n, I.red, I.stream);
+    check_launch();
+    I.stats.hermop_low += cs.n;
+  }
+  // r_i -= alpha_i ap_i, |r_i|^2 into I.red[i]
+  void axpy_norm(const qdwf::RhsSet& cs, const double* alpha)
+  {
+    qdwf::AxpyNormSiteArgs an;
+    an.ap = AP;
+    an.r = R;
+    an.n5 = n5();
+    an.ls = I.ls;
+    an.cols = cs;
+    an.r_fstride = f32;
+    an.part = I.part;
+    for (int i = 0; i < qdwf::kMaxRhs; ++i) an.alpha[i] = i < cs.n ? alpha[i] : 0.0;
+    const int g = qdwf::launch_axpy_norm_site(an, half, I.stream);
+    check_launch();
+    qdwf::launch_reduce_rows(I.part, g, cs.n, I.red, I.stream);
+    check_launch();
+  }
+  // x_c (FP32) += alpha * v_c for a low-storage stack v
+  void axpy_to_x(double alpha, void* v, int c)
+  {
+    qdwf::launch_axpy_site(alpha, at(v, c), X + c * f32, I.ls, n5(), half, I.stream);
+    check_launch();
+  }
+};
+
+void read_red(Inv& I, int n, double* out)
+{
+  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, n * sizeof(double), hipMemcpyDeviceToHost, I.stream));
+  sync(I);
+  for (int i = 0; i < n; ++i) out[i] = I.red_host[i];
+}
+
+// Multi-RHS mixed-precision CG on the three-phase kernels (ThreePhase
+// storage).  Arithmetic is FP32; the FP64 side is as in cg_mixed_multi:
+// per-column reliable updates (x64 += x, r64 = b - A x64 in FP64) when a
+// column's iterated residual dropped by delta, and retirement on the FP64
+// true residual.  b64[c], x64[c]: FP64 source and solution (x64 is
+// overwritten).  Returns the summed iterations.
+long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64, double stop_rsd, long cap,
+                      bool half)
+{
+  // Each column is solved for b / |b| (the low-precision vectors stay O(1)
+  // whatever the source's scale); rr, targets and alpha are in those units.
+  ThreePhase tp(I, ncol, half);
+  const int64_t nd = n64(I);
+  double2* r64 = W(I, W_CG_R);
+  double2* ax64 = W(I, W_CG_AP);
+  std::vector<double> b2(ncol), bn(ncol), rr(ncol), maxrr(ncol);
+  std::vector<long> iters(ncol, 0);
+  std::vector<int> active;
+  const double target = stop_rsd * stop_rsd;
+  for (int c = 0; c < ncol; ++c) {
+    b2[c] = dev_norm2(I, b64[c]);
+    bn[c] = std::sqrt(b2[c]);
+    zero64(I, x64[c]);
+    if (b2[c] == 0.0) continue;  // x = 0 exactly
+    qdwf::launch_d2f(b64[c], tp.R + c * tp.f32, I.ls, I.geo.stride, I.stream, 1.0 / bn[c]);  // x = 0: r = b
+    check_launch();
+    rr[c] = maxrr[c] = 1.0;
+    if (rr[c] > target) active.push_back(c);
+  }
+  QDWF_HIP(hipMemsetAsync(tp.P, 0, (size_t)tp.fh * tp.slot_bytes() * ncol, I.stream));
+  QDWF_HIP(hipMemsetAsync(tp.X, 0, field32_bytes(I) * ncol, I.stream));
+  const std::vector<double> zero(qdwf::kMaxRhs, 0.0);
+  if (!active.empty()) tp.step(active, zero.data(), zero.data(), false);  // p = r, t = K p
+  const double d2 = I.delta * I.delta;
+  // |b - A x64|^2 / |b|^2 with r64 = b - A x64
+  auto true_residual = [&](int c) {
+    op_hermop<double>(I, ax64, x64[c], 1);
+    I.stats.hermop_fp64 += 1;
+    copy64(I, r64, b64[c]);
+    qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
+    check_launch();
+    return dev_norm2(I, r64) / b2[c];
+  };
+  double pap[qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs];
+  long it = 0;
+  while (!active.empty() && it < cap) {
+    const qdwf::RhsSet cs = tp.colset(active);
+    tp.hermop_dot(cs, 1);
+    read_red(I, cs.n, pap);
+    for (int i = 0; i < cs.n; ++i) alpha[i] = rr[active[i]] / pap[i];
+    tp.axpy_norm(cs, alpha);
+    read_red(I, cs.n, rr_new);
+    ++it;
+    std::vector<int> keep;
+    std::vector<double> ka, kb;
+    for (int i = 0; i < cs.n; ++i) {
+      const int c = active[i];
+      iters[c] += 1;
+      if (!std::isfinite(rr_new[i]) || !std::isfinite(alpha[i]))
+        throw std::runtime_error(half ? "CG breakdown: non-finite residual (FP16 storage, multi)"
+                                      : "CG breakdown: non-finite residual (FP32 three-phase, multi)");
+      double step_alpha = alpha[i];
+      if (rr_new[i] <= d2 * maxrr[c] || rr_new[i] <= target) {
+        float4* xc = tp.X + c * tp.f32;
+        tp.axpy_to_x(alpha[i], tp.P, c);
+        qdwf::launch_f2d(xc, x64[c], I.ls, I.geo.stride, 1, I.stream, bn[c]);
+        check_launch();
+        QDWF_HIP(hipMemsetAsync(xc, 0, field32_bytes(I), I.stream));
+        const double rr_true = true_residual(c);
+        I.stats.reliable_updates += 1;
+        if (rr_true <= target) {
+          const double rel = std::sqrt(rr_true);
+          if (rel > I.stats.true_residual) I.stats.true_residual = rel;
+          continue;  // column retired
+        }
+        qdwf::launch_d2f(r64, tp.R + c * tp.f32, I.ls, I.geo.stride, I.stream, 1.0 / bn[c]);
+        check_launch();
+        rr_new[i] = rr_true;
+        maxrr[c] = rr_true;
+        step_alpha = 0.0;
+      }
+      keep.push_back(c);
+      ka.push_back(step_alpha);
+      kb.push_back(rr_new[i] / rr[c]);
+      rr[c] = rr_new[i];
+    }
+    if (!keep.empty()) tp.step(keep, ka.data(), kb.data(), true);
+    active.swap(keep);
+  }
+  long total = 0;
+  for (int c = 0; c < ncol; ++c) total += iters[c];
+  for (int c : active) {  // cap reached: fold in what we have
+    qdwf::launch_f2d(tp.X + c * tp.f32, x64[c], I.ls, I.geo.stride, 1, I.stream, bn[c]);
+    check_launch();
+    const double rel = std::sqrt(true_residual(c));
+    if (rel > I.stats.true_residual) I.stats.true_residual = rel;
+    total += 1;
+  }
+  I.stats.spinor_bits = half ? 16 : 32;
+  return total;
+}
+
+bool use_three_phase(const Inv& I)
+{
+  return (I.solver == QLAT_DWF_SOLVER_MIXED_FP16 || I.three_phase32) && qdwf::three_phase_ok(I.ls);
+}
+
 double true_residual_fp64(Inv& I, const double2* x, const double2* b)
 {
   double2* r = W(I, W_CG_R);
@@ -933,10 +1203,19 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
     }
     I.stats.cycles += (cycle > I.max_cycle) ? I.max_cycle : cycle;
     true_rel = true_residual_fp64(I, out_o_p, in_o_p);
+  } else if (use_three_phase(I)) {
+    const long cap = I.max_num_iter * I.max_cycle;
+    const double saved = I.stats.true_residual;
+    I.stats.true_residual = 0.0;
+    total_iter = cg_mixed_multi_s(I, 1, &in_o_p, &out_o_p, I.stop_rsd, cap, I.solver == QLAT_DWF_SOLVER_MIXED_FP16);
+    true_rel = I.stats.true_residual;
+    I.stats.true_residual = saved;
+    I.stats.cycles += 1;
   } else {
     const long cap = I.max_num_iter * I.max_cycle;
     total_iter = cg_mixed(I, out_o_p, in_o_p, I.stop_rsd, cap, &true_rel);
     I.stats.cycles += 1;
+    I.stats.spinor_bits = 32;
   }
   I.stats.total_iter += total_iter;
   if (true_rel > I.stats.true_residual) I.stats.true_residual = true_rel;
@@ -1141,7 +1420,7 @@ long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
   long total = 0;
   const int ncol = is_prop ? 12 : 1;
   const int tile = std::min(I.rhs_tile, qdwf::kMaxRhs);
-  if (I.solver == QLAT_DWF_SOLVER_MIXED_FP32 && is_prop && tile > 1) {
+  if (I.solver != QLAT_DWF_SOLVER_FP64 && is_prop && tile > 1) {
     // multi-RHS: columns in tiles solved as one block
     for (int c0 = 0; c0 < ncol; c0 += tile) {
       const int w = std::min(tile, ncol - c0);
@@ -1156,7 +1435,19 @@ long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
           zero64(I, C64(I, j, 0));
         }
       }
-      total += cg_mixed_multi(I, w, I.stop_rsd, I.max_num_iter * I.max_cycle);
+      if (use_three_phase(I)) {
+        double2* bs[qdwf::kMaxRhs];
+        double2* xs[qdwf::kMaxRhs];
+        for (int j = 0; j < w; ++j) {
+          bs[j] = C64(I, j, 0);
+          xs[j] = C64(I, j, 1);
+        }
+        total += cg_mixed_multi_s(I, w, bs, xs, I.stop_rsd, I.max_num_iter * I.max_cycle,
+                                  I.solver == QLAT_DWF_SOLVER_MIXED_FP16);
+      } else {
+        total += cg_mixed_multi(I, w, I.stop_rsd, I.max_num_iter * I.max_cycle);
+        I.stats.spinor_bits = 32;
+      }
       I.stats.cycles += 1;
       for (int j = 0; j < w; ++j) {
         restore_column(I, C64(I, j, 1), C64(I, j, 2), oe, oo);
@@ -1201,7 +1492,8 @@ void pack_gauge_dev(Inv& I, const double2* g)
 //   a = K X, c = K H a, Y = X - H c = Mpc X, w = K^+ H^+ Y
 // dS = -X^+ d(Mpc^+ Mpc) X = 2 Re[Y^+ dH c + w^+ dH a], whose link
 // derivative k_force evaluates with A = (w even, Y odd), B = (c even, a odd).
-long fermion_force_dev(Inv& I, const double2* phi, double stop_rsd, double2* force, double2* grad, double* action)
+long fermion_force_dev(Inv& I, const double2* phi, double stop_rsd, double2* force, double2* grad, double* action,
+                       double* action_lo)
 {
   double2* b = W(I, W_IN_OP);
   double2* X = W(I, W_OUT_OP);
@@ -1227,9 +1519,10 @@ long fermion_force_dev(Inv& I, const double2* phi, double stop_rsd, double2* for
   // The sum is compensated so that S(U + dU) - S(U - dU) keeps its digits.
   qdwf::launch_action_dd(b, X, Y, n64(I), I.part, I.red, I.stream);
   check_launch();
-  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, sizeof(double), hipMemcpyDeviceToHost, I.stream));
+  QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, 2 * sizeof(double), hipMemcpyDeviceToHost, I.stream));
   sync(I);
   *action = I.red_host[0];
+  if (action_lo) *action_lo = I.red_host[1];
   qdwf::ForceArgs fa;
   fa.geo = I.geo;
   fa.ls = I.ls;
@@ -1325,6 +1618,7 @@ void free_all(Inv* I)
   for (auto* p : I->w32) f(p);
   for (auto* p : I->stage) f(p);
   for (auto* p : I->m32) f(p);
+  for (auto* p : I->mh) f(p);
   for (auto* p : I->col64) f(p);
   f(I->part);
   f(I->red);
@@ -1340,7 +1634,7 @@ extern "C" {
 
 const char* qlat_dwf_last_error(void) { return g_last_error.c_str(); }
 
-int qlat_dwf_abi_version(void) { return 1; }
+int qlat_dwf_abi_version(void) { return 2; }
 
 int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], const double* gauge,
                              const qlat_dwf_fermion_action* fa)
@@ -1406,6 +1700,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT32")) I->hop_spt32 = std::atoi(e);
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT64")) I->hop_spt64 = std::atoi(e);
     if (const char* e = std::getenv("QLAT_DWF_FUSED")) I->fused32 = std::atoi(e) != 0;
+    if (const char* e = std::getenv("QLAT_DWF_THREE_PHASE")) I->three_phase32 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_RHS_TILE")) I->rhs_tile = std::max(1, std::min(qdwf::kMaxRhs, std::atoi(e)));
     {
       // partial sums: BLAS grids, one per 64-site block (fused kernels), or
@@ -1461,7 +1756,9 @@ int qlat_dwf_inverter_set_solver(qlat_dwf_inverter* inv, int solver, double reli
 {
   return guarded([&] {
     QDWF_REQUIRE(inv, "null inverter handle");
-    QDWF_REQUIRE(solver == QLAT_DWF_SOLVER_FP64 || solver == QLAT_DWF_SOLVER_MIXED_FP32, "unknown solver");
+    QDWF_REQUIRE(solver == QLAT_DWF_SOLVER_FP64 || solver == QLAT_DWF_SOLVER_MIXED_FP32 ||
+                     solver == QLAT_DWF_SOLVER_MIXED_FP16,
+                 "unknown solver");
     QDWF_REQUIRE(reliable_delta >= 0 && reliable_delta < 1, "reliable_delta must be in [0, 1)");
     inv->solver = solver;
     inv->delta = reliable_delta > 0 ? reliable_delta : 0.1;
@@ -1681,10 +1978,25 @@ int qlat_dwf_multiply_hermop_sym2_fp32(qlat_dwf_inverter* inv, double* out, cons
     double2* a = W(I, W_IN_E);
     double2* b = W(I, W_OUT_E);
     qdwf::launch_pack5(d_in, a, I.geo, I.ls, eo, 0, I.stream);
-    qdwf::launch_d2f(a, V(I, V_R), I.ls, I.geo.stride, I.stream);
-    check_launch();
-    op_hermop<float>(I, V(I, V_AP), V(I, V_R), eo);
-    qdwf::launch_f2d(V(I, V_AP), b, I.ls, I.geo.stride, 0, I.stream);
+    if (use_three_phase(I)) {
+      // the inner operator of the three-phase solver: in -> r -> p (storage
+      // rounding), t = K p, ap = A p, ap -> FP32 -> FP64
+      ThreePhase tp(I, 1, I.solver == QLAT_DWF_SOLVER_MIXED_FP16);
+      qdwf::launch_d2f(a, tp.R, I.ls, I.geo.stride, I.stream);
+      check_launch();
+      QDWF_HIP(hipMemsetAsync(tp.P, 0, (size_t)tp.fh * tp.slot_bytes(), I.stream));
+      QDWF_HIP(hipMemsetAsync(tp.X, 0, field32_bytes(I), I.stream));
+      const double zero[qdwf::kMaxRhs] = {};
+      tp.step({0}, zero, zero, false);
+      tp.hermop_dot(tp.colset({0}), eo);
+      tp.axpy_to_x(1.0, tp.AP, 0);
+      qdwf::launch_f2d(tp.X, b, I.ls, I.geo.stride, 0, I.stream);
+    } else {
+      qdwf::launch_d2f(a, V(I, V_R), I.ls, I.geo.stride, I.stream);
+      check_launch();
+      op_hermop<float>(I, V(I, V_AP), V(I, V_R), eo);
+      qdwf::launch_f2d(V(I, V_AP), b, I.ls, I.geo.stride, 0, I.stream);
+    }
     double2* d_out = stage(I, 1, bytes);
     qdwf::launch_unpack5(b, d_out, I.geo, I.ls, eo, 0, I.stream);
     check_launch();
@@ -1756,7 +2068,7 @@ int qlat_dwf_inverter_set_gauge(qlat_dwf_inverter* inv, const double* gauge, int
 }
 
 int qlat_dwf_fermion_force(qlat_dwf_inverter* inv, const double* phi, double stop_rsd, double* force, double* grad,
-                           double* action, long* iter)
+                           double* action, double* action_lo, long* iter)
 {
   return guarded([&] {
     check_inv(inv);
@@ -1770,7 +2082,7 @@ int qlat_dwf_fermion_force(qlat_dwf_inverter* inv, const double* phi, double sto
     double2* d_f = (double2*)dev_alloc(gbytes);
     double2* d_g = grad ? (double2*)dev_alloc(gbytes) : nullptr;
     try {
-      *iter = fermion_force_dev(I, d_phi, stop_rsd, d_f, d_g, action);
+      *iter = fermion_force_dev(I, d_phi, stop_rsd, d_f, d_g, action, action_lo);
       QDWF_HIP(hipMemcpyAsync(force, d_f, gbytes, hipMemcpyDeviceToHost, I.stream));
       if (grad) QDWF_HIP(hipMemcpyAsync(grad, d_g, gbytes, hipMemcpyDeviceToHost, I.stream));
       sync(I);
@@ -1786,7 +2098,7 @@ int qlat_dwf_fermion_force(qlat_dwf_inverter* inv, const double* phi, double sto
 }
 
 int qlat_dwf_fermion_force_device(qlat_dwf_inverter* inv, const double* phi, double stop_rsd, double* force,
-                                  double* grad, double* action, long* iter)
+                                  double* grad, double* action, double* action_lo, long* iter)
 {
   return guarded([&] {
     check_inv(inv);
@@ -1794,7 +2106,7 @@ int qlat_dwf_fermion_force_device(qlat_dwf_inverter* inv, const double* phi, dou
     Inv& I = *inv;
     reset_stats(I);
     const auto t0 = std::chrono::steady_clock::now();
-    *iter = fermion_force_dev(I, (const double2*)phi, stop_rsd, (double2*)force, (double2*)grad, action);
+    *iter = fermion_force_dev(I, (const double2*)phi, stop_rsd, (double2*)force, (double2*)grad, action, action_lo);
     sync(I);
     I.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   });
@@ -1805,11 +2117,31 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
   return guarded([&] {
     check_inv(inv);
     QDWF_REQUIRE(avg_ms && iters > 0, "invalid argument");
-    QDWF_REQUIRE(precision == 64 || precision == 32, "precision must be 64 or 32");
-    QDWF_REQUIRE(which >= 0 && which <= 4, "which must be 0..4");
+    QDWF_REQUIRE(which >= 0 && which <= 7, "which must be 0..7");
+    QDWF_REQUIRE(which >= 5 ? (precision == 32 || precision == 16) : (precision == 64 || precision == 32),
+                 "precision must be 64 or 32 (which 0..4), 32 or 16 (which 5..7)");
     Inv& I = *inv;
+    std::unique_ptr<ThreePhase> tp;
+    std::vector<int> cols;
+    if (which >= 5) {
+      const int tile = std::min(I.rhs_tile, qdwf::kMaxRhs);
+      tp.reset(new ThreePhase(I, tile, precision == 16));
+      for (int c = 0; c < tile; ++c) cols.push_back(c);
+      // finite, non-trivial input values
+      const size_t lb = (size_t)tp->fh * tp->slot_bytes() * tile;
+      for (void* f : {tp->P, tp->T, tp->Y, tp->B}) QDWF_HIP(hipMemsetAsync(f, 0x3f, lb, I.stream));
+      QDWF_HIP(hipMemsetAsync(tp->R, 0x3f, field32_bytes(I) * tile, I.stream));
+      QDWF_HIP(hipMemsetAsync(tp->X, 0, field32_bytes(I) * tile, I.stream));
+    }
+    const std::vector<double> al(qdwf::kMaxRhs, 0.0), be(qdwf::kMaxRhs, 0.5);
     auto run = [&]() {
-      if (precision == 64) {
+      if (which == 5) {
+        tp->hopm5(tp->B, tp->T, 2, false, MAT_K, 1.f, nullptr, 0.f, nullptr, tp->colset(cols));
+      } else if (which == 6) {
+        tp->step(cols, al.data(), be.data(), true);
+      } else if (which == 7) {
+        tp->hermop_dot(tp->colset(cols), 1);
+      } else if (precision == 64) {
         double2* a = W(I, W_IN_E);
         double2* b = W(I, W_OUT_E);
         if (which == 0) hop<double>(I, b, a, 1, false);
@@ -1834,8 +2166,10 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
       }
     };
     // finite, non-trivial input values
-    if (precision == 64) QDWF_HIP(hipMemsetAsync(W(I, W_IN_E), 0x3f, field64_bytes(I), I.stream));
-    else QDWF_HIP(hipMemsetAsync(V(I, V_R), 0x3f, field32_bytes(I), I.stream));
+    if (which < 5) {
+      if (precision == 64) QDWF_HIP(hipMemsetAsync(W(I, W_IN_E), 0x3f, field64_bytes(I), I.stream));
+      else QDWF_HIP(hipMemsetAsync(V(I, V_R), 0x3f, field32_bytes(I), I.stream));
+    }
     run();
     hipEvent_t e0, e1;
     QDWF_HIP(hipEventCreate(&e0));

@@ -4,6 +4,10 @@
 // device half-sites; hb = h / 64, lane = h % 64):
 //   half spinor field, FP64: double2  f[((hb*Ls + s)*12 + k)*64 + lane]   k = spin*3+colour
 //   half spinor field, FP32: float4   f[((hb*Ls + s)*6 + kp)*64 + lane]   complex 2kp, 2kp+1
+//   half spinor field, FP16 with a per-5D-site scale: 16-byte units
+//                            u[(hb*Ls + s)*208 + j*64 + lane]  j < 3: 8 halves = complex 4j..4j+3
+//                            scale: float at unit (hb*Ls + s)*208 + 192, element lane
+//     value = half * scale, scale = max |re|,|im| over the site's 24 reals
 //   links, FP64 / FP32:      double2 / float2
 //                            g[((p*nhb + hb)*72 + d*9 + e)*64 + lane]
 //     p = output parity slot (0 even, 1 odd), d = 0..3 forward U_mu(x),
@@ -17,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace qdwf {
 
@@ -208,6 +213,7 @@ struct SpinorIO;
 template <>
 struct SpinorIO<double> {
   typedef double2 slot_t;
+  typedef double real;
   static constexpr int kSlots = 12;
   __device__ __forceinline__ static void load(const double2* __restrict__ f, int ls, int s, int64_t h,
                                               cplx<double> v[12])
@@ -238,6 +244,7 @@ struct SpinorIO<double> {
 template <>
 struct SpinorIO<float> {
   typedef float4 slot_t;
+  typedef float real;
   static constexpr int kSlots = 6;
   __device__ __forceinline__ static void load(const float4* __restrict__ f, int ls, int s, int64_t h,
                                               cplx<float> v[12])
@@ -267,6 +274,117 @@ struct SpinorIO<float> {
       store_nt(&p[k * 64], make_float4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im));
   }
 };
+
+// FP16 storage with a per-5D-site FP32 scale (arithmetic in FP32).  Stores
+// round to nearest, and store a site whose largest component is below
+// 2^-100 as zero (the solvers keep their Krylov vectors normalised, so such
+// sites are far below the storage precision); store_q also returns the
+// values as stored, so that a
+// kernel's other uses of them (dot products, s-matrix inputs) see exactly
+// what the next kernel will read.
+struct Half16 {};
+
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned pack_h2(float a, float b)
+{
+  h16x2 v;
+  v.x = (_Float16)a;
+  v.y = (_Float16)b;
+  return __builtin_bit_cast(unsigned, v);
+}
+__device__ __forceinline__ cplx<float> unpack_h2(unsigned w, float sc)
+{
+  const h16x2 v = __builtin_bit_cast(h16x2, w);
+  return cmk<float>((float)v.x * sc, (float)v.y * sc);
+}
+
+template <>
+struct SpinorIO<Half16> {
+  typedef uint4 slot_t;
+  typedef float real;
+  static constexpr int kUnits = 208;  // 16-byte units per (64-site block, s)
+  __device__ __forceinline__ static int64_t base(int ls, int s, int64_t h)
+  {
+    return ((h >> 6) * ls + s) * kUnits;
+  }
+  __device__ __forceinline__ static void load(const uint4* __restrict__ f, int ls, int s, int64_t h,
+                                              cplx<float> v[12])
+  {
+    const uint4* p = f + base(ls, s, h);
+    const int lane = (int)(h & 63);
+    const float sc = ((const float*)(p + 192))[lane];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const uint4 q = p[j * 64 + lane];
+      v[4 * j + 0] = unpack_h2(q.x, sc);
+      v[4 * j + 1] = unpack_h2(q.y, sc);
+      v[4 * j + 2] = unpack_h2(q.z, sc);
+      v[4 * j + 3] = unpack_h2(q.w, sc);
+    }
+  }
+  template <bool NT>
+  __device__ __forceinline__ static void put(uint4* __restrict__ f, int ls, int s, int64_t h, cplx<float> v[12])
+  {
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
+    // sites below 2^-100 are stored as zero: 1/m must stay finite
+    if (m < 0x1p-100f) m = 0.f;
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+    uint4* p = f + base(ls, s, h);
+    const int lane = (int)(h & 63);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      uint4 q;
+      q.x = pack_h2(v[4 * j + 0].re * inv, v[4 * j + 0].im * inv);
+      q.y = pack_h2(v[4 * j + 1].re * inv, v[4 * j + 1].im * inv);
+      q.z = pack_h2(v[4 * j + 2].re * inv, v[4 * j + 2].im * inv);
+      q.w = pack_h2(v[4 * j + 3].re * inv, v[4 * j + 3].im * inv);
+      if (NT) {
+        const u32x4 t = {q.x, q.y, q.z, q.w};
+        __builtin_nontemporal_store(t, (u32x4*)&p[j * 64 + lane]);
+      } else {
+        p[j * 64 + lane] = q;
+      }
+      v[4 * j + 0] = unpack_h2(q.x, m);
+      v[4 * j + 1] = unpack_h2(q.y, m);
+      v[4 * j + 2] = unpack_h2(q.z, m);
+      v[4 * j + 3] = unpack_h2(q.w, m);
+    }
+    ((float*)(p + 192))[lane] = m;
+  }
+  __device__ __forceinline__ static void store_q(uint4* __restrict__ f, int ls, int s, int64_t h, cplx<float> v[12])
+  {
+    put<false>(f, ls, s, h, v);
+  }
+  __device__ __forceinline__ static void store(uint4* __restrict__ f, int ls, int s, int64_t h,
+                                               const cplx<float> v0[12])
+  {
+    cplx<float> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = v0[k];
+    put<false>(f, ls, s, h, v);
+  }
+  __device__ __forceinline__ static void store_stream(uint4* __restrict__ f, int ls, int s, int64_t h,
+                                                      const cplx<float> v0[12])
+  {
+    cplx<float> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = v0[k];
+    put<true>(f, ls, s, h, v);
+  }
+};
+
+// store_q for FP32/FP64 storage: values are stored exactly
+template <class ST>
+__device__ __forceinline__ void store_q(typename SpinorIO<ST>::slot_t* __restrict__ f, int ls, int s, int64_t h,
+                                        cplx<typename SpinorIO<ST>::real> v[12])
+{
+  if constexpr (std::is_same<ST, Half16>::value) SpinorIO<Half16>::store_q(f, ls, s, h, v);
+  else SpinorIO<ST>::store(f, ls, s, h, v);
+}
 
 template <class T>
 struct LinkIO;

@@ -64,6 +64,7 @@ struct CgStepArgs {
   void* p;
   const void* r;
   void* t;
+  int64_t xr_fstride;  // column spacing of x and r (three-phase variant: FP32 stacks)
 };
 
 // FP32 fused hop + s-matrix: out = alpha * Mat (H in) + beta * x
@@ -95,6 +96,19 @@ struct AxpyNormArgs {
   double* part;
 };
 
+// r_i -= alpha_i ap_i over 5D sites: ap in the solver's storage (cols.fstride
+// apart), r FP32 (r_fstride float4 apart); partial |r_i|^2 per block
+struct AxpyNormSiteArgs {
+  const void* ap;
+  void* r;
+  int64_t n5;  // 5D sites per field including pads: stride * ls
+  int ls;
+  double alpha[kMaxRhs];
+  RhsSet cols;
+  int64_t r_fstride;
+  double* part;
+};
+
 // Fermion force: A[p], B[p] are FP64 half fields by parity slot (0 even,
 // 1 odd); links is the FP64 8-link layout; force/grad are [V][4][3][3]
 // complex128 (grad optional).
@@ -114,6 +128,14 @@ void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st);
 // returns false (nothing launched) when ls is too large for the fused kernel
 bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st);
 int64_t hop_m5_blocks(const LatGeo& g);
+// Three-phase kernels for FP32 or FP16 (per-site scale) spinor storage
+// (half = true); false when ls does not fit them (6 <= ls <= 16, ls even).
+bool three_phase_ok(int ls);
+bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st);
+bool launch_cg_step_s(const CgStepArgs<float>& a, bool half, hipStream_t st);
+int launch_axpy_norm_site(const AxpyNormSiteArgs& a, bool half, hipStream_t st);
+void launch_axpy_site(double alpha, const void* p, float4* x, int ls, int64_t n5, bool half, hipStream_t st);
+void launch_hop_h16(const HopArgs<float>& a, bool dag, hipStream_t st);
 template <class T>
 void launch_m5(const M5Args<T>& a, hipStream_t st);
 template <class T>
@@ -130,7 +152,8 @@ template <class S>
 void launch_norm2(const S* a, int64_t n, double* part, double* out, hipStream_t st);
 template <class S>
 void launch_dot(const S* a, const S* b, int64_t n, double* part, double* out, hipStream_t st);
-// out[0] = sum 2 Re conj(phi) x - |y|^2 with a compensated (twice-FP64) sum
+// out[0] + out[1] = sum 2 Re conj(phi) x - |y|^2 with a compensated (twice-FP64)
+// sum; out[0] is it rounded to FP64
 void launch_action_dd(const double2* phi, const double2* x, const double2* y, int64_t n, double* part, double* out,
                       hipStream_t st);
 template <class S>
@@ -142,8 +165,9 @@ void launch_cg_update(double a, const S* p, const S* ap, S* x, S* r, int64_t n, 
                       hipStream_t st);
 
 void launch_copy(const float4* x, float4* y, int64_t n, hipStream_t st);
-void launch_d2f(const double2* in, float4* out, int ls, int64_t st, hipStream_t s);
-void launch_f2d(const float4* in, double2* out, int ls, int64_t st, int acc, hipStream_t s);
+// FP64 <-> FP32 blocked fields: out = scale * in (d2f); out (+)= scale * in (f2d, acc = 1 adds)
+void launch_d2f(const double2* in, float4* out, int ls, int64_t st, hipStream_t s, double scale = 1.0);
+void launch_f2d(const float4* in, double2* out, int ls, int64_t st, int acc, hipStream_t s, double scale = 1.0);
 
 void launch_pack5(const double2* aos, double2* soa, const LatGeo& g, int ls, int eo, int full, hipStream_t st);
 void launch_unpack5(const double2* soa, double2* aos, const LatGeo& g, int ls, int eo, int full, hipStream_t st);

@@ -90,10 +90,11 @@ __global__ void __launch_bounds__(kHopBlock) k_hop(HopArgs<T> a)
 // consecutive output half-sites; its NW waves load the 64 x 8 links once,
 // then wave w computes s = w, w+NW, ... for all 64 sites, reading the links
 // from LDS.  One link load from HBM per site per launch, independent of Ls.
-template <class T, int NW, bool DAG, bool XPAY, bool NT>
-__global__ void __launch_bounds__(64 * NW) k_hop_lds(HopArgs<T> a)
+template <class ST, int NW, bool DAG, bool XPAY, bool NT>
+__global__ void __launch_bounds__(64 * NW) k_hop_lds(HopArgs<typename SpinorIO<ST>::real> a)
 {
-  typedef typename SpinorIO<T>::slot_t slot_t;
+  typedef typename SpinorIO<ST>::real T;
+  typedef typename SpinorIO<ST>::slot_t slot_t;
   typedef typename LinkIO<T>::elem_t link_t;
   __shared__ link_t sl[72][64];
   const int64_t lb = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
@@ -131,7 +132,7 @@ __global__ void __launch_bounds__(64 * NW) k_hop_lds(HopArgs<T> a)
       W[e] = cmk<T>(q.x, q.y);                                     \
     }                                                              \
     cplx<T> psi[12];                                               \
-    SpinorIO<T>::load(in, a.ls, s, nb[D], psi);                    \
+    SpinorIO<ST>::load(in, a.ls, s, nb[D], psi);                   \
     hop_term<MU, SG>(W, psi, acc);                                 \
   }
     QDWF_HOP_LDS_DIR(0, 0, SF)
@@ -148,26 +149,26 @@ __global__ void __launch_bounds__(64 * NW) k_hop_lds(HopArgs<T> a)
     for (int k = 0; k < 12; ++k) v[k] = cmk<T>(scale * acc[k].re, scale * acc[k].im);
     if (XPAY) {
       cplx<T> xv[12];
-      SpinorIO<T>::load((const slot_t*)a.x + coff, a.ls, s, h, xv);
+      SpinorIO<ST>::load((const slot_t*)a.x + coff, a.ls, s, h, xv);
       const cplx<T> b = a.beta[s];
 #pragma unroll
       for (int k = 0; k < 12; ++k) v[k] = cfma(v[k], b, xv[k]);
     }
-    if (NT) SpinorIO<T>::store_stream(out, a.ls, s, h, v);
-    else SpinorIO<T>::store(out, a.ls, s, h, v);
+    if (NT) SpinorIO<ST>::store_stream(out, a.ls, s, h, v);
+    else SpinorIO<ST>::store(out, a.ls, s, h, v);
   }
 }
 
-template <class T, int NW, bool NT>
-static void launch_hop_lds(const HopArgs<T>& a, bool dag, hipStream_t st)
+template <class ST, int NW, bool NT>
+static void launch_hop_lds(const HopArgs<typename SpinorIO<ST>::real>& a, bool dag, hipStream_t st)
 {
   const dim3 grid((unsigned)((a.geo.vh + 63) / 64));
   const dim3 blk(64 * NW);
   const bool xpay = a.x != nullptr;
-  if (!dag && !xpay) hipLaunchKernelGGL((k_hop_lds<T, NW, false, false, NT>), grid, blk, 0, st, a);
-  if (!dag && xpay) hipLaunchKernelGGL((k_hop_lds<T, NW, false, true, NT>), grid, blk, 0, st, a);
-  if (dag && !xpay) hipLaunchKernelGGL((k_hop_lds<T, NW, true, false, NT>), grid, blk, 0, st, a);
-  if (dag && xpay) hipLaunchKernelGGL((k_hop_lds<T, NW, true, true, NT>), grid, blk, 0, st, a);
+  if (!dag && !xpay) hipLaunchKernelGGL((k_hop_lds<ST, NW, false, false, NT>), grid, blk, 0, st, a);
+  if (!dag && xpay) hipLaunchKernelGGL((k_hop_lds<ST, NW, false, true, NT>), grid, blk, 0, st, a);
+  if (dag && !xpay) hipLaunchKernelGGL((k_hop_lds<ST, NW, true, false, NT>), grid, blk, 0, st, a);
+  if (dag && xpay) hipLaunchKernelGGL((k_hop_lds<ST, NW, true, true, NT>), grid, blk, 0, st, a);
 }
 
 template <class T, int SPT>
@@ -220,6 +221,8 @@ void launch_hop(const HopArgs<T>& a, bool dag, int spt, hipStream_t st)
 
 template void launch_hop<double>(const HopArgs<double>&, bool, int, hipStream_t);
 template void launch_hop<float>(const HopArgs<float>&, bool, int, hipStream_t);
+
+void launch_hop_h16(const HopArgs<float>& a, bool dag, hipStream_t st) { launch_hop_lds<Half16, 4, false>(a, dag, st); }
 
 // ============================================================ hop + M5
 // Fused out = alpha * Mat (H in) + beta * x for FP32 fields, where Mat is a
@@ -406,6 +409,369 @@ bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st)
   else if (a.ls <= 12) launch_hop_m5_ls<kHopM5Sites, 12>(a, dag, st);
   else launch_hop_m5_ls<kHopM5Sites, 16>(a, dag, st);
   return true;
+}
+
+// ============================================== three-phase hop + M5
+// Variant of k_hop_m5 for any spinor storage ST (FP32, or FP16 with a
+// per-site scale): phase 1 as in k_hop_m5; phase 2 (smat_lds) applies Mat
+// along s in LDS; phase 3, thread (site, s) again, adds beta * x, stores the
+// whole 5D site (so a per-site scale sees all 24 reals) and accumulates the
+// CG dot product from the stored values.
+
+// Thread (site, kp < 6) replaces its slot column hr[.][kp][site] by Mat
+// (per chirality) applied along s.  Every thread of the block calls it; the
+// caller has synchronised after writing hr.
+template <int SB, int LSMAX, bool REAL>
+__device__ __forceinline__ void smat_lds(float4 (*hr)[6][SB], const cplx<float>* __restrict__ mat, int ls,
+                                         int site, int row, bool active)
+{
+  const bool mine = active && row < 6;
+  float4 v[LSMAX];
+  if (mine) {
+#pragma unroll
+    for (int t = 0; t < LSMAX; ++t)
+      if (t < ls) v[t] = hr[t][row][site];
+  }
+  __syncthreads();
+  if (mine) {
+    const cplx<float>* __restrict__ M = mat + (row < 3 ? 0 : ls * ls);
+#pragma unroll
+    for (int s = 0; s < LSMAX; ++s) {
+      if (s < ls) {
+        float r0 = 0.f, i0 = 0.f, r1 = 0.f, i1 = 0.f;
+#pragma unroll
+        for (int t = 0; t < LSMAX; ++t) {
+          if (t < ls) {
+            const cplx<float> m = M[s * ls + t];
+            r0 += m.re * v[t].x;
+            i0 += m.re * v[t].y;
+            r1 += m.re * v[t].z;
+            i1 += m.re * v[t].w;
+            if (!REAL) {
+              r0 -= m.im * v[t].y;
+              i0 += m.im * v[t].x;
+              r1 -= m.im * v[t].w;
+              i1 += m.im * v[t].z;
+            }
+          }
+        }
+        hr[s][row][site] = make_float4(r0, i0, r1, i1);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// streaming store that also returns the stored values
+template <class ST>
+__device__ __forceinline__ void store_qs(typename SpinorIO<ST>::slot_t* __restrict__ f, int ls, int s, int64_t h,
+                                         cplx<float> v[12])
+{
+  if constexpr (std::is_same<ST, Half16>::value) SpinorIO<Half16>::put<true>(f, ls, s, h, v);
+  else SpinorIO<ST>::store_stream(f, ls, s, h, v);
+}
+
+template <class ST, int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT>
+__global__ void __launch_bounds__(SB * LSMAX) __attribute__((amdgpu_waves_per_eu(3)))
+k_hop_m5s(HopM5Args a)
+{
+  typedef typename SpinorIO<ST>::slot_t slot_t;
+  __shared__ float2 sl[72][SB];
+  __shared__ float4 hr[LSMAX][6][SB];
+  const int64_t lbc = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
+  const int ncol = a.cols.n;
+  const int64_t lb = lbc / ncol;
+  const int ci = (int)(lbc - lb * ncol);
+  const int64_t nblk = (int64_t)gridDim.x / ncol;
+  const int64_t h0 = lb * SB;
+  const int pslot = (a.eo_out == 2) ? 0 : 1;
+  const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
+  for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) sl[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
+  __syncthreads();
+  const int ls = a.ls;
+  const int site = threadIdx.x % SB, row = threadIdx.x / SB;
+  const int64_t h = h0 + site;
+  const bool active = h < a.geo.vh;
+  const int64_t coff = (int64_t)pick(a.cols.rhs, ci) * a.cols.fstride;
+  if (active && row < ls) {
+    int x[4];
+    half_coord(a.geo, h, a.eo_out, x);
+    int64_t nb[8];
+    neighbours(a.geo, x, nb);
+    const slot_t* __restrict__ in = (const slot_t*)a.in + coff;
+    constexpr int SF = DAG ? 1 : -1;
+    constexpr int SB_ = -SF;
+    cplx<float> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = cmk<float>(0, 0);
+#define QDWF_HOPM5S_DIR(D, MU, SG)                                  \
+  {                                                                 \
+    cplx<float> W[9];                                               \
+    _Pragma("unroll") for (int e = 0; e < 9; ++e)                   \
+    {                                                               \
+      const float2 q = sl[(D) * 9 + e][site];                       \
+      W[e] = cmk<float>(q.x, q.y);                                  \
+    }                                                               \
+    cplx<float> psi[12];                                            \
+    SpinorIO<ST>::load(in, ls, row, nb[D], psi);                    \
+    hop_term<MU, SG>(W, psi, acc);                                  \
+  }
+    QDWF_HOPM5S_DIR(0, 0, SF)
+    QDWF_HOPM5S_DIR(1, 1, SF)
+    QDWF_HOPM5S_DIR(2, 2, SF)
+    QDWF_HOPM5S_DIR(3, 3, SF)
+    QDWF_HOPM5S_DIR(4, 0, SB_)
+    QDWF_HOPM5S_DIR(5, 1, SB_)
+    QDWF_HOPM5S_DIR(6, 2, SB_)
+    QDWF_HOPM5S_DIR(7, 3, SB_)
+#undef QDWF_HOPM5S_DIR
+    const float sc = -0.5f * a.alpha;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      hr[row][k][site] = make_float4(sc * acc[2 * k].re, sc * acc[2 * k].im, sc * acc[2 * k + 1].re,
+                                     sc * acc[2 * k + 1].im);
+  }
+  __syncthreads();
+  smat_lds<SB, LSMAX, REAL>(hr, a.mat, ls, site, row, active);
+  double dot = 0.0;
+  if (active && row < ls) {
+    cplx<float> v[12];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float4 q = hr[row][k][site];
+      v[2 * k] = cmk<float>(q.x, q.y);
+      v[2 * k + 1] = cmk<float>(q.z, q.w);
+    }
+    if (XPAY) {
+      cplx<float> xv[12];
+      SpinorIO<ST>::load((const slot_t*)a.x + coff, ls, row, h, xv);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v[k] = cmk<float>(v[k].re + a.beta * xv[k].re, v[k].im + a.beta * xv[k].im);
+    }
+    store_qs<ST>((slot_t*)a.out + coff, ls, row, h, v);
+    if (DOT) {
+      cplx<float> pv[12];
+      SpinorIO<ST>::load((const slot_t*)a.dotp + coff, ls, row, h, pv);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) dot += (double)pv[k].re * v[k].re + (double)pv[k].im * v[k].im;
+    }
+  }
+  if (DOT) {
+    __shared__ double red[(SB * LSMAX + 63) / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
+    if (lane == 0) red[wv] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) t += red[i];
+      a.part[ci * nblk + lb] = t;
+    }
+  }
+}
+
+template <class ST, int SB, int LSMAX>
+static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
+{
+  const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB * a.cols.n));
+  const dim3 blk(SB * a.ls);
+  const bool xpay = a.x != nullptr;
+#define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, D, X, R, DT>), grid, blk, 0, st, a)
+  if (a.dotp) {
+    if (dag && xpay && a.real_mat) QDWF_L(true, true, true, true);
+    else if (dag && xpay) QDWF_L(true, true, false, true);
+    return;
+  }
+  if (a.real_mat) {
+    if (!dag && !xpay) QDWF_L(false, false, true, false);
+    if (!dag && xpay) QDWF_L(false, true, true, false);
+    if (dag && !xpay) QDWF_L(true, false, true, false);
+    if (dag && xpay) QDWF_L(true, true, true, false);
+  } else {
+    if (!dag && !xpay) QDWF_L(false, false, false, false);
+    if (!dag && xpay) QDWF_L(false, true, false, false);
+    if (dag && !xpay) QDWF_L(true, false, false, false);
+    if (dag && xpay) QDWF_L(true, true, false, false);
+  }
+#undef QDWF_L
+}
+
+bool three_phase_ok(int ls) { return ls >= 6 && ls <= 16 && (kHopM5Sites * ls) % 64 == 0; }
+
+bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
+{
+  if (!three_phase_ok(a.ls)) return false;
+  if (half) {
+    if (a.ls <= 8) launch_hop_m5s_ls<Half16, kHopM5Sites, 8>(a, dag, st);
+    else if (a.ls <= 12) launch_hop_m5s_ls<Half16, kHopM5Sites, 12>(a, dag, st);
+    else launch_hop_m5s_ls<Half16, kHopM5Sites, 16>(a, dag, st);
+  } else {
+    if (a.ls <= 8) launch_hop_m5s_ls<float, kHopM5Sites, 8>(a, dag, st);
+    else if (a.ls <= 12) launch_hop_m5s_ls<float, kHopM5Sites, 12>(a, dag, st);
+    else launch_hop_m5s_ls<float, kHopM5Sites, 16>(a, dag, st);
+  }
+  return true;
+}
+
+// CG direction update for p, t in storage ST and x, r in FP32:
+// x += alpha p (UPDX); p = r + beta p; t = Mat p, in three phases like
+// k_hop_m5s (block = SB sites x Ls, blockIdx.y = column).  p and t are
+// computed from the stored (rounded) p.
+template <class ST, int SB, int LSMAX, bool UPDX, bool REAL>
+__global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
+{
+  typedef typename SpinorIO<ST>::slot_t slot_t;
+  __shared__ float4 hr[LSMAX][6][SB];
+  const int ci = blockIdx.y;
+  const int ls = a.ls;
+  const int site = threadIdx.x % SB, row = threadIdx.x / SB;
+  const int64_t h = (int64_t)blockIdx.x * SB + site;
+  const bool active = h < a.geo.vh;
+  const int col = pick(a.cols.rhs, ci);
+  const int64_t coff = (int64_t)col * a.cols.fstride;
+  const int64_t xoff = (int64_t)col * a.xr_fstride;
+  if (active && row < ls) {
+    const float al = (float)pick(a.alpha, ci), be = (float)pick(a.beta, ci);
+    cplx<float> pv[12], rv[12];
+    SpinorIO<ST>::load((const slot_t*)a.p + coff, ls, row, h, pv);
+    SpinorIO<float>::load((const float4*)a.r + xoff, ls, row, h, rv);
+    if (UPDX) {
+      cplx<float> xv[12];
+      float4* x = (float4*)a.x + xoff;
+      SpinorIO<float>::load(x, ls, row, h, xv);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + al * pv[k].re, xv[k].im + al * pv[k].im);
+      SpinorIO<float>::store(x, ls, row, h, xv);
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) pv[k] = cmk<float>(be * pv[k].re + rv[k].re, be * pv[k].im + rv[k].im);
+    store_q<ST>((slot_t*)a.p + coff, ls, row, h, pv);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      hr[row][k][site] = make_float4(pv[2 * k].re, pv[2 * k].im, pv[2 * k + 1].re, pv[2 * k + 1].im);
+  }
+  __syncthreads();
+  smat_lds<SB, LSMAX, REAL>(hr, a.mat, ls, site, row, active);
+  if (active && row < ls) {
+    cplx<float> v[12];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float4 q = hr[row][k][site];
+      v[2 * k] = cmk<float>(q.x, q.y);
+      v[2 * k + 1] = cmk<float>(q.z, q.w);
+    }
+    store_q<ST>((slot_t*)a.t + coff, ls, row, h, v);
+  }
+}
+
+template <class ST, int LSMAX>
+static void launch_cg_step_s_ls(const CgStepArgs<float>& a, hipStream_t st)
+{
+  constexpr int SB = kHopM5Sites;
+  const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB), a.cols.n);
+  const dim3 blk(SB * a.ls);
+  const bool updx = a.x != nullptr;
+  if (a.real_mat) {
+    if (updx) hipLaunchKernelGGL((k_cg_step_s<ST, SB, LSMAX, true, true>), grid, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_cg_step_s<ST, SB, LSMAX, false, true>), grid, blk, 0, st, a);
+  } else {
+    if (updx) hipLaunchKernelGGL((k_cg_step_s<ST, SB, LSMAX, true, false>), grid, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_cg_step_s<ST, SB, LSMAX, false, false>), grid, blk, 0, st, a);
+  }
+}
+
+bool launch_cg_step_s(const CgStepArgs<float>& a, bool half, hipStream_t st)
+{
+  if (!three_phase_ok(a.ls)) return false;
+  if (half) {
+    if (a.ls <= 8) launch_cg_step_s_ls<Half16, 8>(a, st);
+    else if (a.ls <= 12) launch_cg_step_s_ls<Half16, 12>(a, st);
+    else launch_cg_step_s_ls<Half16, 16>(a, st);
+  } else {
+    if (a.ls <= 8) launch_cg_step_s_ls<float, 8>(a, st);
+    else if (a.ls <= 12) launch_cg_step_s_ls<float, 12>(a, st);
+    else launch_cg_step_s_ls<float, 16>(a, st);
+  }
+  return true;
+}
+
+// r_i -= alpha_i ap_i over 5D sites (ap in storage ST, r in FP32) with
+// per-column partial |r_i|^2 (part[i*gridDim.x + block]).
+template <class ST>
+__global__ void __launch_bounds__(kBlasBlock) k_axpy_norm_site(AxpyNormSiteArgs a)
+{
+  typedef typename SpinorIO<ST>::slot_t slot_t;
+  const int ci = blockIdx.y;
+  const int col = pick(a.cols.rhs, ci);
+  const slot_t* __restrict__ ap = (const slot_t*)a.ap + (int64_t)col * a.cols.fstride;
+  float4* __restrict__ r = (float4*)a.r + (int64_t)col * a.r_fstride;
+  const float al = (float)pick(a.alpha, ci);
+  const int64_t per_blk = (int64_t)a.ls * 64;
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < a.n5; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t hb = i / per_blk, rem = i - hb * per_blk;
+    const int s = (int)(rem >> 6);
+    const int64_t h = (hb << 6) + (rem & 63);
+    cplx<float> av[12], rv[12];
+    SpinorIO<ST>::load(ap, a.ls, s, h, av);
+    SpinorIO<float>::load(r, a.ls, s, h, rv);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      rv[k] = cmk<float>(rv[k].re - al * av[k].re, rv[k].im - al * av[k].im);
+      v += (double)rv[k].re * rv[k].re + (double)rv[k].im * rv[k].im;
+    }
+    SpinorIO<float>::store(r, a.ls, s, h, rv);
+  }
+  __shared__ double sh[kBlasBlock / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kBlasBlock / 64; ++i) t += sh[i];
+    a.part[(int64_t)ci * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+int launch_axpy_norm_site(const AxpyNormSiteArgs& a, bool half, hipStream_t st)
+{
+  int64_t g = (a.n5 + kBlasBlock - 1) / kBlasBlock;
+  if (g > kMaxBlasGrid) g = kMaxBlasGrid;
+  const dim3 grid((unsigned)g, a.cols.n);
+  if (half) hipLaunchKernelGGL(k_axpy_norm_site<Half16>, grid, dim3(kBlasBlock), 0, st, a);
+  else hipLaunchKernelGGL(k_axpy_norm_site<float>, grid, dim3(kBlasBlock), 0, st, a);
+  return (int)g;
+}
+
+// x += alpha p over the 5D sites of one field (x FP32, p in storage ST)
+template <class ST>
+__global__ void __launch_bounds__(kBlasBlock) k_axpy_site(float alpha, const void* pp, float4* __restrict__ x,
+                                                          int ls, int64_t n5)
+{
+  typedef typename SpinorIO<ST>::slot_t slot_t;
+  const slot_t* __restrict__ p = (const slot_t*)pp;
+  const int64_t per_blk = (int64_t)ls * 64;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n5; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t hb = i / per_blk, rem = i - hb * per_blk;
+    const int s = (int)(rem >> 6);
+    const int64_t h = (hb << 6) + (rem & 63);
+    cplx<float> pv[12], xv[12];
+    SpinorIO<ST>::load(p, ls, s, h, pv);
+    SpinorIO<float>::load(x, ls, s, h, xv);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + alpha * pv[k].re, xv[k].im + alpha * pv[k].im);
+    SpinorIO<float>::store(x, ls, s, h, xv);
+  }
+}
+
+void launch_axpy_site(double alpha, const void* p, float4* x, int ls, int64_t n5, bool half, hipStream_t st)
+{
+  int64_t g = (n5 + kBlasBlock - 1) / kBlasBlock;
+  if (g > kMaxBlasGrid) g = kMaxBlasGrid;
+  if (half) hipLaunchKernelGGL(k_axpy_site<Half16>, dim3((unsigned)g), dim3(kBlasBlock), 0, st, (float)alpha, p, x, ls, n5);
+  else hipLaunchKernelGGL(k_axpy_site<float>, dim3((unsigned)g), dim3(kBlasBlock), 0, st, (float)alpha, p, x, ls, n5);
 }
 
 // ============================================================ force
@@ -929,7 +1295,8 @@ __global__ void __launch_bounds__(kBlasBlock) k_action_dd(const double2* __restr
   }
 }
 
-// out[0] = compensated sum of nblk (s, c) pairs (one 256-thread block)
+// out[0] + out[1] = compensated sum of nblk (s, c) pairs, out[0] the sum
+// rounded to FP64 (one 256-thread block)
 __global__ void __launch_bounds__(256) k_reduce_dd(const double* __restrict__ part, int nblk, double* out)
 {
   double s = 0.0, c = 0.0;
@@ -951,7 +1318,12 @@ __global__ void __launch_bounds__(256) k_reduce_dd(const double* __restrict__ pa
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = sh[0] + shc[0];
+  if (threadIdx.x == 0) {
+    double hi, lo;
+    two_sum(sh[0], shc[0], hi, lo);
+    out[0] = hi;
+    out[1] = lo;
+  }
 }
 
 // y = a*x + y
@@ -1062,7 +1434,7 @@ __global__ void __launch_bounds__(256) k_reduce_final(const double* __restrict__
 // FP64 blocked field (12 double2 slots per 5D site) <-> FP32 blocked field
 // (6 float4 slots per 5D site); i enumerates FP32 slots.
 __global__ void __launch_bounds__(kBlasBlock) k_d2f(const double2* __restrict__ in, float4* __restrict__ out,
-                                                    int64_t nslot32)
+                                                    int64_t nslot32, double scale)
 {
   for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < nslot32; i += (int64_t)gridDim.x * kBlasBlock) {
     const int64_t lane = i & 63, q = i >> 6;  // q = (hb*ls + s)*6 + kp
@@ -1070,28 +1442,29 @@ __global__ void __launch_bounds__(kBlasBlock) k_d2f(const double2* __restrict__ 
     const int64_t o = ((hs * 12 + 2 * kp) << 6) + lane;
     const double2 a = in[o];
     const double2 b = in[o + 64];
-    out[i] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+    out[i] = make_float4((float)(scale * a.x), (float)(scale * a.y), (float)(scale * b.x), (float)(scale * b.y));
   }
 }
 
-// out64 (+)= in32 : acc=0 overwrite, acc=1 add
+// out64 (+)= scale * in32 : acc=0 overwrite, acc=1 add
 __global__ void __launch_bounds__(kBlasBlock) k_f2d(const float4* __restrict__ in, double2* __restrict__ out,
-                                                    int64_t nslot32, int acc)
+                                                    int64_t nslot32, int acc, double scale)
 {
   for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < nslot32; i += (int64_t)gridDim.x * kBlasBlock) {
     const int64_t lane = i & 63, q = i >> 6;
     const int64_t hs = q / 6, kp = q - hs * 6;
     const int64_t o = ((hs * 12 + 2 * kp) << 6) + lane;
-    const float4 a = in[i];
+    const float4 f = in[i];
+    const double2 a0 = make_double2(scale * f.x, scale * f.y), a1 = make_double2(scale * f.z, scale * f.w);
     double2* o0 = &out[o];
     double2* o1 = &out[o + 64];
     if (acc) {
       const double2 p0 = *o0, p1 = *o1;
-      *o0 = make_double2(p0.x + a.x, p0.y + a.y);
-      *o1 = make_double2(p1.x + a.z, p1.y + a.w);
+      *o0 = make_double2(p0.x + a0.x, p0.y + a0.y);
+      *o1 = make_double2(p1.x + a1.x, p1.y + a1.y);
     } else {
-      *o0 = make_double2(a.x, a.y);
-      *o1 = make_double2(a.z, a.w);
+      *o0 = a0;
+      *o1 = a1;
     }
   }
 }
@@ -1200,15 +1573,15 @@ template void launch_cg_update<double2>(double, const double2*, const double2*, 
 template void launch_cg_update<float4>(double, const float4*, const float4*, float4*, float4*, int64_t, double*,
                                        double*, hipStream_t);
 
-void launch_d2f(const double2* in, float4* out, int ls, int64_t st, hipStream_t s)
+void launch_d2f(const double2* in, float4* out, int ls, int64_t st, hipStream_t s, double scale)
 {
   const int64_t n = (int64_t)ls * 6 * st;
-  hipLaunchKernelGGL(k_d2f, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n);
+  hipLaunchKernelGGL(k_d2f, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n, scale);
 }
-void launch_f2d(const float4* in, double2* out, int ls, int64_t st, int acc, hipStream_t s)
+void launch_f2d(const float4* in, double2* out, int ls, int64_t st, int acc, hipStream_t s, double scale)
 {
   const int64_t n = (int64_t)ls * 6 * st;
-  hipLaunchKernelGGL(k_f2d, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n, acc);
+  hipLaunchKernelGGL(k_f2d, dim3(blas_grid(n)), dim3(kBlasBlock), 0, s, in, out, n, acc, scale);
 }
 
 // ============================================================ layouts

@@ -34,9 +34,14 @@ def set_gauge(inv, gf):
                "qlat_dwf_inverter_set_gauge")
 
 
-def fermion_force(inv, phi, stop_rsd=1e-12, with_grad=False):
+def fermion_force(inv, phi, stop_rsd=1e-12, with_grad=False, action_pair=False):
     """Returns (S, force, grad_or_None, cg_iterations) for the odd half
-    field `phi` (FermionField5d with eo=1, or an array (V/2, Ls, 12))."""
+    field `phi` (FermionField5d with eo=1, or an array (V/2, Ls, 12)).
+
+    S = 2 Re <phi, X> - |Mpc X|^2 (second order in the solver error),
+    summed with compensation; with action_pair=True, S is returned as
+    (hi, lo), hi the FP64 rounding of the sum and lo the rest, so that the
+    difference of two nearby actions can be formed to beyond FP64."""
     data = phi.data if isinstance(phi, FermionField5d) else phi
     if isinstance(phi, FermionField5d):
         assert phi.eo == 1, "the pseudofermion lives on odd sites"
@@ -46,12 +51,13 @@ def fermion_force(inv, phi, stop_rsd=1e-12, with_grad=False):
     force = np.empty((V, 4, 3, 3), dtype=np.complex128)
     grad = np.empty_like(force) if with_grad else None
     S = ctypes.c_double()
+    S_lo = ctypes.c_double()
     it = ctypes.c_long()
     _lib.check(_lib.load().qlat_dwf_fermion_force(
         inv.cdata, _lib.ptr(p), float(stop_rsd), _lib.ptr(force),
-        _lib.ptr(grad) if with_grad else None, ctypes.byref(S), ctypes.byref(it)),
+        _lib.ptr(grad) if with_grad else None, ctypes.byref(S), ctypes.byref(S_lo), ctypes.byref(it)),
         "qlat_dwf_fermion_force")
-    return S.value, force, grad, it.value
+    return (S.value, S_lo.value) if action_pair else S.value, force, grad, it.value
 
 
 def _torch_function():

@@ -20,7 +20,8 @@ from . import _lib
 from .fermion_action import FermionAction
 from .fields import FermionField4d, FermionField5d, GaugeField, Prop
 
-SOLVERS = {"fp64": _lib.QLAT_DWF_SOLVER_FP64, "mixed": _lib.QLAT_DWF_SOLVER_MIXED_FP32}
+SOLVERS = {"fp64": _lib.QLAT_DWF_SOLVER_FP64, "mixed": _lib.QLAT_DWF_SOLVER_MIXED_FP32,
+           "mixed16": _lib.QLAT_DWF_SOLVER_MIXED_FP16}
 
 
 class Inverter:
@@ -33,7 +34,9 @@ class InverterDomainWall(Inverter):
     solver: "fp64" runs the reference algorithm (invert_with_cg with the
     sym2 odd preconditioning and the restart cycle loop, dslash.h:440-530)
     in FP64; "mixed" runs the inner CG in FP32 with FP64 reliable updates
-    (the role of a back-end's cg_with_herm_sym_2 override, grid.h:460-484).
+    (the role of a back-end's cg_with_herm_sym_2 override, grid.h:460-484);
+    "mixed16" is "mixed" with the Krylov vectors stored in FP16 with a
+    per-5D-site scale.
     """
 
     def __init__(self, *, gf, fa, qtimer=None, solver="fp64", reliable_delta=0.1):

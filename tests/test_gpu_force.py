@@ -44,9 +44,19 @@ def make_case(site, ls, mass, sigma, seed, mobius_scale=2.0):
 
 
 def action(inv, U, phi, stop):
+    """S(U) as an (hi, lo) pair of the compensated device sum."""
     inv.set_gauge(U)
-    S, _, _, _ = q.fermion_force(inv, phi, stop)
+    S, _, _, _ = q.fermion_force(inv, phi, stop, action_pair=True)
     return S
+
+
+def central_difference(inv, Uf, T, phi, eps, stop=1e-14):
+    """(S(exp(eps T) U) - S(exp(-eps T) U)) / (2 eps), with the two actions
+    subtracted as (hi, lo) pairs: hi - hi is exact for nearby values."""
+    shape = (Uf.shape[0] // 4, 4, 3, 3)
+    sp = action(inv, (expm_antiherm(eps * T) @ Uf).reshape(shape), phi, stop)
+    sm = action(inv, (expm_antiherm(-eps * T) @ Uf).reshape(shape), phi, stop)
+    return ((sp[0] - sm[0]) + (sp[1] - sm[1])) / (2 * eps)
 
 
 @pytest.fixture(scope="module")
@@ -87,13 +97,10 @@ def test_force_matches_central_finite_differences(small):
     geo, fa, inv, U, phi = small
     inv.set_gauge(U)
     _, F, _, _ = q.fermion_force(inv, phi, 1e-14)
-    eps = 1e-5
     Uf = U.reshape(-1, 3, 3)
     for k in range(8):
         T = su3_direction(Uf.shape[0], 100 + k)
-        Up = (expm_antiherm(eps * T) @ Uf).reshape(U.shape)
-        Um = (expm_antiherm(-eps * T) @ Uf).reshape(U.shape)
-        fd = (action(inv, Up, phi, 1e-14) - action(inv, Um, phi, 1e-14)) / (2 * eps)
+        fd = central_difference(inv, Uf, T, phi, 1e-5)
         an = np.einsum("nij,nji->", F.reshape(-1, 3, 3), T).real
         assert abs(fd - an) < 1e-6 * abs(an), (k, fd, an)
     inv.set_gauge(U)
@@ -126,10 +133,7 @@ def test_torch_autograd_function(small):
     S.backward()
     Uf = U.reshape(-1, 3, 3)
     T = su3_direction(Uf.shape[0], 7)
-    eps = 1e-5
-    Up = (expm_antiherm(eps * T) @ Uf).reshape(U.shape)
-    Um = (expm_antiherm(-eps * T) @ Uf).reshape(U.shape)
-    fd = (action(inv, Up, phi, 1e-14) - action(inv, Um, phi, 1e-14)) / (2 * eps)
+    fd = central_difference(inv, Uf, T, phi, 1e-5)
     dU = (T @ Uf).reshape(U.shape)
     an = np.real(np.vdot(links.grad.numpy(), dU))
     assert abs(fd - an) < 1e-6 * abs(an)
@@ -138,19 +142,20 @@ def test_torch_autograd_function(small):
 
 def test_config_d_force_finite_differences():
     """16^3x32, Ls=8, mass=0.05 (sigma 0.3): force vs central finite
-    differences along 32 seeded su(3) directions, 1e-6 relative."""
+    differences (eps = 1e-5, FP64 solves to 1e-14) along 32 seeded su(3)
+    directions, 1e-6 relative per direction.  S ~ 1e7 here, so FP64
+    rounding of S alone (ulp ~ 2e-9) would limit a difference quotient to
+    ~1e-4 absolute; the actions are therefore subtracted as compensated
+    (hi, lo) pairs."""
     print("\n[gpu] force config D 16^3x32 Ls8, 32 FD directions", flush=True)
     geo, fa, inv, U, phi = make_case((16, 16, 16, 32), 8, 0.05, 0.3, 71)
     inv.set_max_num_iter(20000)
     S, F, _, it = q.fermion_force(inv, phi, 1e-14)
-    eps = 1e-5
     Uf = U.reshape(-1, 3, 3)
     worst = 0.0
     for k in range(32):
         T = su3_direction(Uf.shape[0], 1000 + k)
-        Up = (expm_antiherm(eps * T) @ Uf).reshape(U.shape)
-        Um = (expm_antiherm(-eps * T) @ Uf).reshape(U.shape)
-        fd = (action(inv, Up, phi, 1e-14) - action(inv, Um, phi, 1e-14)) / (2 * eps)
+        fd = central_difference(inv, Uf, T, phi, 1e-5)
         an = np.einsum("nij,nji->", F.reshape(-1, 3, 3), T).real
         worst = max(worst, abs(fd - an) / abs(an))
         if k % 8 == 7:

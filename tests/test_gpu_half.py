@@ -1,0 +1,156 @@
+"""GPU parity of the three-phase kernels and of FP16 spinor storage with a
+per-5D-site scale (solver "mixed16"): the inner operator against the FP64
+oracle at the accuracy its storage allows, and solves at the stated
+FP64 true-residual tolerance against the oracle's solution.
+"""
+
+import numpy as np
+import pytest
+
+import qlattice_amd as q
+from oracle import oracle as O
+from tests.helpers import ZMOBIUS_OMEGA, rand_field, rand_gauge, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def make(site, U, *, mass, ls, mobius_scale=2.0, omega=None, solver="mixed16"):
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    fa = q.FermionAction(mass=mass, ls=ls, m5=1.8, mobius_scale=mobius_scale, omega=omega)
+    inv = q.InverterDomainWall(gf=gf, fa=fa, solver=solver)
+    orc = O.OracleInverter(site, U, mass=mass, ls=ls, m5=1.8, mobius_scale=mobius_scale, omega=omega)
+    return geo, inv, orc
+
+
+def half(geo, ls, eo, data):
+    f = q.FermionField5d(geo, ls, eo)
+    f.data = np.ascontiguousarray(data)
+    return f
+
+
+@pytest.mark.parametrize("ls,omega", [(8, None), (12, "z"), (16, None)])
+def test_three_phase_fp32_hermop_matches_oracle(monkeypatch, ls, omega):
+    """FP32 storage on the three-phase kernels: FP32 accuracy."""
+    print(f"\n[gpu] three-phase FP32 hermop Ls{ls}", flush=True)
+    monkeypatch.setenv("QLAT_DWF_THREE_PHASE", "1")
+    site = (4, 4, 4, 8)
+    U = rand_gauge(site, 0.4, 41)
+    geo, inv, orc = make(site, U, mass=0.05, ls=ls, omega=ZMOBIUS_OMEGA if omega else None, solver="mixed")
+    for eo in (1, 2):
+        x = rand_field((geo.total_volume() // 2, ls, 12), 500 + eo)
+        got = q.multiply_hermop_sym2_fp32(None, half(geo, ls, eo, x), inv)
+        assert rel(got.data, orc.apply_half("hermop_sym2", x, eo)) < 2e-6
+
+
+@pytest.mark.parametrize("ls,omega", [(8, None), (12, "z"), (16, None)])
+def test_fp16_storage_hermop_matches_oracle(ls, omega):
+    """FP16 storage with a per-site scale: the input and every intermediate
+    field are rounded to 11 significant bits relative to the site maximum,
+    so the operator agrees with FP64 to a few 1e-4 (and is visibly not
+    FP32: the error is far above FP32 rounding)."""
+    print(f"\n[gpu] FP16-storage hermop Ls{ls}", flush=True)
+    site = (4, 4, 4, 8)
+    U = rand_gauge(site, 0.4, 41)
+    geo, inv, orc = make(site, U, mass=0.05, ls=ls, omega=ZMOBIUS_OMEGA if omega else None)
+    for eo in (1, 2):
+        x = rand_field((geo.total_volume() // 2, ls, 12), 600 + eo)
+        got = q.multiply_hermop_sym2_fp32(None, half(geo, ls, eo, x), inv)
+        err = rel(got.data, orc.apply_half("hermop_sym2", x, eo))
+        assert 1e-5 < err < 3e-3, err
+
+
+def test_fp16_storage_zero_and_scale_invariance():
+    """A zero field stays exactly zero (scale 0), and scaling the input by a
+    power of two scales the output exactly (the per-site scale absorbs it)."""
+    site = (4, 4, 4, 8)
+    U = rand_gauge(site, 0.4, 42)
+    geo, inv, _ = make(site, U, mass=0.05, ls=8)
+    z = np.zeros((geo.total_volume() // 2, 8, 12), np.complex128)
+    assert not np.any(q.multiply_hermop_sym2_fp32(None, half(geo, 8, 1, z), inv).data)
+    x = rand_field((geo.total_volume() // 2, 8, 12), 603)
+    a = q.multiply_hermop_sym2_fp32(None, half(geo, 8, 1, x), inv).data
+    b = q.multiply_hermop_sym2_fp32(None, half(geo, 8, 1, x * 2.0**-20), inv).data
+    assert np.array_equal(a * 2.0**-20, b)
+
+
+@pytest.mark.parametrize("solver,three", [("mixed16", "0"), ("mixed", "1")])
+def test_three_phase_solver_reaches_tolerance(monkeypatch, solver, three):
+    """Single right-hand side through the three-phase solver: FP64 true
+    residual <= stop_rsd and the solution as close to the exact one as the
+    FP64 solver's at the same stop_rsd."""
+    print(f"\n[gpu] {solver} three-phase solve 8^3x16 Ls8", flush=True)
+    monkeypatch.setenv("QLAT_DWF_THREE_PHASE", three)
+    site = (8, 8, 8, 16)
+    U = rand_gauge(site, 0.3, 3)
+    geo, inv, orc = make(site, U, mass=0.05, ls=8, solver="fp64")
+    src = q.FermionField4d(geo)
+    src.data = rand_field((geo.total_volume(), 12), 4)
+    orc.set_params(1e-12, 200, 300)
+    exact, _ = orc.invert_4d(src.data)
+    inv.set_stop_rsd(1e-8)
+    e64 = rel((inv * src).data, exact)
+    inv.set_solver(solver)
+    sol = inv * src
+    st = inv.stats()
+    assert st["true_residual"] <= 1e-8
+    assert st["spinor_bits"] == (16 if solver == "mixed16" else 32)
+    assert st["reliable_updates"] > 0
+    assert rel(sol.data, exact) < 10 * e64 + 1e-9
+
+
+def test_fp16_storage_zmobius_solve():
+    """Complex s-matrices (zMoebius) through the FP16 path."""
+    print("\n[gpu] mixed16 zMoebius solve 4^3x8 Ls12", flush=True)
+    site = (4, 4, 4, 8)
+    U = rand_gauge(site, 0.4, 43)
+    geo, inv, orc = make(site, U, mass=0.05, ls=12, mobius_scale=1.0, omega=ZMOBIUS_OMEGA)
+    src = q.FermionField4d(geo)
+    src.data = rand_field((geo.total_volume(), 12), 44)
+    orc.set_params(1e-12, 200, 300)
+    exact, _ = orc.invert_4d(src.data)
+    inv.set_stop_rsd(1e-9)
+    sol = inv * src
+    assert inv.stats()["true_residual"] <= 1e-9
+    assert rel(sol.data, exact) < 1e-6
+
+
+@pytest.fixture(scope="module")
+def prop_case():
+    print("\n[gpu] FP16 multi-RHS propagator 4^3x8 Ls8", flush=True)
+    site = (4, 4, 4, 8)
+    U = rand_gauge(site, 0.4, 51)
+    orc = O.OracleInverter(site, U, mass=0.05, ls=8, m5=1.8, mobius_scale=2.0)
+    orc.set_params(1e-12, 200, 300)
+    rng = np.random.default_rng(52)
+    src = rng.standard_normal((512, 12, 12)) + 1j * rng.standard_normal((512, 12, 12))
+    src[:, :, 9:] = 0.0  # three zero columns
+    exact, _ = orc.invert_prop(src)
+    return site, U, src, exact
+
+
+@pytest.mark.parametrize("solver,tile,three", [("mixed16", "12", "0"), ("mixed16", "5", "0"),
+                                               ("mixed16", "1", "0"), ("mixed", "12", "1")])
+def test_three_phase_multi_rhs_propagator(monkeypatch, prop_case, solver, tile, three):
+    """12 columns as one block (tile 12), in uneven tiles (5+5+2) and one by
+    one: every column reaches the FP64 true residual and matches the oracle;
+    zero columns stay zero."""
+    monkeypatch.setenv("QLAT_DWF_RHS_TILE", tile)
+    monkeypatch.setenv("QLAT_DWF_THREE_PHASE", three)
+    site, U, src, exact = prop_case
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    fa = q.FermionAction(mass=0.05, ls=8, m5=1.8, mobius_scale=2.0)
+    inv = q.InverterDomainWall(gf=gf, fa=fa, solver=solver)
+    inv.set_stop_rsd(1e-9)
+    p = q.Prop(geo)
+    p.data = src
+    sol = inv * p
+    st = inv.stats()
+    assert st["true_residual"] <= 1e-9
+    assert st["spinor_bits"] == (16 if solver == "mixed16" else 32)
+    assert not np.any(sol.data[:, :, 9:])
+    for j in range(9):
+        assert rel(sol.data[:, :, j], exact[:, :, j]) < 1e-6

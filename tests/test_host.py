@@ -26,7 +26,7 @@ def test_library_exports_every_header_symbol():
     for s in syms:
         assert hasattr(L, s), s
     assert sorted(_lib.EXPORTED) == syms
-    assert L.qlat_dwf_abi_version() == 1
+    assert L.qlat_dwf_abi_version() == 2
 
 
 def test_null_handle_is_an_error_not_a_crash():
