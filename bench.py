@@ -157,9 +157,9 @@ def main():
     m5_32_ms = q.bench_kernel(inv, 2, 32, args.kernel_iters)
     herm32_ms = q.bench_kernel(inv, 3, 32, max(1, args.kernel_iters // 5))
     herm64_ms = q.bench_kernel(inv, 3, 64, max(1, args.kernel_iters // 5))
-    tile = int(os.environ.get("QLAT_DWF_RHS_TILE", "12"))
     prec_tp = 16 if args.solver == "mixed16" else 32
     hm5s_ms = q.bench_kernel(inv, 5, prec_tp, max(1, args.kernel_iters // 5))
+    tile = inv.stats()["rhs_tile"]  # the solver's tile (autotuned unless QLAT_DWF_RHS_TILE is set)
     step_ms = q.bench_kernel(inv, 6, prec_tp, max(1, args.kernel_iters // 5))
     hermtp_ms = q.bench_kernel(inv, 7, prec_tp, max(1, args.kernel_iters // 10))
     site_bytes = 52 if prec_tp == 16 else 96

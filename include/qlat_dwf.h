@@ -78,6 +78,7 @@ typedef struct {
   double seconds;         /* wall clock of the solve call                        */
   int spinor_bits;        /* storage of the low-precision Krylov vectors (16/32),
                              0 for the FP64 solver                              */
+  int rhs_tile;           /* columns per multi-RHS block of the last batched solve */
 } qlat_dwf_solve_stats;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -188,6 +189,24 @@ int qlat_dwf_invert_prop(qlat_dwf_inverter* inv, double* sol, const double* src,
  * from a caller-supplied odd half-field start vector. */
 int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,
                                               double* result);
+
+/* ------------------------------------------------------------- batches
+ * No single counterpart in the reference, which solves one right-hand side
+ * at a time with one InverterDomainWall per fermion action
+ * (invert(FermionField5d&, ...), dslash.h:580-581, dslash.cpp:1335-1341).
+ * sol[j] = invert(src[j]) with inverter invs[j], j < n: full 5D fields in
+ * reference layout ([V][Ls][12] complex128).  All inverters must live on one
+ * device and share the lattice, Ls and gauge field; they may differ in mass,
+ * M5 and b, c.  Solver, stop_rsd and iteration limits are those of invs[0].
+ * With the mixed solvers the systems go through the multi-RHS solver in tiles
+ * (autotuned width, or QLAT_DWF_RHS_TILE), each column with its own
+ * operator, reliable updates and retirement; with the FP64 solver they are
+ * solved one after the other.  iters[j] receives each system's CG
+ * iterations; statistics are collected on invs[0]. */
+int qlat_dwf_invert_5d_batch(qlat_dwf_inverter* const* invs, int n, double* const* sol, const double* const* src,
+                             long* iters);
+int qlat_dwf_invert_5d_batch_device(qlat_dwf_inverter* const* invs, int n, double* const* sol,
+                                    const double* const* src, long* iters);
 
 /* ---------------------------------------------------------------- force
  * No counterpart in the reference (it has no fermion force); the

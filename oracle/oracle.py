@@ -147,7 +147,10 @@ class OracleInverter:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().oracle_inv_destroy(h)
+            try:
+                lib().oracle_inv_destroy(h)
+            except Exception:  # interpreter shutdown: the library may be gone
+                pass
             self._h = None
 
     def set_params(self, stop_rsd=1e-8, max_num_iter=200,

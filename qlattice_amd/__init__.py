@@ -32,6 +32,7 @@ from .inverter import (
     dot_product,
     find_max_eigen_value_hermop_sym2,
     invert,
+    invert_batch,
     multiply_d_minus,
     multiply_hermop_sym2,
     multiply_hermop_sym2_fp32,

@@ -56,6 +56,8 @@ EXPORTED = [
     "qlat_dwf_bench_kernel",
     "qlat_dwf_fermion_force",
     "qlat_dwf_fermion_force_device",
+    "qlat_dwf_invert_5d_batch",
+    "qlat_dwf_invert_5d_batch_device",
     "qlat_dwf_inverter_set_gauge",
 ]
 
@@ -84,6 +86,7 @@ class SolveStatsC(ctypes.Structure):
         ("true_residual", ctypes.c_double),
         ("seconds", ctypes.c_double),
         ("spinor_bits", ctypes.c_int),
+        ("rhs_tile", ctypes.c_int),
     ]
 
 
@@ -142,6 +145,9 @@ def load():
     for name in ("qlat_dwf_fermion_force", "qlat_dwf_fermion_force_device"):
         getattr(L, name).argtypes = [_vp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _lp]
     L.qlat_dwf_inverter_set_gauge.argtypes = [_vp, _dp, ctypes.c_int]
+    for name in ("qlat_dwf_invert_5d_batch", "qlat_dwf_invert_5d_batch_device"):
+        getattr(L, name).argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_dp),
+                                     ctypes.POINTER(_dp), _lp]
     for name in EXPORTED:
         if name not in ("qlat_dwf_last_error", "qlat_dwf_abi_version"):
             getattr(L, name).restype = ctypes.c_int

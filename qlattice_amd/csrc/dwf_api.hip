@@ -149,6 +149,12 @@ enum Work64 {
 };
 enum Work32 { V_R, V_P, V_AP, V_X, V_T, V_T1, V_T2, V_T3, V_Y, W_COUNT32 };
 
+// Tile width (columns per multi-RHS block) and sites per block of the
+// three-phase solver.
+struct TileChoice {
+  int width, sb;
+};
+
 }  // namespace
 
 struct qlat_dwf_inverter {
@@ -168,6 +174,8 @@ struct qlat_dwf_inverter {
   int hop_spt64 = -102, hop_spt32 = -4;
   bool fused32 = true;  // FP32 hop + s-matrix fusion (QLAT_DWF_FUSED=0 disables)
   bool three_phase32 = false;  // FP32 storage on the three-phase kernels (QLAT_DWF_THREE_PHASE=1)
+  int tp_sb = 32;              // sites per block of the three-phase kernels (QLAT_DWF_TP_SB)
+  TileChoice tuned[2] = {};    // autotuned tile per storage precision (FP32, FP16)
   bool real_mat[16] = {};  // s-matrix MatId has only real entries (Moebius)
 
   double2* l64 = nullptr;
@@ -185,7 +193,7 @@ struct qlat_dwf_inverter {
   double* red_host = nullptr;  // pinned
 
   // multi-RHS: FP32 stacks of `mcap` fields and per-column FP64 fields
-  int rhs_tile = 12;  // columns per multi-RHS solve (QLAT_DWF_RHS_TILE)
+  int rhs_tile = 0;  // columns per multi-RHS solve (QLAT_DWF_RHS_TILE; 0 = autotuned / 12)
   int mcap = 0;
   float4* m32[8] = {};
   uint4* mh[5] = {};            // FP16 (per-site scale) stacks of the three-phase solver
@@ -892,12 +900,16 @@ long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
 struct ThreePhase {
   Inv& I;
   bool half;
+  int sb;           // output sites per block of the three-phase kernels
   int64_t f32, fh;  // column strides of the FP32 and of the low-storage stacks
   float4 *R, *X;
   void *P, *AP, *T, *B, *Y;
-  ThreePhase(Inv& inv, int ncol, bool h) : I(inv), half(h)
+  Inv* colinv[qdwf::kMaxRhs];  // the inverter (mass) of each stack column
+  ThreePhase(Inv& inv, int ncol, bool h, int sb_ = 0) : I(inv), half(h), sb(sb_ ? sb_ : inv.tp_sb)
   {
-    if (!qdwf::three_phase_ok(I.ls)) throw ArgError("the three-phase kernels need 6 <= ls <= 16, ls even");
+    if (!qdwf::three_phase_ok(I.ls, sb)) sb = 32;
+    if (!qdwf::three_phase_ok(I.ls, sb)) throw ArgError("the three-phase kernels need 6 <= ls <= 16, ls even");
+    for (auto*& p : colinv) p = &I;
     f32 = n32(I);
     fh = half ? fieldh_units(I) : f32;
     R = MV(I, M_R, ncol);
@@ -927,6 +939,17 @@ struct ThreePhase {
     for (int i = 0; i < qdwf::kMaxRhs; ++i) cs.rhs[i] = i < cs.n ? act[i] : 0;
     return cs;
   }
+  // per-column s-matrices `mat` of the columns of cs; false if any is complex
+  bool mats(const qdwf::RhsSet& cs, int mat, const cplx<float>* out[qdwf::kMaxRhs]) const
+  {
+    bool real = true;
+    for (int i = 0; i < qdwf::kMaxRhs; ++i) {
+      const Inv& J = *colinv[i < cs.n ? cs.rhs[i] : cs.rhs[0]];
+      out[i] = Prec<float>::mat(J, mat);
+      if (i < cs.n) real = real && J.real_mat[mat];
+    }
+    return real;
+  }
   // x += alpha p (updx); p = r + beta p; t = K p
   void step(const std::vector<int>& cols, const double* alpha, const double* beta, bool updx)
   {
@@ -939,8 +962,9 @@ struct ThreePhase {
     }
     a.cols = colset(cols);
     a.xr_fstride = f32;
-    a.mat = Prec<float>::mat(I, MAT_K);
-    a.real_mat = I.real_mat[MAT_K];
+    a.sb = sb;
+    a.mat = nullptr;
+    a.real_mat = mats(a.cols, MAT_K, a.mats);
     a.x = updx ? X : nullptr;
     a.p = P;
     a.r = R;
@@ -964,8 +988,9 @@ struct ThreePhase {
     a.out = out;
     a.x = x;
     a.links = I.l32;
-    a.mat = Prec<float>::mat(I, mat);
-    a.real_mat = I.real_mat[mat];
+    a.sb = sb;
+    a.mat = nullptr;
+    a.real_mat = mats(cs, mat, a.mats);
     qdwf::launch_hop_m5s(a, dag, half, I.stream);
     check_launch();
   }
@@ -991,9 +1016,18 @@ struct ThreePhase {
     check_launch();
     hopm5(B, Y, 3 - eo, true, MAT_K_DAG, 1.f, nullptr, 0.f, nullptr, cs);
     hopm5(AP, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, P, cs);
-    qdwf::launch_reduce_rows(I.part, (int)qdwf::hop_m5_blocks(I.geo), cs.n, I.red, I.stream);
+    qdwf::launch_reduce_rows(I.part, (int)qdwf::hop_m5s_blocks(I.geo, sb), cs.n, I.red, I.stream);
     check_launch();
     I.stats.hermop_low += cs.n;
+  }
+  // one CG iteration's kernels on the columns of cs (for timing)
+  void iteration(const std::vector<int>& cols)
+  {
+    const qdwf::RhsSet cs = colset(cols);
+    const std::vector<double> a(qdwf::kMaxRhs, 1e-3), b(qdwf::kMaxRhs, 0.5);
+    hermop_dot(cs, 1);
+    axpy_norm(cs, a.data());
+    step(cols, a.data(), b.data(), true);
   }
   // r_i -= alpha_i ap_i, |r_i|^2 into I.red[i]
   void axpy_norm(const qdwf::RhsSet& cs, const double* alpha)
@@ -1034,11 +1068,15 @@ void read_red(Inv& I, int n, double* out)
 // true residual.  b64[c], x64[c]: FP64 source and solution (x64 is
 // overwritten).  Returns the summed iterations.
 long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64, double stop_rsd, long cap,
-                      bool half)
+                      bool half, int sb = 0, Inv* const* colinv = nullptr, long* col_iters = nullptr)
 {
   // Each column is solved for b / |b| (the low-precision vectors stay O(1)
   // whatever the source's scale); rr, targets and alpha are in those units.
-  ThreePhase tp(I, ncol, half);
+  // Column c's operator is that of colinv[c] (default I); all of them share
+  // I's stream, lattice and gauge field.
+  ThreePhase tp(I, ncol, half, sb);
+  if (colinv)
+    for (int c = 0; c < ncol; ++c) tp.colinv[c] = colinv[c];
   const int64_t nd = n64(I);
   double2* r64 = W(I, W_CG_R);
   double2* ax64 = W(I, W_CG_AP);
@@ -1063,7 +1101,7 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
   const double d2 = I.delta * I.delta;
   // |b - A x64|^2 / |b|^2 with r64 = b - A x64
   auto true_residual = [&](int c) {
-    op_hermop<double>(I, ax64, x64[c], 1);
+    op_hermop<double>(*tp.colinv[c], ax64, x64[c], 1);
     I.stats.hermop_fp64 += 1;
     copy64(I, r64, b64[c]);
     qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
@@ -1124,9 +1162,68 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
     const double rel = std::sqrt(true_residual(c));
     if (rel > I.stats.true_residual) I.stats.true_residual = rel;
     total += 1;
+    iters[c] += 1;  // like cg_with_f: max_num_iter + 1 when not converged
   }
+  if (col_iters)
+    for (int c = 0; c < ncol; ++c) col_iters[c] = iters[c];
   I.stats.spinor_bits = half ? 16 : 32;
   return total;
+}
+
+// Tile width (columns per multi-RHS block, from {4, 8, 12, 16} up to
+// max(ncol, 4)) and sites per block (16 or 32) of the three-phase solver
+// with the least time per column and CG iteration, measured with HIP events
+// on this lattice; cached per storage precision.
+TileChoice autotune_tile(Inv& I, bool half, int ncol)
+{
+  TileChoice& t = I.tuned[half ? 1 : 0];
+  const int wmax = std::min(qdwf::kMaxRhs, std::max(ncol, 4));
+  if (t.width && t.width <= wmax) return t;
+  const qlat_dwf_solve_stats saved = I.stats;
+  MV(I, M_R, wmax);  // allocate the stacks once at the largest width
+  hipEvent_t e0, e1;
+  QDWF_HIP(hipEventCreate(&e0));
+  QDWF_HIP(hipEventCreate(&e1));
+  double best = 0;
+  TileChoice choice{0, 32};
+  for (int sb : {32, 16}) {
+    if (!qdwf::three_phase_ok(I.ls, sb)) continue;
+    for (int w : {4, 8, 12, 16}) {
+      if (w > wmax) continue;
+      ThreePhase tp(I, w, half, sb);
+      std::vector<int> cols;
+      for (int c = 0; c < w; ++c) cols.push_back(c);
+      const size_t lb = (size_t)tp.fh * tp.slot_bytes() * w;
+      for (void* f : {tp.P, tp.T}) QDWF_HIP(hipMemsetAsync(f, 0x3f, lb, I.stream));
+      QDWF_HIP(hipMemsetAsync(tp.R, 0x3f, field32_bytes(I) * w, I.stream));
+      QDWF_HIP(hipMemsetAsync(tp.X, 0, field32_bytes(I) * w, I.stream));
+      tp.iteration(cols);  // warm-up
+      QDWF_HIP(hipEventRecord(e0, I.stream));
+      for (int k = 0; k < 2; ++k) tp.iteration(cols);
+      QDWF_HIP(hipEventRecord(e1, I.stream));
+      QDWF_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      QDWF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      const double per_col = ms / (2.0 * w);
+      if (!choice.width || per_col < best) {
+        best = per_col;
+        choice = TileChoice{w, sb};
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  I.stats = saved;
+  t = choice;
+  return t;
+}
+
+// Tile used by the three-phase solver for ncol columns: QLAT_DWF_RHS_TILE
+// (with QLAT_DWF_TP_SB) when set, else the autotuner's choice.
+TileChoice tile_for(Inv& I, bool half, int ncol)
+{
+  if (I.rhs_tile > 0) return TileChoice{I.rhs_tile, I.tp_sb};
+  return autotune_tile(I, half, ncol);
 }
 
 bool use_three_phase(const Inv& I)
@@ -1410,6 +1507,64 @@ void restore_column(Inv& I, const double2* x_o, const double2* oep, double2* out
   check_launch();
 }
 
+// Multi-RHS driver of the mixed solvers: columns j < n, column j with the
+// operator of colinv[j] (all sharing I's stream, lattice and gauge field).
+// load(j, J, in_e, in_o) writes column j's 5D source halves, store(j, J,
+// out_e, out_o) takes its solution halves; both run on the shared stream.
+// The columns go in tiles through cg_mixed_multi_s (three-phase kernels) or
+// cg_mixed_multi (FP32 fused kernels, one mass only).  Returns the summed
+// iterations; col_iters[j] receives each column's.
+template <class Load, class Store>
+long solve_columns(Inv& I, int n, Inv* const* colinv, Load&& load, Store&& store, long* col_iters)
+{
+  bool several = false;
+  for (int j = 0; j < n; ++j) several = several || colinv[j] != &I;
+  const bool three = use_three_phase(I) || (several && qdwf::three_phase_ok(I.ls));
+  const bool half = I.solver == QLAT_DWF_SOLVER_MIXED_FP16;
+  TileChoice tc{I.rhs_tile > 0 ? I.rhs_tile : 12, I.tp_sb};
+  if (three) tc = tile_for(I, half, n);
+  tc.width = std::max(1, std::min(tc.width, qdwf::kMaxRhs));
+  I.stats.rhs_tile = tc.width;
+  long total = 0;
+  for (int c0 = 0; c0 < n; c0 += tc.width) {
+    const int w = std::min(tc.width, n - c0);
+    double2* bs[qdwf::kMaxRhs];
+    double2* xs[qdwf::kMaxRhs];
+    for (int j = 0; j < w; ++j) {
+      Inv& J = *colinv[c0 + j];
+      double2* ie = W(J, W_IN_E);
+      double2* io = W(J, W_IN_O);
+      load(c0 + j, J, ie, io);
+      bs[j] = C64(I, j, 0);
+      xs[j] = C64(I, j, 1);
+      const double q = prepare_column(J, ie, io, bs[j], C64(I, j, 2));
+      if (q == 0.0) zero64(I, bs[j]);  // dm_in == 0: solution zero (dslash.h:477-482)
+    }
+    if (three) {
+      total += cg_mixed_multi_s(I, w, bs, xs, I.stop_rsd, I.max_num_iter * I.max_cycle, half, tc.sb, colinv + c0,
+                                col_iters ? col_iters + c0 : nullptr);
+    } else {
+      for (int j = 0; j < w; ++j)
+        if (colinv[c0 + j] != &I) throw ArgError("batched solves over several inverters need the three-phase solver");
+      const long it = cg_mixed_multi(I, w, I.stop_rsd, I.max_num_iter * I.max_cycle);
+      total += it;
+      if (col_iters)
+        for (int j = 0; j < w; ++j) col_iters[c0 + j] = it / w;  // the FP32 fused path counts only the sum
+      I.stats.spinor_bits = 32;
+    }
+    I.stats.cycles += 1;
+    for (int j = 0; j < w; ++j) {
+      Inv& J = *colinv[c0 + j];
+      double2* oe = W(J, W_OUT_E);
+      double2* oo = W(J, W_OUT_O);
+      restore_column(J, xs[j], C64(I, j, 2), oe, oo);
+      store(c0 + j, J, oe, oo);
+    }
+  }
+  I.stats.total_iter += total;
+  return total;
+}
+
 // Propagator (or 4D fermion, ncol = 1 with col = -1) on device.
 long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
 {
@@ -1419,45 +1574,21 @@ long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
   double2* oo = W(I, W_OUT_O);
   long total = 0;
   const int ncol = is_prop ? 12 : 1;
-  const int tile = std::min(I.rhs_tile, qdwf::kMaxRhs);
-  if (I.solver != QLAT_DWF_SOLVER_FP64 && is_prop && tile > 1) {
-    // multi-RHS: columns in tiles solved as one block
-    for (int c0 = 0; c0 < ncol; c0 += tile) {
-      const int w = std::min(tile, ncol - c0);
-      std::vector<int> nonzero;
-      for (int j = 0; j < w; ++j) {
-        qdwf::launch_prop_col_to_5d(src, ie, I.geo, I.ls, 2, c0 + j, I.stream);
-        qdwf::launch_prop_col_to_5d(src, io, I.geo, I.ls, 1, c0 + j, I.stream);
-        check_launch();
-        const double q = prepare_column(I, ie, io, C64(I, j, 0), C64(I, j, 2));
-        if (q == 0.0) {
-          // dm_in == 0: solution zero (dslash.h:477-482)
-          zero64(I, C64(I, j, 0));
-        }
-      }
-      if (use_three_phase(I)) {
-        double2* bs[qdwf::kMaxRhs];
-        double2* xs[qdwf::kMaxRhs];
-        for (int j = 0; j < w; ++j) {
-          bs[j] = C64(I, j, 0);
-          xs[j] = C64(I, j, 1);
-        }
-        total += cg_mixed_multi_s(I, w, bs, xs, I.stop_rsd, I.max_num_iter * I.max_cycle,
-                                  I.solver == QLAT_DWF_SOLVER_MIXED_FP16);
-      } else {
-        total += cg_mixed_multi(I, w, I.stop_rsd, I.max_num_iter * I.max_cycle);
-        I.stats.spinor_bits = 32;
-      }
-      I.stats.cycles += 1;
-      for (int j = 0; j < w; ++j) {
-        restore_column(I, C64(I, j, 1), C64(I, j, 2), oe, oo);
-        qdwf::launch_5d_to_prop_col(oe, sol, I.geo, I.ls, 2, c0 + j, I.stream);
-        qdwf::launch_5d_to_prop_col(oo, sol, I.geo, I.ls, 1, c0 + j, I.stream);
-        check_launch();
-      }
-    }
-    I.stats.total_iter += total;
-    return total;
+  if (I.solver != QLAT_DWF_SOLVER_FP64 && is_prop && I.rhs_tile != 1) {
+    std::vector<Inv*> colinv(ncol, &I);
+    return solve_columns(
+        I, ncol, colinv.data(),
+        [&](int j, Inv& J, double2* e, double2* o) {
+          qdwf::launch_prop_col_to_5d(src, e, J.geo, J.ls, 2, j, J.stream);
+          qdwf::launch_prop_col_to_5d(src, o, J.geo, J.ls, 1, j, J.stream);
+          check_launch();
+        },
+        [&](int j, Inv& J, const double2* e, const double2* o) {
+          qdwf::launch_5d_to_prop_col(e, sol, J.geo, J.ls, 2, j, J.stream);
+          qdwf::launch_5d_to_prop_col(o, sol, J.geo, J.ls, 1, j, J.stream);
+          check_launch();
+        },
+        nullptr);
   }
   for (int j = 0; j < ncol; ++j) {
     const int col = is_prop ? j : -1;
@@ -1470,6 +1601,90 @@ long invert_prop_dev(Inv& I, double2* sol, const double2* src, bool is_prop)
     check_launch();
   }
   return total;
+}
+
+// Every inverter of a batch runs on the first one's stream for the call.
+struct SharedStream {
+  std::vector<Inv*> invs;
+  std::vector<hipStream_t> saved;
+  SharedStream(Inv* const* v, int n)
+  {
+    for (int j = 0; j < n; ++j) {
+      invs.push_back(v[j]);
+      saved.push_back(v[j]->stream);
+    }
+    for (int j = 1; j < n; ++j) v[j]->stream = v[0]->stream;
+  }
+  ~SharedStream()
+  {
+    for (size_t j = 0; j < invs.size(); ++j) invs[j]->stream = saved[j];
+  }
+};
+
+void check_batch(qlat_dwf_inverter* const* invs, int n)
+{
+  QDWF_REQUIRE(invs && n > 0, "invalid argument");
+  for (int j = 0; j < n; ++j) {
+    check_inv(invs[j]);
+    const Inv& A = *invs[0];
+    const Inv& B = *invs[j];
+    QDWF_REQUIRE(B.device == A.device, "batched inverters must share one device");
+    QDWF_REQUIRE(B.ls == A.ls && B.geo.by == A.geo.by, "batched inverters must share Ls and the site order");
+    for (int mu = 0; mu < 4; ++mu) QDWF_REQUIRE(B.geo.n[mu] == A.geo.n[mu], "batched inverters must share the lattice");
+  }
+  QDWF_HIP(hipSetDevice(invs[0]->device));
+}
+
+// sol[j] = invert_5d(src[j]) with invs[j] (device AoS full 5D fields)
+long invert_5d_batch_dev(qlat_dwf_inverter* const* invs, int n, double2* const* sol, const double2* const* src,
+                         long* iters)
+{
+  Inv& I = *invs[0];
+  SharedStream shared(invs, n);
+  for (int j = 0; j < n; ++j) reset_stats(*invs[j]);
+  long total = 0;
+  if (I.solver == QLAT_DWF_SOLVER_FP64) {
+    for (int j = 0; j < n; ++j) {
+      Inv& J = *invs[j];
+      const double s0 = J.stop_rsd;
+      const long m0 = J.max_num_iter, c0 = J.max_cycle;
+      J.stop_rsd = I.stop_rsd;
+      J.max_num_iter = I.max_num_iter;
+      J.max_cycle = I.max_cycle;
+      long it = 0;
+      try {
+        it = invert_5d_dev(J, sol[j], src[j]);
+      } catch (...) {
+        J.stop_rsd = s0;
+        J.max_num_iter = m0;
+        J.max_cycle = c0;
+        throw;
+      }
+      J.stop_rsd = s0;
+      J.max_num_iter = m0;
+      J.max_cycle = c0;
+      iters[j] = it;
+      total += it;
+      I.stats.total_iter += j ? it : 0;
+      I.stats.hermop_fp64 += j ? J.stats.hermop_fp64 : 0;
+      I.stats.true_residual = std::max(I.stats.true_residual, J.stats.true_residual);
+    }
+    return total;
+  }
+  std::vector<Inv*> colinv(invs, invs + n);
+  return solve_columns(
+      I, n, colinv.data(),
+      [&](int j, Inv& J, double2* e, double2* o) {
+        qdwf::launch_pack5(src[j], e, J.geo, J.ls, 2, 1, J.stream);
+        qdwf::launch_pack5(src[j], o, J.geo, J.ls, 1, 1, J.stream);
+        check_launch();
+      },
+      [&](int j, Inv& J, const double2* e, const double2* o) {
+        qdwf::launch_unpack5(e, sol[j], J.geo, J.ls, 2, 1, J.stream);
+        qdwf::launch_unpack5(o, sol[j], J.geo, J.ls, 1, 1, J.stream);
+        check_launch();
+      },
+      iters);
 }
 
 // Pack an AoS gauge field already in device memory into the link layouts.
@@ -1701,7 +1916,8 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT64")) I->hop_spt64 = std::atoi(e);
     if (const char* e = std::getenv("QLAT_DWF_FUSED")) I->fused32 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_THREE_PHASE")) I->three_phase32 = std::atoi(e) != 0;
-    if (const char* e = std::getenv("QLAT_DWF_RHS_TILE")) I->rhs_tile = std::max(1, std::min(qdwf::kMaxRhs, std::atoi(e)));
+    if (const char* e = std::getenv("QLAT_DWF_RHS_TILE")) I->rhs_tile = std::max(0, std::min(qdwf::kMaxRhs, std::atoi(e)));
+    if (const char* e = std::getenv("QLAT_DWF_TP_SB")) I->tp_sb = std::atoi(e) == 16 ? 16 : 32;
     {
       // partial sums: BLAS grids, one per 64-site block (fused kernels), or
       // one per (256-site block, slot) (s-matrix kernel)
@@ -2034,6 +2250,55 @@ int qlat_dwf_invert_5d_device(qlat_dwf_inverter* inv, double* out, const double*
   });
 }
 
+int qlat_dwf_invert_5d_batch_device(qlat_dwf_inverter* const* invs, int n, double* const* sol,
+                                    const double* const* src, long* iters)
+{
+  return guarded([&] {
+    check_batch(invs, n);
+    QDWF_REQUIRE(sol && src && iters, "invalid argument");
+    for (int j = 0; j < n; ++j) QDWF_REQUIRE(sol[j] && src[j], "null field pointer");
+    const auto t0 = std::chrono::steady_clock::now();
+    invert_5d_batch_dev(invs, n, (double2* const*)sol, (const double2* const*)src, iters);
+    sync(*invs[0]);
+    invs[0]->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_invert_5d_batch(qlat_dwf_inverter* const* invs, int n, double* const* sol, const double* const* src,
+                             long* iters)
+{
+  return guarded([&] {
+    check_batch(invs, n);
+    QDWF_REQUIRE(sol && src && iters, "invalid argument");
+    for (int j = 0; j < n; ++j) QDWF_REQUIRE(sol[j] && src[j], "null field pointer");
+    Inv& I = *invs[0];
+    const size_t bytes = full_aos_bytes(I);
+    // one device copy of every source and solution for the call
+    double2* buf = (double2*)dev_alloc(2 * bytes * n);
+    try {
+      std::vector<double2*> dsol(n);
+      std::vector<const double2*> dsrc(n);
+      for (int j = 0; j < n; ++j) {
+        double2* d = buf + (size_t)2 * j * (bytes / sizeof(double2));
+        QDWF_HIP(hipMemcpyAsync(d, src[j], bytes, hipMemcpyHostToDevice, I.stream));
+        dsrc[j] = d;
+        dsol[j] = d + bytes / sizeof(double2);
+      }
+      const auto t0 = std::chrono::steady_clock::now();
+      invert_5d_batch_dev(invs, n, dsol.data(), dsrc.data(), iters);
+      for (int j = 0; j < n; ++j)
+        QDWF_HIP(hipMemcpyAsync(sol[j], dsol[j], bytes, hipMemcpyDeviceToHost, I.stream));
+      sync(I);
+      I.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } catch (...) {
+      (void)hipStreamSynchronize(I.stream);
+      (void)hipFree(buf);
+      throw;
+    }
+    QDWF_HIP(hipFree(buf));
+  });
+}
+
 int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter)
 {
   return guarded([&] {
@@ -2124,8 +2389,11 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
     std::unique_ptr<ThreePhase> tp;
     std::vector<int> cols;
     if (which >= 5) {
-      const int tile = std::min(I.rhs_tile, qdwf::kMaxRhs);
-      tp.reset(new ThreePhase(I, tile, precision == 16));
+      // the tile the solver uses for a 12-column propagator
+      const TileChoice tc = tile_for(I, precision == 16, 12);
+      const int tile = std::max(1, std::min(tc.width, qdwf::kMaxRhs));
+      I.stats.rhs_tile = tile;
+      tp.reset(new ThreePhase(I, tile, precision == 16, tc.sb));
       for (int c = 0; c < tile; ++c) cols.push_back(c);
       // finite, non-trivial input values
       const size_t lb = (size_t)tp->fh * tp->slot_bytes() * tile;

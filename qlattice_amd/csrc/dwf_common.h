@@ -273,6 +273,24 @@ struct SpinorIO<float> {
     for (int k = 0; k < 6; ++k)
       store_nt(&p[k * 64], make_float4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im));
   }
+  // raw loads (issued together, decoded later: more loads in flight per wave)
+  struct Raw {
+    float4 q[6];
+  };
+  __device__ __forceinline__ static void load_raw(const float4* __restrict__ f, int ls, int s, int64_t h, Raw& r)
+  {
+    const float4* p = f + sidx<6>(ls, s, 0, h);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.q[k] = p[k * 64];
+  }
+  __device__ __forceinline__ static void decode(const Raw& r, cplx<float> v[12])
+  {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      v[2 * k] = cmk<float>(r.q[k].x, r.q[k].y);
+      v[2 * k + 1] = cmk<float>(r.q[k].z, r.q[k].w);
+    }
+  }
 };
 
 // FP16 storage with a per-5D-site FP32 scale (arithmetic in FP32).  Stores
@@ -308,6 +326,28 @@ struct SpinorIO<Half16> {
   __device__ __forceinline__ static int64_t base(int ls, int s, int64_t h)
   {
     return ((h >> 6) * ls + s) * kUnits;
+  }
+  struct Raw {
+    uint4 q[3];
+    float sc;
+  };
+  __device__ __forceinline__ static void load_raw(const uint4* __restrict__ f, int ls, int s, int64_t h, Raw& r)
+  {
+    const uint4* p = f + base(ls, s, h);
+    const int lane = (int)(h & 63);
+    r.sc = ((const float*)(p + 192))[lane];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.q[j] = p[j * 64 + lane];
+  }
+  __device__ __forceinline__ static void decode(const Raw& r, cplx<float> v[12])
+  {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      v[4 * j + 0] = unpack_h2(r.q[j].x, r.sc);
+      v[4 * j + 1] = unpack_h2(r.q[j].y, r.sc);
+      v[4 * j + 2] = unpack_h2(r.q[j].z, r.sc);
+      v[4 * j + 3] = unpack_h2(r.q[j].w, r.sc);
+    }
   }
   __device__ __forceinline__ static void load(const uint4* __restrict__ f, int ls, int s, int64_t h,
                                               cplx<float> v[12])

@@ -64,7 +64,12 @@ struct CgStepArgs {
   void* p;
   const void* r;
   void* t;
-  int64_t xr_fstride;  // column spacing of x and r (three-phase variant: FP32 stacks)
+  // three-phase variant: column spacing of x and r (FP32 stacks), the
+  // s-matrix of each column (columns may belong to different masses) and
+  // the sites per block (16 or 32)
+  int64_t xr_fstride;
+  const cplx<T>* mats[kMaxRhs];
+  int sb;
 };
 
 // FP32 fused hop + s-matrix: out = alpha * Mat (H in) + beta * x
@@ -83,6 +88,9 @@ struct HopM5Args {
   const void* dotp;  // optional (dag xpay form): part[i*nblk + block] = Re <dotp, out>
   double* part;
   RhsSet cols;       // in/out/x/dotp are stacks of fields
+  // three-phase variant: the s-matrix of each column, sites per block (16 or 32)
+  const cplx<float>* mats[kMaxRhs];
+  int sb;
 };
 
 // r_i -= alpha_i ap_i with per-column partial |r_i|^2 (part[i*nblk + block])
@@ -130,7 +138,8 @@ bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st);
 int64_t hop_m5_blocks(const LatGeo& g);
 // Three-phase kernels for FP32 or FP16 (per-site scale) spinor storage
 // (half = true); false when ls does not fit them (6 <= ls <= 16, ls even).
-bool three_phase_ok(int ls);
+bool three_phase_ok(int ls, int sb = 32);
+int64_t hop_m5s_blocks(const LatGeo& g, int sb);
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st);
 bool launch_cg_step_s(const CgStepArgs<float>& a, bool half, hipStream_t st);
 int launch_axpy_norm_site(const AxpyNormSiteArgs& a, bool half, hipStream_t st);

@@ -504,7 +504,12 @@ k_hop_m5s(HopM5Args a)
     cplx<float> acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = cmk<float>(0, 0);
-#define QDWF_HOPM5S_DIR(D, MU, SG)                                  \
+    // neighbour spinors are loaded in batches of NB directions before any
+    // of them is used, so that each wave keeps several gathers in flight
+    constexpr int NB = std::is_same<ST, Half16>::value ? 8 : 4;
+    typename SpinorIO<ST>::Raw raw[NB];
+#define QDWF_LD(J, D) SpinorIO<ST>::load_raw(in, ls, row, nb[D], raw[J]);
+#define QDWF_CP(J, D, MU, SG)                                       \
   {                                                                 \
     cplx<float> W[9];                                               \
     _Pragma("unroll") for (int e = 0; e < 9; ++e)                   \
@@ -513,18 +518,22 @@ k_hop_m5s(HopM5Args a)
       W[e] = cmk<float>(q.x, q.y);                                  \
     }                                                               \
     cplx<float> psi[12];                                            \
-    SpinorIO<ST>::load(in, ls, row, nb[D], psi);                    \
+    SpinorIO<ST>::decode(raw[J], psi);                              \
     hop_term<MU, SG>(W, psi, acc);                                  \
   }
-    QDWF_HOPM5S_DIR(0, 0, SF)
-    QDWF_HOPM5S_DIR(1, 1, SF)
-    QDWF_HOPM5S_DIR(2, 2, SF)
-    QDWF_HOPM5S_DIR(3, 3, SF)
-    QDWF_HOPM5S_DIR(4, 0, SB_)
-    QDWF_HOPM5S_DIR(5, 1, SB_)
-    QDWF_HOPM5S_DIR(6, 2, SB_)
-    QDWF_HOPM5S_DIR(7, 3, SB_)
-#undef QDWF_HOPM5S_DIR
+    if constexpr (NB == 8) {
+      QDWF_LD(0, 0) QDWF_LD(1, 1) QDWF_LD(2, 2) QDWF_LD(3, 3)
+      QDWF_LD(4, 4) QDWF_LD(5, 5) QDWF_LD(6, 6) QDWF_LD(7 % NB, 7)
+      QDWF_CP(0, 0, 0, SF) QDWF_CP(1, 1, 1, SF) QDWF_CP(2, 2, 2, SF) QDWF_CP(3, 3, 3, SF)
+      QDWF_CP(4 % NB, 4, 0, SB_) QDWF_CP(5 % NB, 5, 1, SB_) QDWF_CP(6 % NB, 6, 2, SB_) QDWF_CP(7 % NB, 7, 3, SB_)
+    } else {
+      QDWF_LD(0, 0) QDWF_LD(1, 1) QDWF_LD(2, 2) QDWF_LD(3, 3)
+      QDWF_CP(0, 0, 0, SF) QDWF_CP(1, 1, 1, SF) QDWF_CP(2, 2, 2, SF) QDWF_CP(3, 3, 3, SF)
+      QDWF_LD(0, 4) QDWF_LD(1, 5) QDWF_LD(2, 6) QDWF_LD(3, 7)
+      QDWF_CP(0, 4, 0, SB_) QDWF_CP(1, 5, 1, SB_) QDWF_CP(2, 6, 2, SB_) QDWF_CP(3, 7, 3, SB_)
+    }
+#undef QDWF_LD
+#undef QDWF_CP
     const float sc = -0.5f * a.alpha;
 #pragma unroll
     for (int k = 0; k < 6; ++k)
@@ -532,7 +541,7 @@ k_hop_m5s(HopM5Args a)
                                      sc * acc[2 * k + 1].im);
   }
   __syncthreads();
-  smat_lds<SB, LSMAX, REAL>(hr, a.mat, ls, site, row, active);
+  smat_lds<SB, LSMAX, REAL>(hr, pick(a.mats, ci), ls, site, row, active);
   double dot = 0.0;
   if (active && row < ls) {
     cplx<float> v[12];
@@ -597,19 +606,28 @@ static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
 #undef QDWF_L
 }
 
-bool three_phase_ok(int ls) { return ls >= 6 && ls <= 16 && (kHopM5Sites * ls) % 64 == 0; }
+// phase 2 needs >= 6 rows; whole waves (sb * ls multiple of 64) for the
+// block reductions
+bool three_phase_ok(int ls, int sb) { return ls >= 6 && ls <= 16 && (sb == 16 || sb == 32) && (sb * ls) % 64 == 0; }
+int64_t hop_m5s_blocks(const LatGeo& g, int sb) { return (g.vh + sb - 1) / sb; }
+
+template <class ST, int SB>
+static void launch_hop_m5s_sb(const HopM5Args& a, bool dag, hipStream_t st)
+{
+  if (a.ls <= 8) launch_hop_m5s_ls<ST, SB, 8>(a, dag, st);
+  else if (a.ls <= 12) launch_hop_m5s_ls<ST, SB, 12>(a, dag, st);
+  else launch_hop_m5s_ls<ST, SB, 16>(a, dag, st);
+}
 
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
 {
-  if (!three_phase_ok(a.ls)) return false;
+  if (!three_phase_ok(a.ls, a.sb)) return false;
   if (half) {
-    if (a.ls <= 8) launch_hop_m5s_ls<Half16, kHopM5Sites, 8>(a, dag, st);
-    else if (a.ls <= 12) launch_hop_m5s_ls<Half16, kHopM5Sites, 12>(a, dag, st);
-    else launch_hop_m5s_ls<Half16, kHopM5Sites, 16>(a, dag, st);
+    if (a.sb == 16) launch_hop_m5s_sb<Half16, 16>(a, dag, st);
+    else launch_hop_m5s_sb<Half16, 32>(a, dag, st);
   } else {
-    if (a.ls <= 8) launch_hop_m5s_ls<float, kHopM5Sites, 8>(a, dag, st);
-    else if (a.ls <= 12) launch_hop_m5s_ls<float, kHopM5Sites, 12>(a, dag, st);
-    else launch_hop_m5s_ls<float, kHopM5Sites, 16>(a, dag, st);
+    if (a.sb == 16) launch_hop_m5s_sb<float, 16>(a, dag, st);
+    else launch_hop_m5s_sb<float, 32>(a, dag, st);
   }
   return true;
 }
@@ -652,7 +670,7 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
       hr[row][k][site] = make_float4(pv[2 * k].re, pv[2 * k].im, pv[2 * k + 1].re, pv[2 * k + 1].im);
   }
   __syncthreads();
-  smat_lds<SB, LSMAX, REAL>(hr, a.mat, ls, site, row, active);
+  smat_lds<SB, LSMAX, REAL>(hr, pick(a.mats, ci), ls, site, row, active);
   if (active && row < ls) {
     cplx<float> v[12];
 #pragma unroll
@@ -665,10 +683,9 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
   }
 }
 
-template <class ST, int LSMAX>
+template <class ST, int SB, int LSMAX>
 static void launch_cg_step_s_ls(const CgStepArgs<float>& a, hipStream_t st)
 {
-  constexpr int SB = kHopM5Sites;
   const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB), a.cols.n);
   const dim3 blk(SB * a.ls);
   const bool updx = a.x != nullptr;
@@ -681,17 +698,23 @@ static void launch_cg_step_s_ls(const CgStepArgs<float>& a, hipStream_t st)
   }
 }
 
+template <class ST, int SB>
+static void launch_cg_step_s_sb(const CgStepArgs<float>& a, hipStream_t st)
+{
+  if (a.ls <= 8) launch_cg_step_s_ls<ST, SB, 8>(a, st);
+  else if (a.ls <= 12) launch_cg_step_s_ls<ST, SB, 12>(a, st);
+  else launch_cg_step_s_ls<ST, SB, 16>(a, st);
+}
+
 bool launch_cg_step_s(const CgStepArgs<float>& a, bool half, hipStream_t st)
 {
-  if (!three_phase_ok(a.ls)) return false;
+  if (!three_phase_ok(a.ls, a.sb)) return false;
   if (half) {
-    if (a.ls <= 8) launch_cg_step_s_ls<Half16, 8>(a, st);
-    else if (a.ls <= 12) launch_cg_step_s_ls<Half16, 12>(a, st);
-    else launch_cg_step_s_ls<Half16, 16>(a, st);
+    if (a.sb == 16) launch_cg_step_s_sb<Half16, 16>(a, st);
+    else launch_cg_step_s_sb<Half16, 32>(a, st);
   } else {
-    if (a.ls <= 8) launch_cg_step_s_ls<float, 8>(a, st);
-    else if (a.ls <= 12) launch_cg_step_s_ls<float, 12>(a, st);
-    else launch_cg_step_s_ls<float, 16>(a, st);
+    if (a.sb == 16) launch_cg_step_s_sb<float, 16>(a, st);
+    else launch_cg_step_s_sb<float, 32>(a, st);
   }
   return true;
 }

@@ -333,6 +333,33 @@ def invert(out, in_, inv):
     return it.value
 
 
+def invert_batch(invs, srcs):
+    """Solve srcs[j] (full FermionField5d) with invs[j] for every j in one
+    batched call (qlat_dwf_invert_5d_batch): the inverters share lattice, Ls
+    and gauge field and may differ in mass, M5, b, c; with the mixed solvers
+    the systems run as multi-RHS tiles with per-column retirement.  Solver
+    and stopping parameters are those of invs[0].  Returns (solutions,
+    per-system CG iterations)."""
+    n = len(invs)
+    assert n == len(srcs) and n > 0
+    data = [np.ascontiguousarray(f.data, dtype=np.complex128) for f in srcs]
+    for f in srcs:
+        assert isinstance(f, FermionField5d) and f.eo == 0
+    outs = [np.empty_like(d) for d in data]
+    hv = (ctypes.c_void_p * n)(*[inv.cdata for inv in invs])
+    sp = (_lib._dp * n)(*[_lib.ptr(d) for d in data])
+    op = (_lib._dp * n)(*[_lib.ptr(o) for o in outs])
+    it = np.zeros(n, dtype=np.int64)
+    _lib.check(_lib.load().qlat_dwf_invert_5d_batch(hv, n, op, sp, it.ctypes.data_as(_lib._lp)),
+               "qlat_dwf_invert_5d_batch")
+    sols = []
+    for f, o in zip(srcs, outs):
+        r = FermionField5d(f.geo, f.ls, 0)
+        r.data = o
+        sols.append(r)
+    return sols, it
+
+
 def find_max_eigen_value_hermop_sym2(inv, start, max_iter=100):
     """Power iteration for the largest eigenvalue of Mpc^+ Mpc
     (dslash.cpp:1349-1369) from the odd half-field `start`."""
@@ -347,7 +374,9 @@ def find_max_eigen_value_hermop_sym2(inv, start, max_iter=100):
 
 def bench_kernel(inv, which, precision, iters):
     """Average launch time (ms) of one kernel, HIP events on the inverter's
-    stream.  which: 0 hop, 1 hop^+, 2 s-matrix, 3 Mpc^+ Mpc."""
+    stream.  which: 0 hop, 1 hop^+, 2 s-matrix, 3 Mpc^+ Mpc, 4 copy
+    (precision 64/32); 5 hop + s-matrix, 6 CG step, 7 Mpc^+ Mpc with dot on
+    the three-phase solver's tile (precision 32/16)."""
     ms = ctypes.c_double()
     _lib.check(_lib.load().qlat_dwf_bench_kernel(inv.cdata, int(which), int(precision), int(iters),
                                                  ctypes.byref(ms)),
