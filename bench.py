@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--solver", default="mixed16", choices=["mixed16", "mixed"],
                     help="mixed16: FP16 Krylov storage with per-site scale; mixed: FP32 storage")
+    ap.add_argument("--reliable-delta", type=float, default=0.01,
+                    help="mixed solvers: residual drop between FP64 reliable updates")
     ap.add_argument("--cpu-hermops", type=int, default=3, help="oracle hermop applications (0 = skip)")
     ap.add_argument("--kernel-iters", type=int, default=50)
     ap.add_argument("--kernels-only", action="store_true",
@@ -201,7 +203,7 @@ def main():
     it = ctypes.c_long()
 
     def solve(solver):
-        inv.set_solver(solver)
+        inv.set_solver(solver, args.reliable_delta)
         torch.cuda.synchronize()
         _lib.check(L.qlat_dwf_invert_prop_device(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src),
                                                  ctypes.byref(it)), "qlat_dwf_invert_prop_device")

@@ -107,7 +107,7 @@ int qlat_dwf_inverter_get_params(const qlat_dwf_inverter* inv, double* stop_rsd,
 
 /* Choose the inner solver, the role of a back-end's cg_with_herm_sym_2
  * override (grid.h:460-484).  reliable_delta: residual reduction between
- * FP64 residual replacements of the mixed solver (0 selects 0.1). */
+ * FP64 residual replacements of the mixed solvers (0 selects 0.01). */
 int qlat_dwf_inverter_set_solver(qlat_dwf_inverter* inv, int solver, double reliable_delta);
 
 int qlat_dwf_inverter_get_stats(const qlat_dwf_inverter* inv, qlat_dwf_solve_stats* stats);

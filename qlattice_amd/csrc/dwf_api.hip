@@ -169,13 +169,14 @@ struct qlat_dwf_inverter {
   long max_num_iter = 200;
   long max_cycle = 300;
   int solver = QLAT_DWF_SOLVER_FP64;
-  double delta = 0.1;
+  double delta = 0.01;
   // hop launch shapes (launch_hop): LDS-staged links; FP64 with streaming stores
   int hop_spt64 = -102, hop_spt32 = -4;
   bool fused32 = true;  // FP32 hop + s-matrix fusion (QLAT_DWF_FUSED=0 disables)
   bool three_phase32 = false;  // FP32 storage on the three-phase kernels (QLAT_DWF_THREE_PHASE=1)
   int tp_sb = 32;              // sites per block of the three-phase kernels (QLAT_DWF_TP_SB)
-  TileChoice tuned[2] = {};    // autotuned tile per storage precision (FP32, FP16)
+  int fp16_mode = 1;           // FP16 solver storage mode 1 or 2 (ThreePhase; QLAT_DWF_FP16_MODE)
+  TileChoice tuned[4] = {};    // autotuned tile per storage mode of the three-phase solver
   bool real_mat[16] = {};  // s-matrix MatId has only real entries (Moebius)
 
   double2* l64 = nullptr;
@@ -196,7 +197,7 @@ struct qlat_dwf_inverter {
   int rhs_tile = 0;  // columns per multi-RHS solve (QLAT_DWF_RHS_TILE; 0 = autotuned / 12)
   int mcap = 0;
   float4* m32[8] = {};
-  uint4* mh[5] = {};            // FP16 (per-site scale) stacks of the three-phase solver
+  uint4* mh[7] = {};            // FP16 (per-site scale) stacks of the three-phase solver
   std::vector<double2*> col64;  // 3 per column: b, x, out_e_p
 
   qlat_dwf_solve_stats stats{};
@@ -265,7 +266,7 @@ float4* MV(Inv& I, int id, int ncol)
 }
 
 // FP16 stacks (16-byte units per field: fieldh_units)
-enum MultiH { H_P, H_AP, H_T, H_B, H_Y };
+enum MultiH { H_P, H_AP, H_T, H_B, H_Y, H_R, H_X };
 int64_t fieldh_units(const Inv& I) { return (I.geo.stride >> 6) * I.ls * 208; }
 uint4* MH(Inv& I, int id, int ncol)
 {
@@ -278,7 +279,8 @@ uint4* MH(Inv& I, int id, int ncol)
   return I.mh[id];
 }
 
-// Per-column FP64 field: kind 0 = odd-prec source b, 1 = solution x, 2 = out_e_p.
+// Per-column FP64 field: kind 0 = odd-prec source b, 1 = solution x, 2 = out_e_p
+// (the FP32 fused multi-RHS path; the three-phase driver recomputes out_e_p).
 double2* C64(Inv& I, int col, int kind)
 {
   const size_t k = (size_t)col * 3 + kind;
@@ -894,27 +896,32 @@ long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
   return total;
 }
 
-// Krylov stacks and passes of the three-phase solver.  p, A p, t = K p and
-// the operator temporaries b, y are stored in FP16 with a per-5D-site scale
-// (half = true) or in FP32; r and the increment x are FP32 stacks.
+// Krylov stacks and passes of the three-phase solver.  storage 0: all
+// FP32; 1: p, A p, t = K p and the operator temporaries b, y in FP16 with a
+// per-5D-site scale, r and the increment x in FP32; 2: r and x in FP16 too
+// (measured to double the iterations at 24^3x64: an option, not the default).
 struct ThreePhase {
   Inv& I;
-  bool half;
-  int sb;           // output sites per block of the three-phase kernels
-  int64_t f32, fh;  // column strides of the FP32 and of the low-storage stacks
-  float4 *R, *X;
+  int storage;
+  int sb;           // output sites per block of the three-phase hop kernels
+  int64_t fh, fr, fx;  // column strides of the p/ap/t/b/y, the r and the x stacks
+  void *R, *X;
   void *P, *AP, *T, *B, *Y;
   Inv* colinv[qdwf::kMaxRhs];  // the inverter (mass) of each stack column
-  ThreePhase(Inv& inv, int ncol, bool h, int sb_ = 0) : I(inv), half(h), sb(sb_ ? sb_ : inv.tp_sb)
+  ThreePhase(Inv& inv, int ncol, int storage_, int sb_ = 0)
+      : I(inv), storage(storage_), sb(sb_ ? sb_ : inv.tp_sb)
   {
     if (!qdwf::three_phase_ok(I.ls, sb)) sb = 32;
     if (!qdwf::three_phase_ok(I.ls, sb)) throw ArgError("the three-phase kernels need 6 <= ls <= 16, ls even");
+    if (storage && fieldh_units(I) * 16 >= (1ll << 32))
+      throw ArgError("FP16 spinor storage needs V/2 * Ls * 52 bytes below 4 GB per field");
     for (auto*& p : colinv) p = &I;
-    f32 = n32(I);
-    fh = half ? fieldh_units(I) : f32;
-    R = MV(I, M_R, ncol);
-    X = MV(I, M_X, ncol);
-    if (half) {
+    fh = half() ? fieldh_units(I) : n32(I);
+    fr = rxh() ? fieldh_units(I) : n32(I);
+    fx = xh() ? fieldh_units(I) : n32(I);
+    R = rxh() ? (void*)MH(I, H_R, ncol) : (void*)MV(I, M_R, ncol);
+    X = xh() ? (void*)MH(I, H_X, ncol) : (void*)MV(I, M_X, ncol);
+    if (half()) {
       P = MH(I, H_P, ncol);
       AP = MH(I, H_AP, ncol);
       T = MH(I, H_T, ncol);
@@ -928,9 +935,32 @@ struct ThreePhase {
       Y = MV(I, M_Y, ncol);
     }
   }
-  size_t slot_bytes() const { return half ? sizeof(uint4) : sizeof(float4); }
+  bool half() const { return storage >= 1; }
+  bool rxh() const { return storage == 2; }
+  bool xh() const { return storage == 2; }
+  size_t slot_bytes() const { return half() ? sizeof(uint4) : sizeof(float4); }
+  size_t rslot_bytes() const { return rxh() ? sizeof(uint4) : sizeof(float4); }
+  size_t xslot_bytes() const { return xh() ? sizeof(uint4) : sizeof(float4); }
   void* at(void* base, int c) const { return (void*)((char*)base + (size_t)c * fh * slot_bytes()); }
+  void* rat(void* base, int c) const { return (void*)((char*)base + (size_t)c * fr * rslot_bytes()); }
+  void* xat(int c) const { return (void*)((char*)X + (size_t)c * fx * xslot_bytes()); }
   int64_t n5() const { return (int64_t)I.geo.stride * I.ls; }
+  // column c of r = scale * b64
+  void set_r(const double2* b64, int c, double scale)
+  {
+    if (rxh()) qdwf::launch_d2h(b64, (uint4*)rat(R, c), I.ls, n5(), scale, I.stream);
+    else qdwf::launch_d2f(b64, (float4*)rat(R, c), I.ls, I.geo.stride, I.stream, scale);
+    check_launch();
+  }
+  // x64 += scale * column c of x
+  void add_x(double2* x64, int c, double scale)
+  {
+    if (xh()) qdwf::launch_h2d((const uint4*)xat(c), x64, I.ls, n5(), 1, scale, I.stream);
+    else qdwf::launch_f2d((const float4*)xat(c), x64, I.ls, I.geo.stride, 1, I.stream, scale);
+    check_launch();
+  }
+  void zero_x(int c0, int n) { QDWF_HIP(hipMemsetAsync(xat(c0), 0, (size_t)fx * xslot_bytes() * n, I.stream)); }
+  void zero_p(int n) { QDWF_HIP(hipMemsetAsync(P, 0, (size_t)fh * slot_bytes() * n, I.stream)); }
   qdwf::RhsSet colset(const std::vector<int>& act) const
   {
     qdwf::RhsSet cs;
@@ -961,15 +991,16 @@ struct ThreePhase {
       a.beta[i] = i < (int)cols.size() ? beta[i] : 0.0;
     }
     a.cols = colset(cols);
-    a.xr_fstride = f32;
-    a.sb = sb;
+    a.xr_fstride = fr;
+    a.x_fstride = fx;
+    a.sb = 32;
     a.mat = nullptr;
     a.real_mat = mats(a.cols, MAT_K, a.mats);
     a.x = updx ? X : nullptr;
     a.p = P;
     a.r = R;
     a.t = T;
-    qdwf::launch_cg_step_s(a, half, I.stream);
+    qdwf::launch_cg_step_s(a, storage, I.stream);
     check_launch();
   }
   void hopm5(void* out, const void* in, int eo_out, bool dag, int mat, float alpha, const void* x, float beta,
@@ -990,8 +1021,13 @@ struct ThreePhase {
     a.links = I.l32;
     a.sb = sb;
     a.mat = nullptr;
-    a.real_mat = mats(cs, mat, a.mats);
-    qdwf::launch_hop_m5s(a, dag, half, I.stream);
+    if (mat >= 0) {
+      a.real_mat = mats(cs, mat, a.mats);
+    } else {  // plain hop
+      for (auto*& m : a.mats) m = nullptr;
+      a.real_mat = true;
+    }
+    qdwf::launch_hop_m5s(a, dag, half(), I.stream);
     check_launch();
   }
   // ap = Mpc^+ Mpc p on parity eo given t = K p, as y = p - H K H t,
@@ -999,21 +1035,7 @@ struct ThreePhase {
   void hermop_dot(const qdwf::RhsSet& cs, int eo)
   {
     hopm5(B, T, 3 - eo, false, MAT_K, 1.f, nullptr, 0.f, nullptr, cs);
-    qdwf::HopArgs<float> a;
-    a.cols = cs;
-    a.geo = I.geo;
-    a.ls = I.ls;
-    a.eo_out = eo;
-    a.nchunk = 0;
-    a.alpha = -1.f;
-    a.in = B;
-    a.out = Y;
-    a.x = P;
-    a.beta = Prec<float>::beta(I, 0);
-    a.links = I.l32;
-    if (half) qdwf::launch_hop_h16(a, false, I.stream);
-    else qdwf::launch_hop<float>(a, false, I.hop_spt32, I.stream);
-    check_launch();
+    hopm5(Y, B, eo, false, -1, -1.f, P, 1.f, nullptr, cs);
     hopm5(B, Y, 3 - eo, true, MAT_K_DAG, 1.f, nullptr, 0.f, nullptr, cs);
     hopm5(AP, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, P, cs);
     qdwf::launch_reduce_rows(I.part, (int)qdwf::hop_m5s_blocks(I.geo, sb), cs.n, I.red, I.stream);
@@ -1038,21 +1060,28 @@ struct ThreePhase {
     an.n5 = n5();
     an.ls = I.ls;
     an.cols = cs;
-    an.r_fstride = f32;
+    an.r_fstride = fr;
     an.part = I.part;
     for (int i = 0; i < qdwf::kMaxRhs; ++i) an.alpha[i] = i < cs.n ? alpha[i] : 0.0;
-    const int g = qdwf::launch_axpy_norm_site(an, half, I.stream);
+    const int g = qdwf::launch_axpy_norm_site(an, storage, I.stream);
     check_launch();
     qdwf::launch_reduce_rows(I.part, g, cs.n, I.red, I.stream);
     check_launch();
   }
-  // x_c (FP32) += alpha * v_c for a low-storage stack v
+  // column c of x += alpha * column c of the p-type stack v
   void axpy_to_x(double alpha, void* v, int c)
   {
-    qdwf::launch_axpy_site(alpha, at(v, c), X + c * f32, I.ls, n5(), half, I.stream);
+    qdwf::launch_axpy_site(alpha, at(v, c), xat(c), I.ls, n5(), storage, I.stream);
     check_launch();
   }
 };
+
+// Storage mode of the three-phase solver for this inverter's solver.
+int storage_of(const Inv& I)
+{
+  if (I.solver != QLAT_DWF_SOLVER_MIXED_FP16) return 0;
+  return I.fp16_mode;
+}
 
 void read_red(Inv& I, int n, double* out)
 {
@@ -1068,13 +1097,13 @@ void read_red(Inv& I, int n, double* out)
 // true residual.  b64[c], x64[c]: FP64 source and solution (x64 is
 // overwritten).  Returns the summed iterations.
 long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64, double stop_rsd, long cap,
-                      bool half, int sb = 0, Inv* const* colinv = nullptr, long* col_iters = nullptr)
+                      int storage, int sb = 0, Inv* const* colinv = nullptr, long* col_iters = nullptr)
 {
   // Each column is solved for b / |b| (the low-precision vectors stay O(1)
   // whatever the source's scale); rr, targets and alpha are in those units.
   // Column c's operator is that of colinv[c] (default I); all of them share
   // I's stream, lattice and gauge field.
-  ThreePhase tp(I, ncol, half, sb);
+  ThreePhase tp(I, ncol, storage, sb);
   if (colinv)
     for (int c = 0; c < ncol; ++c) tp.colinv[c] = colinv[c];
   const int64_t nd = n64(I);
@@ -1089,13 +1118,12 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
     bn[c] = std::sqrt(b2[c]);
     zero64(I, x64[c]);
     if (b2[c] == 0.0) continue;  // x = 0 exactly
-    qdwf::launch_d2f(b64[c], tp.R + c * tp.f32, I.ls, I.geo.stride, I.stream, 1.0 / bn[c]);  // x = 0: r = b
-    check_launch();
+    tp.set_r(b64[c], c, 1.0 / bn[c]);  // x = 0: r = b
     rr[c] = maxrr[c] = 1.0;
     if (rr[c] > target) active.push_back(c);
   }
-  QDWF_HIP(hipMemsetAsync(tp.P, 0, (size_t)tp.fh * tp.slot_bytes() * ncol, I.stream));
-  QDWF_HIP(hipMemsetAsync(tp.X, 0, field32_bytes(I) * ncol, I.stream));
+  tp.zero_p(ncol);
+  tp.zero_x(0, ncol);
   const std::vector<double> zero(qdwf::kMaxRhs, 0.0);
   if (!active.empty()) tp.step(active, zero.data(), zero.data(), false);  // p = r, t = K p
   const double d2 = I.delta * I.delta;
@@ -1124,15 +1152,13 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
       const int c = active[i];
       iters[c] += 1;
       if (!std::isfinite(rr_new[i]) || !std::isfinite(alpha[i]))
-        throw std::runtime_error(half ? "CG breakdown: non-finite residual (FP16 storage, multi)"
-                                      : "CG breakdown: non-finite residual (FP32 three-phase, multi)");
+        throw std::runtime_error(storage ? "CG breakdown: non-finite residual (FP16 storage, multi)"
+                                         : "CG breakdown: non-finite residual (FP32 three-phase, multi)");
       double step_alpha = alpha[i];
       if (rr_new[i] <= d2 * maxrr[c] || rr_new[i] <= target) {
-        float4* xc = tp.X + c * tp.f32;
         tp.axpy_to_x(alpha[i], tp.P, c);
-        qdwf::launch_f2d(xc, x64[c], I.ls, I.geo.stride, 1, I.stream, bn[c]);
-        check_launch();
-        QDWF_HIP(hipMemsetAsync(xc, 0, field32_bytes(I), I.stream));
+        tp.add_x(x64[c], c, bn[c]);
+        tp.zero_x(c, 1);
         const double rr_true = true_residual(c);
         I.stats.reliable_updates += 1;
         if (rr_true <= target) {
@@ -1140,8 +1166,7 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
           if (rel > I.stats.true_residual) I.stats.true_residual = rel;
           continue;  // column retired
         }
-        qdwf::launch_d2f(r64, tp.R + c * tp.f32, I.ls, I.geo.stride, I.stream, 1.0 / bn[c]);
-        check_launch();
+        tp.set_r(r64, c, 1.0 / bn[c]);
         rr_new[i] = rr_true;
         maxrr[c] = rr_true;
         step_alpha = 0.0;
@@ -1157,8 +1182,7 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
   long total = 0;
   for (int c = 0; c < ncol; ++c) total += iters[c];
   for (int c : active) {  // cap reached: fold in what we have
-    qdwf::launch_f2d(tp.X + c * tp.f32, x64[c], I.ls, I.geo.stride, 1, I.stream, bn[c]);
-    check_launch();
+    tp.add_x(x64[c], c, bn[c]);
     const double rel = std::sqrt(true_residual(c));
     if (rel > I.stats.true_residual) I.stats.true_residual = rel;
     total += 1;
@@ -1166,7 +1190,7 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
   }
   if (col_iters)
     for (int c = 0; c < ncol; ++c) col_iters[c] = iters[c];
-  I.stats.spinor_bits = half ? 16 : 32;
+  I.stats.spinor_bits = storage ? 16 : 32;
   return total;
 }
 
@@ -1174,9 +1198,9 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
 // max(ncol, 4)) and sites per block (16 or 32) of the three-phase solver
 // with the least time per column and CG iteration, measured with HIP events
 // on this lattice; cached per storage precision.
-TileChoice autotune_tile(Inv& I, bool half, int ncol)
+TileChoice autotune_tile(Inv& I, int storage, int ncol)
 {
-  TileChoice& t = I.tuned[half ? 1 : 0];
+  TileChoice& t = I.tuned[storage];
   const int wmax = std::min(qdwf::kMaxRhs, std::max(ncol, 4));
   if (t.width && t.width <= wmax) return t;
   const qlat_dwf_solve_stats saved = I.stats;
@@ -1190,13 +1214,13 @@ TileChoice autotune_tile(Inv& I, bool half, int ncol)
     if (!qdwf::three_phase_ok(I.ls, sb)) continue;
     for (int w : {4, 8, 12, 16}) {
       if (w > wmax) continue;
-      ThreePhase tp(I, w, half, sb);
+      ThreePhase tp(I, w, storage, sb);
       std::vector<int> cols;
       for (int c = 0; c < w; ++c) cols.push_back(c);
       const size_t lb = (size_t)tp.fh * tp.slot_bytes() * w;
       for (void* f : {tp.P, tp.T}) QDWF_HIP(hipMemsetAsync(f, 0x3f, lb, I.stream));
-      QDWF_HIP(hipMemsetAsync(tp.R, 0x3f, field32_bytes(I) * w, I.stream));
-      QDWF_HIP(hipMemsetAsync(tp.X, 0, field32_bytes(I) * w, I.stream));
+      QDWF_HIP(hipMemsetAsync(tp.R, 0x3f, (size_t)tp.fr * tp.rslot_bytes() * w, I.stream));
+      tp.zero_x(0, w);
       tp.iteration(cols);  // warm-up
       QDWF_HIP(hipEventRecord(e0, I.stream));
       for (int k = 0; k < 2; ++k) tp.iteration(cols);
@@ -1220,10 +1244,10 @@ TileChoice autotune_tile(Inv& I, bool half, int ncol)
 
 // Tile used by the three-phase solver for ncol columns: QLAT_DWF_RHS_TILE
 // (with QLAT_DWF_TP_SB) when set, else the autotuner's choice.
-TileChoice tile_for(Inv& I, bool half, int ncol)
+TileChoice tile_for(Inv& I, int storage, int ncol)
 {
   if (I.rhs_tile > 0) return TileChoice{I.rhs_tile, I.tp_sb};
-  return autotune_tile(I, half, ncol);
+  return autotune_tile(I, storage, ncol);
 }
 
 bool use_three_phase(const Inv& I)
@@ -1304,7 +1328,7 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
     const long cap = I.max_num_iter * I.max_cycle;
     const double saved = I.stats.true_residual;
     I.stats.true_residual = 0.0;
-    total_iter = cg_mixed_multi_s(I, 1, &in_o_p, &out_o_p, I.stop_rsd, cap, I.solver == QLAT_DWF_SOLVER_MIXED_FP16);
+    total_iter = cg_mixed_multi_s(I, 1, &in_o_p, &out_o_p, I.stop_rsd, cap, storage_of(I));
     true_rel = I.stats.true_residual;
     I.stats.true_residual = saved;
     I.stats.cycles += 1;
@@ -1486,13 +1510,23 @@ double prepare_column(Inv& I, const double2* in_e, const double2* in_o, double2*
   }
   const double q = dev_norm2(I, dm_e) + dev_norm2(I, dm_o);
   double2* tmp = W(I, W_TMP);
-  m5<double>(I, oep, dm_e, MAT_MEE_INV);
+  m5<double>(I, oep, dm_e, MAT_MEE_INV);  // out_e' = M_ee^-1 (D_- in)_e
   op_m_e_o<double>(I, tmp, oep, 2);
   copy64(I, b, dm_o);
   qdwf::launch_axpy<double2>(-1.0, tmp, b, n64(I), I.stream);
   check_launch();
   op_mpcdag<double>(I, b, b, 1);
   return q;
+}
+
+// out_e' = M_ee^-1 (D_- in)_e of set_odd_prec_field_sym2 alone (recomputed at
+// restore time instead of being kept per column)
+void out_e_p_of(Inv& I, const double2* in_e, const double2* in_o, double2* oep)
+{
+  double2* dm_e = W(I, W_DM_E);
+  if (I.dminus) op_d_minus(I, dm_e, W(I, W_DM_O), in_e, in_o);
+  else copy64(I, dm_e, in_e);
+  m5<double>(I, oep, dm_e, MAT_MEE_INV);
 }
 
 // restore_field_from_odd_prec_sym2 (dslash.h:419-438)
@@ -1520,9 +1554,9 @@ long solve_columns(Inv& I, int n, Inv* const* colinv, Load&& load, Store&& store
   bool several = false;
   for (int j = 0; j < n; ++j) several = several || colinv[j] != &I;
   const bool three = use_three_phase(I) || (several && qdwf::three_phase_ok(I.ls));
-  const bool half = I.solver == QLAT_DWF_SOLVER_MIXED_FP16;
+  const int storage = storage_of(I);
   TileChoice tc{I.rhs_tile > 0 ? I.rhs_tile : 12, I.tp_sb};
-  if (three) tc = tile_for(I, half, n);
+  if (three) tc = tile_for(I, storage, n);
   tc.width = std::max(1, std::min(tc.width, qdwf::kMaxRhs));
   I.stats.rhs_tile = tc.width;
   long total = 0;
@@ -1537,11 +1571,11 @@ long solve_columns(Inv& I, int n, Inv* const* colinv, Load&& load, Store&& store
       load(c0 + j, J, ie, io);
       bs[j] = C64(I, j, 0);
       xs[j] = C64(I, j, 1);
-      const double q = prepare_column(J, ie, io, bs[j], C64(I, j, 2));
+      const double q = prepare_column(J, ie, io, bs[j], W(J, W_OUT_EP));
       if (q == 0.0) zero64(I, bs[j]);  // dm_in == 0: solution zero (dslash.h:477-482)
     }
     if (three) {
-      total += cg_mixed_multi_s(I, w, bs, xs, I.stop_rsd, I.max_num_iter * I.max_cycle, half, tc.sb, colinv + c0,
+      total += cg_mixed_multi_s(I, w, bs, xs, I.stop_rsd, I.max_num_iter * I.max_cycle, storage, tc.sb, colinv + c0,
                                 col_iters ? col_iters + c0 : nullptr);
     } else {
       for (int j = 0; j < w; ++j)
@@ -1555,9 +1589,14 @@ long solve_columns(Inv& I, int n, Inv* const* colinv, Load&& load, Store&& store
     I.stats.cycles += 1;
     for (int j = 0; j < w; ++j) {
       Inv& J = *colinv[c0 + j];
+      double2* ie = W(J, W_IN_E);
+      double2* io = W(J, W_IN_O);
       double2* oe = W(J, W_OUT_E);
       double2* oo = W(J, W_OUT_O);
-      restore_column(J, xs[j], C64(I, j, 2), oe, oo);
+      double2* oep = W(J, W_OUT_EP);
+      load(c0 + j, J, ie, io);
+      out_e_p_of(J, ie, io, oep);
+      restore_column(J, xs[j], oep, oe, oo);
       store(c0 + j, J, oe, oo);
     }
   }
@@ -1867,6 +1906,8 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
                    "every lattice extent must be even and >= 2 (even-odd layout, core.h:232-261)");
       V *= total_site[i];
     }
+    QDWF_REQUIRE(V / 2 < (1ll << 30), "lattice too large for 32-bit site indices");
+    QDWF_REQUIRE((V / 2 + 255) * (int64_t)fa->ls * 12 < (1ll << 32), "lattice x Ls too large for 32-bit field indices");
     QDWF_REQUIRE(fa->ls >= 2 && fa->ls <= 32, "ls must be in [2, 32]");
     QDWF_REQUIRE(fa->cg_diagonal_mee == 2 || fa->cg_diagonal_mee == 0,
                  "only cg_diagonal_mee == 2 is supported (dslash.h:483, :520)");
@@ -1887,6 +1928,10 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     }
     I->geo.vh = V / 2;
     I->geo.stride = (I->geo.vh + qdwf::kStrideAlign - 1) / qdwf::kStrideAlign * qdwf::kStrideAlign;
+    I->geo.dxh = qdwf::make_fastdiv((unsigned)I->geo.xh);
+    I->geo.dby = qdwf::make_fastdiv((unsigned)I->geo.by);
+    I->geo.dnz = qdwf::make_fastdiv((unsigned)total_site[2]);
+    I->geo.dnt = qdwf::make_fastdiv((unsigned)total_site[3]);
     I->ls = fa->ls;
     I->mass = fa->mass;
     I->m5 = fa->m5;
@@ -1918,6 +1963,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_THREE_PHASE")) I->three_phase32 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_RHS_TILE")) I->rhs_tile = std::max(0, std::min(qdwf::kMaxRhs, std::atoi(e)));
     if (const char* e = std::getenv("QLAT_DWF_TP_SB")) I->tp_sb = std::atoi(e) == 16 ? 16 : 32;
+    if (const char* e = std::getenv("QLAT_DWF_FP16_MODE")) I->fp16_mode = std::max(1, std::min(2, std::atoi(e)));
     {
       // partial sums: BLAS grids, one per 64-site block (fused kernels), or
       // one per (256-site block, slot) (s-matrix kernel)
@@ -1977,7 +2023,7 @@ int qlat_dwf_inverter_set_solver(qlat_dwf_inverter* inv, int solver, double reli
                  "unknown solver");
     QDWF_REQUIRE(reliable_delta >= 0 && reliable_delta < 1, "reliable_delta must be in [0, 1)");
     inv->solver = solver;
-    inv->delta = reliable_delta > 0 ? reliable_delta : 0.1;
+    inv->delta = reliable_delta > 0 ? reliable_delta : 0.01;
   });
 }
 
@@ -2197,16 +2243,16 @@ int qlat_dwf_multiply_hermop_sym2_fp32(qlat_dwf_inverter* inv, double* out, cons
     if (use_three_phase(I)) {
       // the inner operator of the three-phase solver: in -> r -> p (storage
       // rounding), t = K p, ap = A p, ap -> FP32 -> FP64
-      ThreePhase tp(I, 1, I.solver == QLAT_DWF_SOLVER_MIXED_FP16);
-      qdwf::launch_d2f(a, tp.R, I.ls, I.geo.stride, I.stream);
-      check_launch();
-      QDWF_HIP(hipMemsetAsync(tp.P, 0, (size_t)tp.fh * tp.slot_bytes(), I.stream));
-      QDWF_HIP(hipMemsetAsync(tp.X, 0, field32_bytes(I), I.stream));
+      ThreePhase tp(I, 1, storage_of(I));
+      tp.set_r(a, 0, 1.0);
+      tp.zero_p(1);
+      tp.zero_x(0, 1);
       const double zero[qdwf::kMaxRhs] = {};
       tp.step({0}, zero, zero, false);
       tp.hermop_dot(tp.colset({0}), eo);
       tp.axpy_to_x(1.0, tp.AP, 0);
-      qdwf::launch_f2d(tp.X, b, I.ls, I.geo.stride, 0, I.stream);
+      zero64(I, b);
+      tp.add_x(b, 0, 1.0);
     } else {
       qdwf::launch_d2f(a, V(I, V_R), I.ls, I.geo.stride, I.stream);
       check_launch();
@@ -2390,16 +2436,17 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
     std::vector<int> cols;
     if (which >= 5) {
       // the tile the solver uses for a 12-column propagator
-      const TileChoice tc = tile_for(I, precision == 16, 12);
+      const int storage = precision == 16 ? I.fp16_mode : 0;
+      const TileChoice tc = tile_for(I, storage, 12);
       const int tile = std::max(1, std::min(tc.width, qdwf::kMaxRhs));
       I.stats.rhs_tile = tile;
-      tp.reset(new ThreePhase(I, tile, precision == 16, tc.sb));
+      tp.reset(new ThreePhase(I, tile, storage, tc.sb));
       for (int c = 0; c < tile; ++c) cols.push_back(c);
       // finite, non-trivial input values
       const size_t lb = (size_t)tp->fh * tp->slot_bytes() * tile;
       for (void* f : {tp->P, tp->T, tp->Y, tp->B}) QDWF_HIP(hipMemsetAsync(f, 0x3f, lb, I.stream));
-      QDWF_HIP(hipMemsetAsync(tp->R, 0x3f, field32_bytes(I) * tile, I.stream));
-      QDWF_HIP(hipMemsetAsync(tp->X, 0, field32_bytes(I) * tile, I.stream));
+      QDWF_HIP(hipMemsetAsync(tp->R, 0x3f, (size_t)tp->fr * tp->rslot_bytes() * tile, I.stream));
+      tp->zero_x(0, tile);
     }
     const std::vector<double> al(qdwf::kMaxRhs, 0.0), be(qdwf::kMaxRhs, 0.5);
     auto run = [&]() {

@@ -84,12 +84,34 @@ __device__ __forceinline__ cplx<T> cneg(cplx<T> a)
 // X/2*Y*Z, which keeps it in the L2 of the XCD that reads it.  by = Y is
 // the reference order.  Layout conversion at the boundary applies the
 // permutation (pack/unpack kernels).
+// Unsigned division n / d for n < 2^31 by a divisor fixed at set-up, as a
+// multiply and shift (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994): with 2^(s-1) < d <= 2^s,
+// m = floor(2^(31+s) / d) + 1 < 2^32 and n / d = (n * m) >> (31 + s); the
+// error term n (m - 2^(31+s)/d) / 2^(31+s) < 2^-s <= 1/d keeps the floor
+// exact.
+struct FastDiv {
+  unsigned d, m, sh;
+};
+inline FastDiv make_fastdiv(unsigned d)
+{
+  unsigned s = 0;
+  while ((1ull << s) < d) ++s;
+  const unsigned long long m = ((1ull << (31 + s)) / d) + 1;
+  return FastDiv{d, (unsigned)m, 31 + s};
+}
+__device__ __forceinline__ unsigned fdiv(unsigned n, const FastDiv& f)
+{
+  return (unsigned)(((unsigned long long)n * f.m) >> f.sh);
+}
+
 struct LatGeo {
   int n[4];       // X Y Z T
   int xh;         // X/2
   int by;         // tile height in y (divides Y)
   int64_t vh;     // V/2
   int64_t stride; // vh rounded up to kStrideAlign
+  FastDiv dxh, dby, dnz, dnt;  // divisions by xh, by, Z, T (set with the extents)
 };
 
 // Full coordinate of the device half-site d on parity eo (2 even, 1 odd).
@@ -99,13 +121,13 @@ __device__ __forceinline__ void half_coord(const LatGeo& g, int64_t d64, int eo,
 {
   const unsigned d = (unsigned)d64;
   const unsigned xh = (unsigned)g.xh, by = (unsigned)g.by, nz = (unsigned)g.n[2], nt = (unsigned)g.n[3];
-  unsigned r = d / xh;
+  unsigned r = fdiv(d, g.dxh);
   const unsigned xr = d - r * xh;
-  unsigned q = r / by;
+  unsigned q = fdiv(r, g.dby);
   const unsigned yi = r - q * by;
-  r = q / nz;
+  r = fdiv(q, g.dnz);
   x[2] = (int)(q - r * nz);
-  q = r / nt;
+  q = fdiv(r, g.dnt);
   x[3] = (int)(r - q * nt);
   x[1] = (int)(q * by + yi);
   const int odd = (eo == 1) ? 1 : 0;
@@ -116,17 +138,19 @@ __device__ __forceinline__ void half_coord(const LatGeo& g, int64_t d64, int eo,
 __device__ __forceinline__ int64_t half_index(const LatGeo& g, const int x[4])
 {
   const unsigned by = (unsigned)g.by;
-  const unsigned y = (unsigned)x[1], yt = y / by, yi = y - yt * by;
+  const unsigned y = (unsigned)x[1], yt = fdiv(y, g.dby), yi = y - yt * by;
   return (int64_t)((unsigned)(x[0] >> 1) +
                    (unsigned)g.xh * (yi + by * ((unsigned)x[2] + (unsigned)g.n[2] * ((unsigned)x[3] + (unsigned)g.n[3] * yt))));
 }
 
 // Slot index of (site h, fifth coordinate s, 16-byte slot) in a blocked
 // spinor field with NS slots per 5D site.
+// 32-bit: the inverter requires every field to have fewer than 2^32 slots.
 template <int NS>
-__device__ __forceinline__ int64_t sidx(int ls, int s, int slot, int64_t h)
+__device__ __forceinline__ unsigned sidx(int ls, int s, int slot, int64_t h)
 {
-  return ((((h >> 6) * ls + s) * NS + slot) << 6) + (h & 63);
+  const unsigned hh = (unsigned)h;
+  return ((((hh >> 6) * (unsigned)ls + (unsigned)s) * NS + (unsigned)slot) << 6) + (hh & 63u);
 }
 
 // Element index of link (p, d, e) of site h in the blocked link layout.
@@ -323,21 +347,38 @@ struct SpinorIO<Half16> {
   typedef uint4 slot_t;
   typedef float real;
   static constexpr int kUnits = 208;  // 16-byte units per (64-site block, s)
-  __device__ __forceinline__ static int64_t base(int ls, int s, int64_t h)
+  __device__ __forceinline__ static unsigned base(int ls, int s, int64_t h)
   {
-    return ((h >> 6) * ls + s) * kUnits;
+    return (((unsigned)h >> 6) * (unsigned)ls + (unsigned)s) * (unsigned)kUnits;
   }
   struct Raw {
     uint4 q[3];
     float sc;
   };
+  // Byte offsets within one field are 32-bit (a field is below 4 GB), so
+  // that a wave-uniform field pointer plus a 32-bit offset addresses it.
+  __device__ __forceinline__ static const uint4* slot_at(const uint4* f, unsigned unit)
+  {
+    return (const uint4*)((const char*)f + unit * 16u);
+  }
+  __device__ __forceinline__ static uint4* slot_at(uint4* f, unsigned unit)
+  {
+    return (uint4*)((char*)f + unit * 16u);
+  }
+  __device__ __forceinline__ static const float* scale_at(const uint4* f, unsigned b, unsigned lane)
+  {
+    return (const float*)((const char*)f + (b + 192u) * 16u + lane * 4u);
+  }
+  __device__ __forceinline__ static float* scale_at(uint4* f, unsigned b, unsigned lane)
+  {
+    return (float*)((char*)f + (b + 192u) * 16u + lane * 4u);
+  }
   __device__ __forceinline__ static void load_raw(const uint4* __restrict__ f, int ls, int s, int64_t h, Raw& r)
   {
-    const uint4* p = f + base(ls, s, h);
-    const int lane = (int)(h & 63);
-    r.sc = ((const float*)(p + 192))[lane];
+    const unsigned b = base(ls, s, h), lane = (unsigned)h & 63u;
+    r.sc = *scale_at(f, b, lane);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) r.q[j] = p[j * 64 + lane];
+    for (int j = 0; j < 3; ++j) r.q[j] = *slot_at(f, b + j * 64u + lane);
   }
   __device__ __forceinline__ static void decode(const Raw& r, cplx<float> v[12])
   {
@@ -352,12 +393,11 @@ struct SpinorIO<Half16> {
   __device__ __forceinline__ static void load(const uint4* __restrict__ f, int ls, int s, int64_t h,
                                               cplx<float> v[12])
   {
-    const uint4* p = f + base(ls, s, h);
-    const int lane = (int)(h & 63);
-    const float sc = ((const float*)(p + 192))[lane];
+    const unsigned b = base(ls, s, h), lane = (unsigned)h & 63u;
+    const float sc = *scale_at(f, b, lane);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const uint4 q = p[j * 64 + lane];
+      const uint4 q = *slot_at(f, b + j * 64u + lane);
       v[4 * j + 0] = unpack_h2(q.x, sc);
       v[4 * j + 1] = unpack_h2(q.y, sc);
       v[4 * j + 2] = unpack_h2(q.z, sc);
@@ -373,8 +413,7 @@ struct SpinorIO<Half16> {
     // sites below 2^-100 are stored as zero: 1/m must stay finite
     if (m < 0x1p-100f) m = 0.f;
     const float inv = m > 0.f ? 1.f / m : 0.f;
-    uint4* p = f + base(ls, s, h);
-    const int lane = (int)(h & 63);
+    const unsigned b = base(ls, s, h), lane = (unsigned)h & 63u;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       uint4 q;
@@ -382,18 +421,19 @@ struct SpinorIO<Half16> {
       q.y = pack_h2(v[4 * j + 1].re * inv, v[4 * j + 1].im * inv);
       q.z = pack_h2(v[4 * j + 2].re * inv, v[4 * j + 2].im * inv);
       q.w = pack_h2(v[4 * j + 3].re * inv, v[4 * j + 3].im * inv);
+      uint4* dst = slot_at(f, b + j * 64u + lane);
       if (NT) {
         const u32x4 t = {q.x, q.y, q.z, q.w};
-        __builtin_nontemporal_store(t, (u32x4*)&p[j * 64 + lane]);
+        __builtin_nontemporal_store(t, (u32x4*)dst);
       } else {
-        p[j * 64 + lane] = q;
+        *dst = q;
       }
       v[4 * j + 0] = unpack_h2(q.x, m);
       v[4 * j + 1] = unpack_h2(q.y, m);
       v[4 * j + 2] = unpack_h2(q.z, m);
       v[4 * j + 3] = unpack_h2(q.w, m);
     }
-    ((float*)(p + 192))[lane] = m;
+    *scale_at(f, b, lane) = m;
   }
   __device__ __forceinline__ static void store_q(uint4* __restrict__ f, int ls, int s, int64_t h, cplx<float> v[12])
   {

@@ -67,7 +67,8 @@ struct CgStepArgs {
   // three-phase variant: column spacing of x and r (FP32 stacks), the
   // s-matrix of each column (columns may belong to different masses) and
   // the sites per block (16 or 32)
-  int64_t xr_fstride;
+  int64_t xr_fstride;  // of r
+  int64_t x_fstride;   // of x
   const cplx<T>* mats[kMaxRhs];
   int sb;
 };
@@ -88,7 +89,8 @@ struct HopM5Args {
   const void* dotp;  // optional (dag xpay form): part[i*nblk + block] = Re <dotp, out>
   double* part;
   RhsSet cols;       // in/out/x/dotp are stacks of fields
-  // three-phase variant: the s-matrix of each column, sites per block (16 or 32)
+  // three-phase variant: the s-matrix of each column (mats[0] null: plain
+  // hop, no s-matrix), sites per block (16 or 32)
   const cplx<float>* mats[kMaxRhs];
   int sb;
 };
@@ -141,9 +143,13 @@ int64_t hop_m5_blocks(const LatGeo& g);
 bool three_phase_ok(int ls, int sb = 32);
 int64_t hop_m5s_blocks(const LatGeo& g, int sb);
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st);
-bool launch_cg_step_s(const CgStepArgs<float>& a, bool half, hipStream_t st);
-int launch_axpy_norm_site(const AxpyNormSiteArgs& a, bool half, hipStream_t st);
-void launch_axpy_site(double alpha, const void* p, float4* x, int ls, int64_t n5, bool half, hipStream_t st);
+// storage: 0 all FP32; 1 p, t (ap) FP16 and x, r FP32; 2 all FP16
+bool launch_cg_step_s(const CgStepArgs<float>& a, int storage, hipStream_t st);
+int launch_axpy_norm_site(const AxpyNormSiteArgs& a, int storage, hipStream_t st);
+void launch_axpy_site(double alpha, const void* p, void* x, int ls, int64_t n5, int storage, hipStream_t st);
+// FP64 blocked field <-> FP16 column over n5 = stride * ls 5D sites
+void launch_d2h(const double2* in, uint4* out, int ls, int64_t n5, double scale, hipStream_t st);
+void launch_h2d(const uint4* in, double2* out, int ls, int64_t n5, int acc, double scale, hipStream_t st);
 void launch_hop_h16(const HopArgs<float>& a, bool dag, hipStream_t st);
 template <class T>
 void launch_m5(const M5Args<T>& a, hipStream_t st);

@@ -416,11 +416,13 @@ bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st)
 // per-site scale): phase 1 as in k_hop_m5; phase 2 (smat_lds) applies Mat
 // along s in LDS; phase 3, thread (site, s) again, adds beta * x, stores the
 // whole 5D site (so a per-site scale sees all 24 reals) and accumulates the
-// CG dot product from the stored values.
+// CG dot product from the stored values.  SMAT = false skips phase 2 (plain
+// hop, out = alpha H in + beta x; selected by a null mats[0]).
 
 // Thread (site, kp < 6) replaces its slot column hr[.][kp][site] by Mat
 // (per chirality) applied along s.  Every thread of the block calls it; the
-// caller has synchronised after writing hr.
+// caller has synchronised after writing hr.  (Splitting the s range over
+// more threads was measured slower at 24^3x64 Ls 12.)
 template <int SB, int LSMAX, bool REAL>
 __device__ __forceinline__ void smat_lds(float4 (*hr)[6][SB], const cplx<float>* __restrict__ mat, int ls,
                                          int site, int row, bool active)
@@ -471,13 +473,12 @@ __device__ __forceinline__ void store_qs(typename SpinorIO<ST>::slot_t* __restri
   else SpinorIO<ST>::store_stream(f, ls, s, h, v);
 }
 
-template <class ST, int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT>
+template <class ST, int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT, bool SMAT>
 __global__ void __launch_bounds__(SB * LSMAX) __attribute__((amdgpu_waves_per_eu(3)))
 k_hop_m5s(HopM5Args a)
 {
   typedef typename SpinorIO<ST>::slot_t slot_t;
   __shared__ float2 sl[72][SB];
-  __shared__ float4 hr[LSMAX][6][SB];
   const int64_t lbc = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
   const int ncol = a.cols.n;
   const int64_t lb = lbc / ncol;
@@ -487,23 +488,32 @@ k_hop_m5s(HopM5Args a)
   const int pslot = (a.eo_out == 2) ? 0 : 1;
   const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
   for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) sl[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
+  // neighbour indices once per site (row 0), shared by the Ls rows
+  __shared__ int nbs[8][SB];
+  if (threadIdx.x < SB && h0 + threadIdx.x < a.geo.vh) {
+    int x[4];
+    half_coord(a.geo, h0 + threadIdx.x, a.eo_out, x);
+    int64_t nb[8];
+    neighbours(a.geo, x, nb);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) nbs[d][threadIdx.x] = (int)nb[d];
+  }
   __syncthreads();
   const int ls = a.ls;
   const int site = threadIdx.x % SB, row = threadIdx.x / SB;
   const int64_t h = h0 + site;
   const bool active = h < a.geo.vh;
   const int64_t coff = (int64_t)pick(a.cols.rhs, ci) * a.cols.fstride;
+  cplx<float> acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = cmk<float>(0, 0);
   if (active && row < ls) {
-    int x[4];
-    half_coord(a.geo, h, a.eo_out, x);
     int64_t nb[8];
-    neighbours(a.geo, x, nb);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) nb[d] = nbs[d][site];
     const slot_t* __restrict__ in = (const slot_t*)a.in + coff;
     constexpr int SF = DAG ? 1 : -1;
     constexpr int SB_ = -SF;
-    cplx<float> acc[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] = cmk<float>(0, 0);
     // neighbour spinors are loaded in batches of NB directions before any
     // of them is used, so that each wave keeps several gathers in flight
     constexpr int NB = std::is_same<ST, Half16>::value ? 8 : 4;
@@ -534,23 +544,33 @@ k_hop_m5s(HopM5Args a)
     }
 #undef QDWF_LD
 #undef QDWF_CP
-    const float sc = -0.5f * a.alpha;
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-      hr[row][k][site] = make_float4(sc * acc[2 * k].re, sc * acc[2 * k].im, sc * acc[2 * k + 1].re,
-                                     sc * acc[2 * k + 1].im);
   }
-  __syncthreads();
-  smat_lds<SB, LSMAX, REAL>(hr, pick(a.mats, ci), ls, site, row, active);
+  const float sc = -0.5f * a.alpha;
+  cplx<float> v[12];
+  if constexpr (SMAT) {
+    __shared__ float4 hr[LSMAX][6][SB];
+    if (active && row < ls) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        hr[row][k][site] = make_float4(sc * acc[2 * k].re, sc * acc[2 * k].im, sc * acc[2 * k + 1].re,
+                                       sc * acc[2 * k + 1].im);
+    }
+    __syncthreads();
+    smat_lds<SB, LSMAX, REAL>(hr, pick(a.mats, ci), ls, site, row, active);
+    if (active && row < ls) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const float4 q = hr[row][k][site];
+        v[2 * k] = cmk<float>(q.x, q.y);
+        v[2 * k + 1] = cmk<float>(q.z, q.w);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cmk<float>(sc * acc[k].re, sc * acc[k].im);
+  }
   double dot = 0.0;
   if (active && row < ls) {
-    cplx<float> v[12];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const float4 q = hr[row][k][site];
-      v[2 * k] = cmk<float>(q.x, q.y);
-      v[2 * k + 1] = cmk<float>(q.z, q.w);
-    }
     if (XPAY) {
       cplx<float> xv[12];
       SpinorIO<ST>::load((const slot_t*)a.x + coff, ls, row, h, xv);
@@ -586,7 +606,14 @@ static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
   const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB * a.cols.n));
   const dim3 blk(SB * a.ls);
   const bool xpay = a.x != nullptr;
-#define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, D, X, R, DT>), grid, blk, 0, st, a)
+#define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, D, X, R, DT, true>), grid, blk, 0, st, a)
+  if (!a.mats[0]) {  // plain hop: out = alpha H in + beta x
+    if (!dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, false, true, false, false>), grid, blk, 0, st, a);
+    if (!dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, true, true, false, false>), grid, blk, 0, st, a);
+    if (dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, false, true, false, false>), grid, blk, 0, st, a);
+    if (dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, true, true, false, false>), grid, blk, 0, st, a);
+    return;
+  }
   if (a.dotp) {
     if (dag && xpay && a.real_mat) QDWF_L(true, true, true, true);
     else if (dag && xpay) QDWF_L(true, true, false, true);
@@ -632,14 +659,16 @@ bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
   return true;
 }
 
-// CG direction update for p, t in storage ST and x, r in FP32:
+// CG direction update for p, t in storage ST, r in SR and x in SX:
 // x += alpha p (UPDX); p = r + beta p; t = Mat p, in three phases like
 // k_hop_m5s (block = SB sites x Ls, blockIdx.y = column).  p and t are
 // computed from the stored (rounded) p.
-template <class ST, int SB, int LSMAX, bool UPDX, bool REAL>
+template <class ST, class SR, class SX, int SB, int LSMAX, bool UPDX, bool REAL>
 __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
 {
   typedef typename SpinorIO<ST>::slot_t slot_t;
+  typedef typename SpinorIO<SR>::slot_t rslot_t;
+  typedef typename SpinorIO<SX>::slot_t xslot_t;
   __shared__ float4 hr[LSMAX][6][SB];
   const int ci = blockIdx.y;
   const int ls = a.ls;
@@ -648,19 +677,19 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
   const bool active = h < a.geo.vh;
   const int col = pick(a.cols.rhs, ci);
   const int64_t coff = (int64_t)col * a.cols.fstride;
-  const int64_t xoff = (int64_t)col * a.xr_fstride;
+  const int64_t roff = (int64_t)col * a.xr_fstride, xoff = (int64_t)col * a.x_fstride;
   if (active && row < ls) {
     const float al = (float)pick(a.alpha, ci), be = (float)pick(a.beta, ci);
     cplx<float> pv[12], rv[12];
     SpinorIO<ST>::load((const slot_t*)a.p + coff, ls, row, h, pv);
-    SpinorIO<float>::load((const float4*)a.r + xoff, ls, row, h, rv);
+    SpinorIO<SR>::load((const rslot_t*)a.r + roff, ls, row, h, rv);
     if (UPDX) {
       cplx<float> xv[12];
-      float4* x = (float4*)a.x + xoff;
-      SpinorIO<float>::load(x, ls, row, h, xv);
+      xslot_t* x = (xslot_t*)a.x + xoff;
+      SpinorIO<SX>::load(x, ls, row, h, xv);
 #pragma unroll
       for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + al * pv[k].re, xv[k].im + al * pv[k].im);
-      SpinorIO<float>::store(x, ls, row, h, xv);
+      SpinorIO<SX>::store(x, ls, row, h, xv);
     }
 #pragma unroll
     for (int k = 0; k < 12; ++k) pv[k] = cmk<float>(be * pv[k].re + rv[k].re, be * pv[k].im + rv[k].im);
@@ -683,52 +712,50 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
   }
 }
 
-template <class ST, int SB, int LSMAX>
+template <class ST, class SR, class SX, int SB, int LSMAX>
 static void launch_cg_step_s_ls(const CgStepArgs<float>& a, hipStream_t st)
 {
   const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB), a.cols.n);
   const dim3 blk(SB * a.ls);
   const bool updx = a.x != nullptr;
   if (a.real_mat) {
-    if (updx) hipLaunchKernelGGL((k_cg_step_s<ST, SB, LSMAX, true, true>), grid, blk, 0, st, a);
-    else hipLaunchKernelGGL((k_cg_step_s<ST, SB, LSMAX, false, true>), grid, blk, 0, st, a);
+    if (updx) hipLaunchKernelGGL((k_cg_step_s<ST, SR, SX, SB, LSMAX, true, true>), grid, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_cg_step_s<ST, SR, SX, SB, LSMAX, false, true>), grid, blk, 0, st, a);
   } else {
-    if (updx) hipLaunchKernelGGL((k_cg_step_s<ST, SB, LSMAX, true, false>), grid, blk, 0, st, a);
-    else hipLaunchKernelGGL((k_cg_step_s<ST, SB, LSMAX, false, false>), grid, blk, 0, st, a);
+    if (updx) hipLaunchKernelGGL((k_cg_step_s<ST, SR, SX, SB, LSMAX, true, false>), grid, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_cg_step_s<ST, SR, SX, SB, LSMAX, false, false>), grid, blk, 0, st, a);
   }
 }
 
-template <class ST, int SB>
-static void launch_cg_step_s_sb(const CgStepArgs<float>& a, hipStream_t st)
+template <class ST, class SR, class SX>
+static void launch_cg_step_s_st(const CgStepArgs<float>& a, hipStream_t st)
 {
-  if (a.ls <= 8) launch_cg_step_s_ls<ST, SB, 8>(a, st);
-  else if (a.ls <= 12) launch_cg_step_s_ls<ST, SB, 12>(a, st);
-  else launch_cg_step_s_ls<ST, SB, 16>(a, st);
+  if (a.ls <= 8) launch_cg_step_s_ls<ST, SR, SX, 32, 8>(a, st);
+  else if (a.ls <= 12) launch_cg_step_s_ls<ST, SR, SX, 32, 12>(a, st);
+  else launch_cg_step_s_ls<ST, SR, SX, 32, 16>(a, st);
 }
 
-bool launch_cg_step_s(const CgStepArgs<float>& a, bool half, hipStream_t st)
+// storage: 0 all FP32; 1 p, t FP16, r and x FP32; 2 all FP16
+bool launch_cg_step_s(const CgStepArgs<float>& a, int storage, hipStream_t st)
 {
-  if (!three_phase_ok(a.ls, a.sb)) return false;
-  if (half) {
-    if (a.sb == 16) launch_cg_step_s_sb<Half16, 16>(a, st);
-    else launch_cg_step_s_sb<Half16, 32>(a, st);
-  } else {
-    if (a.sb == 16) launch_cg_step_s_sb<float, 16>(a, st);
-    else launch_cg_step_s_sb<float, 32>(a, st);
-  }
+  if (!three_phase_ok(a.ls, 32)) return false;
+  if (storage == 2) launch_cg_step_s_st<Half16, Half16, Half16>(a, st);
+  else if (storage == 1) launch_cg_step_s_st<Half16, float, float>(a, st);
+  else launch_cg_step_s_st<float, float, float>(a, st);
   return true;
 }
 
-// r_i -= alpha_i ap_i over 5D sites (ap in storage ST, r in FP32) with
-// per-column partial |r_i|^2 (part[i*gridDim.x + block]).
-template <class ST>
+// r_i -= alpha_i ap_i over 5D sites (ap in storage ST, r in SR) with
+// per-column partial |r_i|^2 of the stored r (part[i*gridDim.x + block]).
+template <class ST, class SR>
 __global__ void __launch_bounds__(kBlasBlock) k_axpy_norm_site(AxpyNormSiteArgs a)
 {
   typedef typename SpinorIO<ST>::slot_t slot_t;
+  typedef typename SpinorIO<SR>::slot_t rslot_t;
   const int ci = blockIdx.y;
   const int col = pick(a.cols.rhs, ci);
   const slot_t* __restrict__ ap = (const slot_t*)a.ap + (int64_t)col * a.cols.fstride;
-  float4* __restrict__ r = (float4*)a.r + (int64_t)col * a.r_fstride;
+  rslot_t* __restrict__ r = (rslot_t*)a.r + (int64_t)col * a.r_fstride;
   const float al = (float)pick(a.alpha, ci);
   const int64_t per_blk = (int64_t)a.ls * 64;
   double v = 0.0;
@@ -738,13 +765,12 @@ __global__ void __launch_bounds__(kBlasBlock) k_axpy_norm_site(AxpyNormSiteArgs 
     const int64_t h = (hb << 6) + (rem & 63);
     cplx<float> av[12], rv[12];
     SpinorIO<ST>::load(ap, a.ls, s, h, av);
-    SpinorIO<float>::load(r, a.ls, s, h, rv);
+    SpinorIO<SR>::load(r, a.ls, s, h, rv);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      rv[k] = cmk<float>(rv[k].re - al * av[k].re, rv[k].im - al * av[k].im);
-      v += (double)rv[k].re * rv[k].re + (double)rv[k].im * rv[k].im;
-    }
-    SpinorIO<float>::store(r, a.ls, s, h, rv);
+    for (int k = 0; k < 12; ++k) rv[k] = cmk<float>(rv[k].re - al * av[k].re, rv[k].im - al * av[k].im);
+    store_q<SR>(r, a.ls, s, h, rv);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v += (double)rv[k].re * rv[k].re + (double)rv[k].im * rv[k].im;
   }
   __shared__ double sh[kBlasBlock / 64];
 #pragma unroll
@@ -758,23 +784,31 @@ __global__ void __launch_bounds__(kBlasBlock) k_axpy_norm_site(AxpyNormSiteArgs 
   }
 }
 
-int launch_axpy_norm_site(const AxpyNormSiteArgs& a, bool half, hipStream_t st)
+static int site_grid(int64_t n5)
 {
-  int64_t g = (a.n5 + kBlasBlock - 1) / kBlasBlock;
+  int64_t g = (n5 + kBlasBlock - 1) / kBlasBlock;
   if (g > kMaxBlasGrid) g = kMaxBlasGrid;
-  const dim3 grid((unsigned)g, a.cols.n);
-  if (half) hipLaunchKernelGGL(k_axpy_norm_site<Half16>, grid, dim3(kBlasBlock), 0, st, a);
-  else hipLaunchKernelGGL(k_axpy_norm_site<float>, grid, dim3(kBlasBlock), 0, st, a);
   return (int)g;
 }
 
-// x += alpha p over the 5D sites of one field (x FP32, p in storage ST)
-template <class ST>
-__global__ void __launch_bounds__(kBlasBlock) k_axpy_site(float alpha, const void* pp, float4* __restrict__ x,
-                                                          int ls, int64_t n5)
+int launch_axpy_norm_site(const AxpyNormSiteArgs& a, int storage, hipStream_t st)
+{
+  const int g = site_grid(a.n5);
+  const dim3 grid((unsigned)g, a.cols.n);
+  if (storage == 2) hipLaunchKernelGGL((k_axpy_norm_site<Half16, Half16>), grid, dim3(kBlasBlock), 0, st, a);
+  else if (storage == 1) hipLaunchKernelGGL((k_axpy_norm_site<Half16, float>), grid, dim3(kBlasBlock), 0, st, a);
+  else hipLaunchKernelGGL((k_axpy_norm_site<float, float>), grid, dim3(kBlasBlock), 0, st, a);
+  return g;
+}
+
+// x += alpha p over the 5D sites of one field (p in storage ST, x in SX)
+template <class ST, class SX>
+__global__ void __launch_bounds__(kBlasBlock) k_axpy_site(float alpha, const void* pp, void* xx, int ls, int64_t n5)
 {
   typedef typename SpinorIO<ST>::slot_t slot_t;
+  typedef typename SpinorIO<SX>::slot_t xslot_t;
   const slot_t* __restrict__ p = (const slot_t*)pp;
+  xslot_t* __restrict__ x = (xslot_t*)xx;
   const int64_t per_blk = (int64_t)ls * 64;
   for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n5; i += (int64_t)gridDim.x * kBlasBlock) {
     const int64_t hb = i / per_blk, rem = i - hb * per_blk;
@@ -782,19 +816,67 @@ __global__ void __launch_bounds__(kBlasBlock) k_axpy_site(float alpha, const voi
     const int64_t h = (hb << 6) + (rem & 63);
     cplx<float> pv[12], xv[12];
     SpinorIO<ST>::load(p, ls, s, h, pv);
-    SpinorIO<float>::load(x, ls, s, h, xv);
+    SpinorIO<SX>::load(x, ls, s, h, xv);
 #pragma unroll
     for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + alpha * pv[k].re, xv[k].im + alpha * pv[k].im);
-    SpinorIO<float>::store(x, ls, s, h, xv);
+    SpinorIO<SX>::store(x, ls, s, h, xv);
   }
 }
 
-void launch_axpy_site(double alpha, const void* p, float4* x, int ls, int64_t n5, bool half, hipStream_t st)
+void launch_axpy_site(double alpha, const void* p, void* x, int ls, int64_t n5, int storage, hipStream_t st)
 {
-  int64_t g = (n5 + kBlasBlock - 1) / kBlasBlock;
-  if (g > kMaxBlasGrid) g = kMaxBlasGrid;
-  if (half) hipLaunchKernelGGL(k_axpy_site<Half16>, dim3((unsigned)g), dim3(kBlasBlock), 0, st, (float)alpha, p, x, ls, n5);
-  else hipLaunchKernelGGL(k_axpy_site<float>, dim3((unsigned)g), dim3(kBlasBlock), 0, st, (float)alpha, p, x, ls, n5);
+  const dim3 grid((unsigned)site_grid(n5));
+  const float a = (float)alpha;
+  if (storage == 2) hipLaunchKernelGGL((k_axpy_site<Half16, Half16>), grid, dim3(kBlasBlock), 0, st, a, p, x, ls, n5);
+  else if (storage == 1) hipLaunchKernelGGL((k_axpy_site<Half16, float>), grid, dim3(kBlasBlock), 0, st, a, p, x, ls, n5);
+  else hipLaunchKernelGGL((k_axpy_site<float, float>), grid, dim3(kBlasBlock), 0, st, a, p, x, ls, n5);
+}
+
+// FP64 blocked field <-> FP16 (per-site scale) stack column, over 5D sites:
+// out = scale * in (d2h); out64 (+)= scale * in (h2d, acc = 1 adds).
+__global__ void __launch_bounds__(kBlasBlock) k_d2h(const double2* __restrict__ in, uint4* __restrict__ out, int ls,
+                                                    int64_t n5, double scale)
+{
+  const int64_t per_blk = (int64_t)ls * 64;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n5; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t hb = i / per_blk, rem = i - hb * per_blk;
+    const int s = (int)(rem >> 6);
+    const int64_t h = (hb << 6) + (rem & 63);
+    cplx<double> d[12];
+    SpinorIO<double>::load(in, ls, s, h, d);
+    cplx<float> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cmk<float>((float)(scale * d[k].re), (float)(scale * d[k].im));
+    SpinorIO<Half16>::store(out, ls, s, h, v);
+  }
+}
+__global__ void __launch_bounds__(kBlasBlock) k_h2d(const uint4* __restrict__ in, double2* __restrict__ out, int ls,
+                                                    int64_t n5, int acc, double scale)
+{
+  const int64_t per_blk = (int64_t)ls * 64;
+  for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n5; i += (int64_t)gridDim.x * kBlasBlock) {
+    const int64_t hb = i / per_blk, rem = i - hb * per_blk;
+    const int s = (int)(rem >> 6);
+    const int64_t h = (hb << 6) + (rem & 63);
+    cplx<float> v[12];
+    SpinorIO<Half16>::load(in, ls, s, h, v);
+    cplx<double> d[12];
+    if (acc) SpinorIO<double>::load(out, ls, s, h, d);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const double re = scale * v[k].re, im = scale * v[k].im;
+      d[k] = acc ? cmk<double>(d[k].re + re, d[k].im + im) : cmk<double>(re, im);
+    }
+    SpinorIO<double>::store(out, ls, s, h, d);
+  }
+}
+void launch_d2h(const double2* in, uint4* out, int ls, int64_t n5, double scale, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_d2h, dim3((unsigned)site_grid(n5)), dim3(kBlasBlock), 0, st, in, out, ls, n5, scale);
+}
+void launch_h2d(const uint4* in, double2* out, int ls, int64_t n5, int acc, double scale, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_h2d, dim3((unsigned)site_grid(n5)), dim3(kBlasBlock), 0, st, in, out, ls, n5, acc, scale);
 }
 
 // ============================================================ force

@@ -39,7 +39,7 @@ class InverterDomainWall(Inverter):
     per-5D-site scale.
     """
 
-    def __init__(self, *, gf, fa, qtimer=None, solver="fp64", reliable_delta=0.1):
+    def __init__(self, *, gf, fa, qtimer=None, solver="fp64", reliable_delta=0.01):
         assert isinstance(gf, GaugeField)
         assert isinstance(fa, FermionAction)
         L = _lib.load()
@@ -105,7 +105,7 @@ class InverterDomainWall(Inverter):
         s, n, _ = self._params()
         self._set_params(s, n, int(max_mixed_precision_cycle))
 
-    def set_solver(self, solver, reliable_delta=0.1):
+    def set_solver(self, solver, reliable_delta=0.01):
         if solver not in SOLVERS:
             raise ValueError(f"solver must be one of {sorted(SOLVERS)}")
         _lib.check(_lib.load().qlat_dwf_inverter_set_solver(self.cdata, SOLVERS[solver],

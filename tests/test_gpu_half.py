@@ -75,13 +75,15 @@ def test_fp16_storage_zero_and_scale_invariance():
     assert np.array_equal(a * 2.0**-20, b)
 
 
-@pytest.mark.parametrize("solver,three", [("mixed16", "0"), ("mixed", "1")])
-def test_three_phase_solver_reaches_tolerance(monkeypatch, solver, three):
-    """Single right-hand side through the three-phase solver: FP64 true
-    residual <= stop_rsd and the solution as close to the exact one as the
-    FP64 solver's at the same stop_rsd."""
-    print(f"\n[gpu] {solver} three-phase solve 8^3x16 Ls8", flush=True)
+@pytest.mark.parametrize("solver,three,rx", [("mixed16", "0", "0"), ("mixed16", "0", "1"), ("mixed", "1", "0")])
+def test_three_phase_solver_reaches_tolerance(monkeypatch, solver, three, rx):
+    """Single right-hand side through the three-phase solver (FP16 Krylov
+    vectors with FP32 or FP16 r and x, or all FP32): FP64 true residual <=
+    stop_rsd and the solution as close to the exact one as the FP64
+    solver's at the same stop_rsd."""
+    print(f"\n[gpu] {solver} three-phase solve 8^3x16 Ls8 (rx16={rx})", flush=True)
     monkeypatch.setenv("QLAT_DWF_THREE_PHASE", three)
+    monkeypatch.setenv("QLAT_DWF_FP16_MODE", "2" if rx == "1" else "1")
     site = (8, 8, 8, 16)
     U = rand_gauge(site, 0.3, 3)
     geo, inv, orc = make(site, U, mass=0.05, ls=8, solver="fp64")
@@ -130,14 +132,16 @@ def prop_case():
     return site, U, src, exact
 
 
-@pytest.mark.parametrize("solver,tile,three", [("mixed16", "12", "0"), ("mixed16", "5", "0"),
-                                               ("mixed16", "1", "0"), ("mixed", "12", "1")])
-def test_three_phase_multi_rhs_propagator(monkeypatch, prop_case, solver, tile, three):
-    """12 columns as one block (tile 12), in uneven tiles (5+5+2) and one by
-    one: every column reaches the FP64 true residual and matches the oracle;
-    zero columns stay zero."""
+@pytest.mark.parametrize("solver,tile,three,rx", [("mixed16", "12", "0", "0"), ("mixed16", "5", "0", "0"),
+                                                  ("mixed16", "1", "0", "0"), ("mixed", "12", "1", "0"),
+                                                  ("mixed16", "12", "0", "1"), ("mixed16", "0", "0", "1")])
+def test_three_phase_multi_rhs_propagator(monkeypatch, prop_case, solver, tile, three, rx):
+    """12 columns as one block (tile 12), in uneven tiles (5+5+2), one by one
+    and in the autotuned tile (0): every column reaches the FP64 true
+    residual and matches the oracle; zero columns stay zero."""
     monkeypatch.setenv("QLAT_DWF_RHS_TILE", tile)
     monkeypatch.setenv("QLAT_DWF_THREE_PHASE", three)
+    monkeypatch.setenv("QLAT_DWF_FP16_MODE", "2" if rx == "1" else "1")
     site, U, src, exact = prop_case
     geo = q.Geometry(site)
     gf = q.GaugeField(geo)
