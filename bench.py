@@ -178,9 +178,14 @@ def main():
         f"m5 fp32 {m5_32_ms:.3f} ms | hermop fp32 {herm32_ms:.3f} ms fp64 {herm64_ms:.3f} ms")
 
     if args.kernels_only:
+        # plain copy of one FP32 field: the calibration kernel for counter runs
+        copy_ms = q.bench_kernel(inv, 4, 32, args.kernel_iters)
+        copy_bytes = 2 * (V // 2) * ls * 96
+        log(f"copy fp32 {copy_ms:.3f} ms {copy_bytes / (copy_ms * 1e-3) / 1e9:.0f} GB/s ({copy_bytes} bytes)")
         print(json.dumps({"hop_fp32_ms": hop32_ms, "hop_fp32_gbps": gbs32, "hop_fp64_ms": hop64_ms,
                           "hop_fp64_gbps": gbs64, "m5_fp32_ms": m5_32_ms, "hermop_fp32_ms": herm32_ms,
-                          "hermop_fp64_ms": herm64_ms}), flush=True)
+                          "hermop_fp64_ms": herm64_ms, "hop_m5s_ms": hm5s_ms, "hop_m5s_bytes": bm5s,
+                          "rhs_tile": tile, "copy_fp32_ms": copy_ms, "copy_bytes": copy_bytes}), flush=True)
         return
 
     if args.profile_column:

@@ -175,6 +175,7 @@ struct qlat_dwf_inverter {
   bool fused32 = true;  // FP32 hop + s-matrix fusion (QLAT_DWF_FUSED=0 disables)
   bool three_phase32 = false;  // FP32 storage on the three-phase kernels (QLAT_DWF_THREE_PHASE=1)
   int tp_sb = 32;              // sites per block of the three-phase kernels (QLAT_DWF_TP_SB)
+  int site_major = 0;          // three-phase hop block order (QLAT_DWF_SITE_MAJOR)
   int fp16_mode = 1;           // FP16 solver storage mode 1 or 2 (ThreePhase; QLAT_DWF_FP16_MODE)
   TileChoice tuned[4] = {};    // autotuned tile per storage mode of the three-phase solver
   bool real_mat[16] = {};  // s-matrix MatId has only real entries (Moebius)
@@ -428,7 +429,7 @@ bool hop_m5_32(Inv& I, float4* out, const float4* in, int eo_out, bool dag, int 
 {
   if (!I.fused32) return false;
   if (dotp && !(dag && x)) return false;
-  qdwf::HopM5Args a;
+  qdwf::HopM5Args a{};
   a.dotp = dotp;
   a.part = I.part;
   a.cols = cols ? *cols : one_col();
@@ -1006,7 +1007,7 @@ struct ThreePhase {
   void hopm5(void* out, const void* in, int eo_out, bool dag, int mat, float alpha, const void* x, float beta,
              const void* dotp, const qdwf::RhsSet& cs)
   {
-    qdwf::HopM5Args a;
+    qdwf::HopM5Args a{};
     a.dotp = dotp;
     a.part = I.part;
     a.cols = cs;
@@ -1020,6 +1021,7 @@ struct ThreePhase {
     a.x = x;
     a.links = I.l32;
     a.sb = sb;
+    a.site_major = I.site_major;
     a.mat = nullptr;
     if (mat >= 0) {
       a.real_mat = mats(cs, mat, a.mats);
@@ -1963,6 +1965,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_THREE_PHASE")) I->three_phase32 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_RHS_TILE")) I->rhs_tile = std::max(0, std::min(qdwf::kMaxRhs, std::atoi(e)));
     if (const char* e = std::getenv("QLAT_DWF_TP_SB")) I->tp_sb = std::atoi(e) == 16 ? 16 : 32;
+    if (const char* e = std::getenv("QLAT_DWF_SITE_MAJOR")) I->site_major = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_FP16_MODE")) I->fp16_mode = std::max(1, std::min(2, std::atoi(e)));
     {
       // partial sums: BLAS grids, one per 64-site block (fused kernels), or

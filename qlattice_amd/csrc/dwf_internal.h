@@ -90,9 +90,10 @@ struct HopM5Args {
   double* part;
   RhsSet cols;       // in/out/x/dotp are stacks of fields
   // three-phase variant: the s-matrix of each column (mats[0] null: plain
-  // hop, no s-matrix), sites per block (16 or 32)
+  // hop, no s-matrix), sites per block (16 or 32), block order
   const cplx<float>* mats[kMaxRhs];
   int sb;
+  int site_major;
 };
 
 // r_i -= alpha_i ap_i with per-column partial |r_i|^2 (part[i*nblk + block])

@@ -481,9 +481,11 @@ k_hop_m5s(HopM5Args a)
   __shared__ float2 sl[72][SB];
   const int64_t lbc = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
   const int ncol = a.cols.n;
-  const int64_t lb = lbc / ncol;
-  const int ci = (int)(lbc - lb * ncol);
   const int64_t nblk = (int64_t)gridDim.x / ncol;
+  // column-fastest (the columns of a site block back to back: link reuse)
+  // or site-fastest (one column's sites back to back: neighbour reuse)
+  const int64_t lb = a.site_major ? lbc % nblk : lbc / ncol;
+  const int ci = a.site_major ? (int)(lbc / nblk) : (int)(lbc - lb * ncol);
   const int64_t h0 = lb * SB;
   const int pslot = (a.eo_out == 2) ? 0 : 1;
   const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
