@@ -191,6 +191,7 @@ struct qlat_dwf_inverter {
   double2* stage[2] = {nullptr, nullptr};
   size_t stage_bytes = 0;
   double* part = nullptr;
+  int64_t part_cap = 0;      // doubles in part
   double* red = nullptr;     // device reduction results
   double* red_host = nullptr;  // pinned
 
@@ -546,7 +547,7 @@ template <class T>
 void cg_step(Inv& I, typename Prec<T>::slot* x, typename Prec<T>::slot* p, const typename Prec<T>::slot* r,
              typename Prec<T>::slot* t, const double* alpha, const double* beta, const qdwf::RhsSet& cols)
 {
-  qdwf::CgStepArgs<T> a;
+  qdwf::CgStepArgs<T> a{};
   a.geo = I.geo;
   a.ls = I.ls;
   for (int i = 0; i < qdwf::kMaxRhs; ++i) {
@@ -558,7 +559,7 @@ void cg_step(Inv& I, typename Prec<T>::slot* x, typename Prec<T>::slot* p, const
   a.real_mat = I.real_mat[MAT_K];
   a.x = x;
   a.p = p;
-  a.r = r;
+  a.r = const_cast<typename Prec<T>::slot*>(r);  // read only in this form
   a.t = t;
   qdwf::launch_cg_step<T>(a, I.stream);
   check_launch();
@@ -981,16 +982,23 @@ struct ThreePhase {
     }
     return real;
   }
-  // x += alpha p (updx); p = r + beta p; t = K p
-  void step(const std::vector<int>& cols, const double* alpha, const double* beta, bool updx)
+  // The merged CG form (r in FP32): the hermop pass also reduces Re <r, Ap>
+  // and |Ap|^2, so |r - alpha Ap|^2 is known before r is updated, and the
+  // update of r moves into the CG step.
+  bool merged() const { return !rxh(); }
+  // [r -= alpha_r ap (alpha_r not null)]; x += alpha p (updx); p = r + beta p; t = K p
+  void step(const std::vector<int>& cols, const double* alpha, const double* beta, bool updx,
+            const double* alpha_r = nullptr)
   {
-    qdwf::CgStepArgs<float> a;
+    qdwf::CgStepArgs<float> a{};
     a.geo = I.geo;
     a.ls = I.ls;
     for (int i = 0; i < qdwf::kMaxRhs; ++i) {
       a.alpha[i] = i < (int)cols.size() ? alpha[i] : 0.0;
       a.beta[i] = i < (int)cols.size() ? beta[i] : 0.0;
+      a.alpha_r[i] = (alpha_r && i < (int)cols.size()) ? alpha_r[i] : 0.0;
     }
+    a.ap = alpha_r ? AP : nullptr;
     a.cols = colset(cols);
     a.xr_fstride = fr;
     a.x_fstride = fx;
@@ -1005,10 +1013,12 @@ struct ThreePhase {
     check_launch();
   }
   void hopm5(void* out, const void* in, int eo_out, bool dag, int mat, float alpha, const void* x, float beta,
-             const void* dotp, const qdwf::RhsSet& cs)
+             const void* dotp, const qdwf::RhsSet& cs, bool dotr = false)
   {
     qdwf::HopM5Args a{};
     a.dotp = dotp;
+    a.dotr = dotr ? R : nullptr;
+    a.dotr_fstride = fr;
     a.part = I.part;
     a.cols = cs;
     a.geo = I.geo;
@@ -1033,14 +1043,18 @@ struct ThreePhase {
     check_launch();
   }
   // ap = Mpc^+ Mpc p on parity eo given t = K p, as y = p - H K H t,
-  // ap = y - K^+ H^+ K^+ H^+ y; Re <p, ap> per column into I.red[i]
+  // ap = y - K^+ H^+ K^+ H^+ y.  I.red[i] = Re <p, ap> per column and, in
+  // the merged form, I.red[n + i] = Re <r, ap>, I.red[2n + i] = |ap|^2.
   void hermop_dot(const qdwf::RhsSet& cs, int eo)
   {
+    const int nv = merged() ? 3 : 1;
+    const int64_t nblk = qdwf::hop_m5s_blocks(I.geo, sb);
+    if (nblk * cs.n * nv > I.part_cap) throw std::runtime_error("partial-sum buffer too small");
     hopm5(B, T, 3 - eo, false, MAT_K, 1.f, nullptr, 0.f, nullptr, cs);
     hopm5(Y, B, eo, false, -1, -1.f, P, 1.f, nullptr, cs);
     hopm5(B, Y, 3 - eo, true, MAT_K_DAG, 1.f, nullptr, 0.f, nullptr, cs);
-    hopm5(AP, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, P, cs);
-    qdwf::launch_reduce_rows(I.part, (int)qdwf::hop_m5s_blocks(I.geo, sb), cs.n, I.red, I.stream);
+    hopm5(AP, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, P, cs, merged());
+    qdwf::launch_reduce_rows(I.part, (int)nblk, nv * cs.n, I.red, I.stream);
     check_launch();
     I.stats.hermop_low += cs.n;
   }
@@ -1050,8 +1064,12 @@ struct ThreePhase {
     const qdwf::RhsSet cs = colset(cols);
     const std::vector<double> a(qdwf::kMaxRhs, 1e-3), b(qdwf::kMaxRhs, 0.5);
     hermop_dot(cs, 1);
-    axpy_norm(cs, a.data());
-    step(cols, a.data(), b.data(), true);
+    if (merged()) {
+      step(cols, a.data(), b.data(), true, a.data());
+    } else {
+      axpy_norm(cs, a.data());
+      step(cols, a.data(), b.data(), true);
+    }
   }
   // r_i -= alpha_i ap_i, |r_i|^2 into I.red[i]
   void axpy_norm(const qdwf::RhsSet& cs, const double* alpha)
@@ -1138,18 +1156,27 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
     check_launch();
     return dev_norm2(I, r64) / b2[c];
   };
-  double pap[qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs];
+  double dots[3 * qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs];
+  const bool merged = tp.merged();
   long it = 0;
   while (!active.empty() && it < cap) {
     const qdwf::RhsSet cs = tp.colset(active);
     tp.hermop_dot(cs, 1);
-    read_red(I, cs.n, pap);
-    for (int i = 0; i < cs.n; ++i) alpha[i] = rr[active[i]] / pap[i];
-    tp.axpy_norm(cs, alpha);
-    read_red(I, cs.n, rr_new);
+    read_red(I, (merged ? 3 : 1) * cs.n, dots);
+    for (int i = 0; i < cs.n; ++i) alpha[i] = rr[active[i]] / dots[i];
+    if (merged) {
+      // |r - alpha Ap|^2 = |r|^2 - 2 alpha Re <r, Ap> + alpha^2 |Ap|^2
+      for (int i = 0; i < cs.n; ++i) {
+        const double a = alpha[i];
+        rr_new[i] = std::max(0.0, rr[active[i]] - 2.0 * a * dots[cs.n + i] + a * a * dots[2 * cs.n + i]);
+      }
+    } else {
+      tp.axpy_norm(cs, alpha);
+      read_red(I, cs.n, rr_new);
+    }
     ++it;
     std::vector<int> keep;
-    std::vector<double> ka, kb;
+    std::vector<double> ka, kb, kr;
     for (int i = 0; i < cs.n; ++i) {
       const int c = active[i];
       iters[c] += 1;
@@ -1176,9 +1203,10 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
       keep.push_back(c);
       ka.push_back(step_alpha);
       kb.push_back(rr_new[i] / rr[c]);
+      kr.push_back(step_alpha);  // r was replaced by the true residual when 0
       rr[c] = rr_new[i];
     }
-    if (!keep.empty()) tp.step(keep, ka.data(), kb.data(), true);
+    if (!keep.empty()) tp.step(keep, ka.data(), kb.data(), true, merged ? kr.data() : nullptr);
     active.swap(keep);
   }
   long total = 0;
@@ -1968,15 +1996,18 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_SITE_MAJOR")) I->site_major = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_FP16_MODE")) I->fp16_mode = std::max(1, std::min(2, std::atoi(e)));
     {
-      // partial sums: BLAS grids, one per 64-site block (fused kernels), or
-      // one per (256-site block, slot) (s-matrix kernel)
+      // partial sums per column: BLAS grids, one per block of the fused
+      // kernels (32 sites) and of the three-phase kernels (16 sites, up to
+      // three sums each), or one per (256-site block, slot) (s-matrix kernel)
       int64_t np = qdwf::kMaxBlasGrid * 4;
       np = std::max<int64_t>(np, (I->geo.vh + 31) / 32 + 16);
+      np = std::max<int64_t>(np, 3 * ((I->geo.vh + 15) / 16) + 16);
       np = std::max<int64_t>(np, (I->geo.vh + qdwf::kM5Block - 1) / qdwf::kM5Block * 12 + 16);
-      I->part = (double*)dev_alloc(np * qdwf::kMaxRhs * sizeof(double));
+      I->part_cap = np * qdwf::kMaxRhs;
+      I->part = (double*)dev_alloc(I->part_cap * sizeof(double));
     }
-    I->red = (double*)dev_alloc(2 * qdwf::kMaxRhs * sizeof(double));
-    QDWF_HIP(hipHostMalloc((void**)&I->red_host, 2 * qdwf::kMaxRhs * sizeof(double), hipHostMallocDefault));
+    I->red = (double*)dev_alloc(4 * qdwf::kMaxRhs * sizeof(double));
+    QDWF_HIP(hipHostMalloc((void**)&I->red_host, 4 * qdwf::kMaxRhs * sizeof(double), hipHostMallocDefault));
     *out = I;
   });
   if (rc != QLAT_DWF_OK && I) {
@@ -2456,7 +2487,7 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
       if (which == 5) {
         tp->hopm5(tp->B, tp->T, 2, false, MAT_K, 1.f, nullptr, 0.f, nullptr, tp->colset(cols));
       } else if (which == 6) {
-        tp->step(cols, al.data(), be.data(), true);
+        tp->step(cols, al.data(), be.data(), true, tp->merged() ? al.data() : nullptr);
       } else if (which == 7) {
         tp->hermop_dot(tp->colset(cols), 1);
       } else if (precision == 64) {

@@ -62,8 +62,11 @@ struct CgStepArgs {
   bool real_mat;
   void* x;
   void* p;
-  const void* r;
+  void* r;
   void* t;
+  // merged form (ap not null): r -= alpha_r ap before p is formed
+  const void* ap;
+  double alpha_r[kMaxRhs];
   // three-phase variant: column spacing of x and r (FP32 stacks), the
   // s-matrix of each column (columns may belong to different masses) and
   // the sites per block (16 or 32)
@@ -89,6 +92,10 @@ struct HopM5Args {
   const void* dotp;  // optional (dag xpay form): part[i*nblk + block] = Re <dotp, out>
   double* part;
   RhsSet cols;       // in/out/x/dotp are stacks of fields
+  // three-phase DOT form with dotr (FP32 stack, dotr_fstride apart): also
+  // part[(n + i)*nblk + block] = Re <dotr, out>, part[(2n + i)*nblk + block] = |out|^2
+  const void* dotr;
+  int64_t dotr_fstride;
   // three-phase variant: the s-matrix of each column (mats[0] null: plain
   // hop, no s-matrix), sites per block (16 or 32), block order
   const cplx<float>* mats[kMaxRhs];

@@ -571,7 +571,7 @@ k_hop_m5s(HopM5Args a)
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = cmk<float>(sc * acc[k].re, sc * acc[k].im);
   }
-  double dot = 0.0;
+  double dot = 0.0, dotr = 0.0, dota = 0.0;
   if (active && row < ls) {
     if (XPAY) {
       cplx<float> xv[12];
@@ -585,19 +585,37 @@ k_hop_m5s(HopM5Args a)
       SpinorIO<ST>::load((const slot_t*)a.dotp + coff, ls, row, h, pv);
 #pragma unroll
       for (int k = 0; k < 12; ++k) dot += (double)pv[k].re * v[k].re + (double)pv[k].im * v[k].im;
+      if (a.dotr) {
+        cplx<float> rv[12];
+        SpinorIO<float>::load((const float4*)a.dotr + (int64_t)pick(a.cols.rhs, ci) * a.dotr_fstride, ls, row, h,
+                              rv);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          dotr += (double)rv[k].re * v[k].re + (double)rv[k].im * v[k].im;
+          dota += (double)v[k].re * v[k].re + (double)v[k].im * v[k].im;
+        }
+      }
     }
   }
   if (DOT) {
-    __shared__ double red[(SB * LSMAX + 63) / 64];
+    __shared__ double red[3][(SB * LSMAX + 63) / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nv = a.dotr ? 3 : 1;
+    double vals[3] = {dot, dotr, dota};
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
-    if (lane == 0) red[wv] = dot;
+    for (int q = 0; q < 3; ++q) {
+      if (q < nv) {
+        double t = vals[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+        if (lane == 0) red[q][wv] = t;
+      }
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < nv) {
       double t = 0.0;
-      for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) t += red[i];
-      a.part[ci * nblk + lb] = t;
+      for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) t += red[threadIdx.x][i];
+      a.part[((int64_t)threadIdx.x * ncol + ci) * nblk + lb] = t;
     }
   }
 }
@@ -662,7 +680,8 @@ bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
 }
 
 // CG direction update for p, t in storage ST, r in SR and x in SX:
-// x += alpha p (UPDX); p = r + beta p; t = Mat p, in three phases like
+// [r -= alpha_r ap (merged form)]; x += alpha p (UPDX); p = r + beta p;
+// t = Mat p, in three phases like
 // k_hop_m5s (block = SB sites x Ls, blockIdx.y = column).  p and t are
 // computed from the stored (rounded) p.
 template <class ST, class SR, class SX, int SB, int LSMAX, bool UPDX, bool REAL>
@@ -685,6 +704,14 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
     cplx<float> pv[12], rv[12];
     SpinorIO<ST>::load((const slot_t*)a.p + coff, ls, row, h, pv);
     SpinorIO<SR>::load((const rslot_t*)a.r + roff, ls, row, h, rv);
+    if (a.ap) {  // merged form: r -= alpha_r ap
+      cplx<float> av[12];
+      SpinorIO<ST>::load((const slot_t*)a.ap + coff, ls, row, h, av);
+      const float ar = (float)pick(a.alpha_r, ci);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) rv[k] = cmk<float>(rv[k].re - ar * av[k].re, rv[k].im - ar * av[k].im);
+      store_q<SR>((rslot_t*)a.r + roff, ls, row, h, rv);
+    }
     if (UPDX) {
       cplx<float> xv[12];
       xslot_t* x = (xslot_t*)a.x + xoff;
