@@ -413,16 +413,18 @@ bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st)
 
 // ============================================== three-phase hop + M5
 // Variant of k_hop_m5 for any spinor storage ST (FP32, or FP16 with a
-// per-site scale): phase 1 as in k_hop_m5; phase 2 (smat_lds) applies Mat
-// along s in LDS; phase 3, thread (site, s) again, adds beta * x, stores the
+// per-site scale).  Phase 1 as in k_hop_m5: thread (site, s) computes
+// (H in)(site, s) and parks it in LDS.  Phase 2: the same thread forms its
+// own output row, sum_t Mat[s][t] (H in)(site, t) for all 12 components,
+// from LDS (the s-matrices are staged there too), so every row of the block
+// works and no second barrier is needed.  Phase 3 adds beta * x, stores the
 // whole 5D site (so a per-site scale sees all 24 reals) and accumulates the
-// CG dot product from the stored values.  SMAT = false skips phase 2 (plain
-// hop, out = alpha H in + beta x; selected by a null mats[0]).
+// CG dot products from the stored values.  SMAT = false skips the LDS round
+// trip (plain hop, out = alpha H in + beta x; selected by a null mats[0]).
 
 // Thread (site, kp < 6) replaces its slot column hr[.][kp][site] by Mat
-// (per chirality) applied along s.  Every thread of the block calls it; the
-// caller has synchronised after writing hr.  (Splitting the s range over
-// more threads was measured slower at 24^3x64 Ls 12.)
+// (per chirality) applied along s (the CG step's t = K p).  Every thread of
+// the block calls it; the caller has synchronised after writing hr.
 template <int SB, int LSMAX, bool REAL>
 __device__ __forceinline__ void smat_lds(float4 (*hr)[6][SB], const cplx<float>* __restrict__ mat, int ls,
                                          int site, int row, bool active)
@@ -551,6 +553,15 @@ k_hop_m5s(HopM5Args a)
   cplx<float> v[12];
   if constexpr (SMAT) {
     __shared__ float4 hr[LSMAX][6][SB];
+    // the block's s-matrices (both chiralities) in LDS
+    __shared__ cplx<float> km[2][LSMAX][LSMAX];
+    {
+      const cplx<float>* __restrict__ M = pick(a.mats, ci);
+      for (int i = threadIdx.x; i < 2 * ls * ls; i += blockDim.x) {
+        const int chi = i / (ls * ls), rem = i - chi * ls * ls, r = rem / ls;
+        km[chi][r][rem - r * ls] = M[i];
+      }
+    }
     if (active && row < ls) {
 #pragma unroll
       for (int k = 0; k < 6; ++k)
@@ -558,13 +569,36 @@ k_hop_m5s(HopM5Args a)
                                        sc * acc[2 * k + 1].im);
     }
     __syncthreads();
-    smat_lds<SB, LSMAX, REAL>(hr, pick(a.mats, ci), ls, site, row, active);
+    // thread (site, s) forms its own output row s for all 12 components
     if (active && row < ls) {
+      float4 o[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) o[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int t = 0; t < LSMAX; ++t) {
+        if (t < ls) {
+          const cplx<float> k0 = km[0][row][t], k1 = km[1][row][t];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            const cplx<float> m = k < 3 ? k0 : k1;
+            const float4 q = hr[t][k][site];
+            o[k].x += m.re * q.x;
+            o[k].y += m.re * q.y;
+            o[k].z += m.re * q.z;
+            o[k].w += m.re * q.w;
+            if (!REAL) {
+              o[k].x -= m.im * q.y;
+              o[k].y += m.im * q.x;
+              o[k].z -= m.im * q.w;
+              o[k].w += m.im * q.z;
+            }
+          }
+        }
+      }
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        const float4 q = hr[row][k][site];
-        v[2 * k] = cmk<float>(q.x, q.y);
-        v[2 * k + 1] = cmk<float>(q.z, q.w);
+        v[2 * k] = cmk<float>(o[k].x, o[k].y);
+        v[2 * k + 1] = cmk<float>(o[k].z, o[k].w);
       }
     }
   } else {
