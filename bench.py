@@ -162,12 +162,13 @@ def main():
     prec_tp = 16 if args.solver == "mixed16" else 32
     hm5s_ms = q.bench_kernel(inv, 5, prec_tp, max(1, args.kernel_iters // 5))
     tile = inv.stats()["rhs_tile"]  # the solver's tile (autotuned unless QLAT_DWF_RHS_TILE is set)
+    sb = inv.stats()["sites_per_block"]
     step_ms = q.bench_kernel(inv, 6, prec_tp, max(1, args.kernel_iters // 5))
     hermtp_ms = q.bench_kernel(inv, 7, prec_tp, max(1, args.kernel_iters // 10))
     site_bytes = 52 if prec_tp == 16 else 96
     bm5s = hop_m5s_bytes(site, ls, tile, site_bytes)
     gbs_m5s = bm5s / (hm5s_ms * 1e-3) / 1e9
-    log(f"three-phase ({prec_tp}-bit storage, {tile} columns): hop+s-matrix {hm5s_ms:.3f} ms "
+    log(f"three-phase ({prec_tp}-bit storage, {tile} columns, {sb} sites/block): hop+s-matrix {hm5s_ms:.3f} ms "
         f"{gbs_m5s:.0f} GB/s | cg step {step_ms:.3f} ms | hermop+dot {hermtp_ms:.3f} ms "
         f"({hermtp_ms / tile:.3f} ms per column)")
     b32 = hop_bytes(site, ls, 4)
@@ -250,7 +251,7 @@ def main():
         cpu = cpu_baseline(cfg, U, iters_ref, args.cpu_hermops)
 
     N = V // 2 * ls
-    tr = read_traffic(f"hop_m5s_{'fp16' if prec_tp == 16 else 'fp32'}_{args.config}_tile{tile}")
+    tr = read_traffic(f"hop_m5s_{'fp16' if prec_tp == 16 else 'fp32'}_{args.config}_tile{tile}_sb{sb}")
     traffic_bytes = None if tr is None else tr["hbm_bytes_per_launch"]
     line = {
         "metric": METRIC,
@@ -285,12 +286,12 @@ def main():
             "hop_fp64_ms": hop64_ms, "hop_fp64_gbps": gbs64,
             "m5_fp32_ms": m5_32_ms, "hermop_fp32_ms": herm32_ms, "hermop_fp64_ms": herm64_ms,
             "hermop_ref_flops_convention": 5500 * ls * V,
-            "three_phase": {"storage_bits": prec_tp, "columns": tile, "hop_m5_ms": hm5s_ms,
+            "three_phase": {"storage_bits": prec_tp, "columns": tile, "sites_per_block": sb, "hop_m5_ms": hm5s_ms,
                             "hop_m5_gbps": gbs_m5s, "cg_step_ms": step_ms, "hermop_dot_ms": hermtp_ms},
         },
         "roofline": {
-            "kernel": f"k_hop_m5s<{'Half16' if prec_tp == 16 else 'float'}> (hop + s-matrix, {tile} columns "
-                      f"per launch, {prec_tp}-bit spinor storage, FP32 links)",
+            "kernel": f"k_hop_m5s<{'Half16' if prec_tp == 16 else 'float'}, {sb}, {ls}> (hop + s-matrix, "
+                      f"{tile} columns per launch, {prec_tp}-bit spinor storage, FP32 links)",
             "bound": "hbm",
             "achieved": gbs_m5s,
             "peak": PEAK_HBM_GBS,

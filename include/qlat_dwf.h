@@ -79,6 +79,7 @@ typedef struct {
   int spinor_bits;        /* storage of the low-precision Krylov vectors (16/32),
                              0 for the FP64 solver                              */
   int rhs_tile;           /* columns per multi-RHS block of the last batched solve */
+  int sites_per_block;    /* output sites per block of its hop kernels            */
 } qlat_dwf_solve_stats;
 
 /* Message of the last failed call on this thread ("" if none). */

@@ -87,6 +87,7 @@ class SolveStatsC(ctypes.Structure):
         ("seconds", ctypes.c_double),
         ("spinor_bits", ctypes.c_int),
         ("rhs_tile", ctypes.c_int),
+        ("sites_per_block", ctypes.c_int),
     ]
 
 

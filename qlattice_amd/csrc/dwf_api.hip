@@ -173,7 +173,8 @@ struct qlat_dwf_inverter {
   // hop launch shapes (launch_hop): LDS-staged links; FP64 with streaming stores
   int hop_spt64 = -102, hop_spt32 = -4;
   bool fused32 = true;  // FP32 hop + s-matrix fusion (QLAT_DWF_FUSED=0 disables)
-  bool three_phase32 = false;  // FP32 storage on the three-phase kernels (QLAT_DWF_THREE_PHASE=1)
+  bool three_phase32 = true;   // FP32 solver on the three-phase kernels (QLAT_DWF_THREE_PHASE=0: fused
+                               // FP32 kernels, single s-matrix pass per hop)
   int tp_sb = 32;              // sites per block of the three-phase kernels (QLAT_DWF_TP_SB)
   int site_major = 0;          // three-phase hop block order (QLAT_DWF_SITE_MAJOR)
   int fp16_mode = 1;           // FP16 solver storage mode 1 or 2 (ThreePhase; QLAT_DWF_FP16_MODE)
@@ -1589,6 +1590,7 @@ long solve_columns(Inv& I, int n, Inv* const* colinv, Load&& load, Store&& store
   if (three) tc = tile_for(I, storage, n);
   tc.width = std::max(1, std::min(tc.width, qdwf::kMaxRhs));
   I.stats.rhs_tile = tc.width;
+  I.stats.sites_per_block = three ? tc.sb : 32;
   long total = 0;
   for (int c0 = 0; c0 < n; c0 += tc.width) {
     const int w = std::min(tc.width, n - c0);
@@ -2474,6 +2476,7 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
       const TileChoice tc = tile_for(I, storage, 12);
       const int tile = std::max(1, std::min(tc.width, qdwf::kMaxRhs));
       I.stats.rhs_tile = tile;
+      I.stats.sites_per_block = tc.sb;
       tp.reset(new ThreePhase(I, tile, storage, tc.sb));
       for (int c = 0; c < tile; ++c) cols.push_back(c);
       // finite, non-trivial input values

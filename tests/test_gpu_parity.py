@@ -232,10 +232,13 @@ def test_config_a_fp64_solve_matches_oracle():
     assert rel(sol.data, ref) < 1e-10
 
 
-def test_mixed_precision_solver_reaches_tolerance():
-    """FP32 inner CG with FP64 reliable updates: FP64 true residual of the
-    odd system <= stop_rsd, and the solution is as close to the exact one as
-    the FP64 solver's at the same stop_rsd."""
+@pytest.mark.parametrize("three", ["0", "1"])
+def test_mixed_precision_solver_reaches_tolerance(monkeypatch, three):
+    """FP32 inner CG with FP64 reliable updates (fused single-RHS kernels, or
+    the three-phase kernels): FP64 true residual of the odd system <=
+    stop_rsd, and the solution is as close to the exact one as the FP64
+    solver's at the same stop_rsd."""
+    monkeypatch.setenv("QLAT_DWF_THREE_PHASE", three)
     print("\n[gpu] mixed solver 8^3x16 Moebius Ls8", flush=True)
     site = (8, 8, 8, 16)
     U = rand_gauge(site, 0.3, 3)
@@ -318,6 +321,7 @@ def test_multi_rhs_mixed_propagator(monkeypatch, prop_case, tile, fused):
     true residual and matches the oracle propagator; zero columns stay zero."""
     monkeypatch.setenv("QLAT_DWF_RHS_TILE", tile)
     monkeypatch.setenv("QLAT_DWF_FUSED", fused)
+    monkeypatch.setenv("QLAT_DWF_THREE_PHASE", "0")  # the FP32 fused / unfused kernels
     site, U, src, exact = prop_case
     geo = q.Geometry(site)
     gf = q.GaugeField(geo)
