@@ -32,6 +32,11 @@ CONFIGS = {
     "B": dict(site=(24, 24, 24, 64), ls=12, sigma=0.3, mass=0.01, m5=1.8, scale=2.0, seed=2024),
     # small configuration for quick local runs (not a bench line)
     "S": dict(site=(8, 8, 8, 16), ls=12, sigma=0.3, mass=0.01, m5=1.8, scale=2.0, seed=2024),
+    # batched multi-RHS: 16 Gaussian sources x 4 masses (its own JSON line)
+    "C": dict(site=(16, 16, 16, 32), ls=8, sigma=0.3, masses=(0.005, 0.01, 0.03, 0.1), nsrc=16, m5=1.8,
+              scale=2.0, seed=3031),
+    # large single-GPU share: 12-column propagator and one force evaluation (its own JSON line)
+    "E": dict(site=(32, 32, 32, 64), ls=16, sigma=0.25, mass=0.004, m5=1.8, scale=2.0, seed=4041),
 }
 
 
@@ -108,6 +113,153 @@ def cpu_baseline(cfg, U, iters_fp64, sample_hermops):
     )
 
 
+def bench_config_c(args):
+    """64 systems (16 seeded Gaussian 5D sources x 4 masses) at 16^3x32 Ls8 in
+    one batched call with the mixed16 solver (autotuned multi-RHS tiles with
+    per-column masses and retirement), then the same systems with the FP64
+    solver one by one; prints one JSON line."""
+    import numpy as np
+    import torch
+
+    import qlattice_amd as q
+    from qlattice_amd import _lib
+    from qlattice_amd.gauge import rand_gauge_links, twist_boundary_at_boundary
+
+    cfg = CONFIGS["C"]
+    site, ls = cfg["site"], cfg["ls"]
+    V = int(np.prod(site))
+    U = rand_gauge_links(site, cfg["sigma"], cfg["seed"])
+    twist_boundary_at_boundary(U, site, -0.5, 3)
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    invs = []
+    for m in cfg["masses"]:
+        inv = q.InverterDomainWall(gf=gf, fa=q.FermionAction(mass=m, ls=ls, m5=cfg["m5"], mobius_scale=cfg["scale"]),
+                                   solver="mixed16", reliable_delta=args.reliable_delta)
+        inv.set_stop_rsd(1e-8)
+        invs.append(inv)
+    rng = np.random.default_rng(cfg["seed"] + 1)
+    srcs = [torch.from_numpy(rng.standard_normal((V, ls, 12)) + 1j * rng.standard_normal((V, ls, 12))).cuda()
+            for _ in range(cfg["nsrc"])]
+    pairs = [(mi, k) for k in range(cfg["nsrc"]) for mi in range(len(invs))]
+    sols = [torch.zeros_like(srcs[0]) for _ in pairs]
+    L = _lib.load()
+    n = len(pairs)
+
+    def run(solver):
+        for inv in invs:
+            inv.set_solver(solver, args.reliable_delta)
+        hv = (ctypes.c_void_p * n)(*[invs[mi].cdata for mi, _ in pairs])
+        sp = (_lib._dp * n)(*[_lib.dev_ptr(srcs[k]) for _, k in pairs])
+        op = (_lib._dp * n)(*[_lib.dev_ptr(x) for x in sols])
+        it = np.zeros(n, dtype=np.int64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(L.qlat_dwf_invert_5d_batch_device(hv, n, op, sp, it.ctypes.data_as(_lib._lp)),
+                   "qlat_dwf_invert_5d_batch_device")
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, it, invs[0].stats()
+
+    log("config C: warm-up batch (tile autotuning)")
+    run("mixed16")
+    t16, it16, st16 = run("mixed16")
+    log(f"mixed16 batch: {t16:.2f} s, tile {st16['rhs_tile']}, iterations per mass "
+        f"{[int(np.mean(it16[[j for j, (mi, _) in enumerate(pairs) if mi == m]])) for m in range(len(invs))]}, "
+        f"true residual {st16['true_residual']:.2e}")
+    x16 = [x.clone() for x in sols]
+    t64, it64, st64 = run("fp64")
+    diff = max((torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item() for a, b in zip(x16, sols))
+    log(f"fp64 one by one: {t64:.2f} s, max rel diff mixed16 vs fp64 {diff:.2e}")
+    line = {
+        "metric": "batched multi-RHS time-to-1e-8, 16^3x32 Ls8, 16 sources x 4 masses",
+        "value": t16, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp32",
+        "data": "synthetic: seeded exp(i*0.3*H) links, antiperiodic t, Gaussian 5D sources",
+        "config": {"workload": "64 systems (16 sources x masses 0.005, 0.01, 0.03, 0.1) in one batched call, "
+                               "Moebius b=1.5 c=0.5, M5=1.8, Ls=8, stop 1e-8 (FP64 true residual)",
+                   "rhs_tile": st16["rhs_tile"], "sites_per_block": st16["sites_per_block"]},
+        "iters_per_mass": {str(cfg["masses"][m]): int(np.mean(it16[[j for j, (mi, _) in enumerate(pairs) if mi == m]]))
+                           for m in range(len(invs))},
+        "fp64_seconds": t64, "speedup_vs_fp64": t64 / t16, "max_rel_diff_vs_fp64": diff,
+        "true_residual": st16["true_residual"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_config_e(args):
+    """32^3x64 Ls16 mass 0.004: 12-column point-source propagator with the
+    mixed16 multi-RHS solver, then one fermion-force evaluation (FP64 CG to
+    1e-10 plus the adjoint kernel) on a seeded pseudofermion; one JSON line."""
+    import numpy as np
+    import torch
+
+    import qlattice_amd as q
+    from qlattice_amd import _lib
+    from qlattice_amd.gauge import rand_gauge_links, twist_boundary_at_boundary
+
+    cfg = CONFIGS["E"]
+    site, ls = cfg["site"], cfg["ls"]
+    V = int(np.prod(site))
+    log("config E: generating gauge field")
+    U = rand_gauge_links(site, cfg["sigma"], cfg["seed"])
+    twist_boundary_at_boundary(U, site, -0.5, 3)
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    fa = q.FermionAction(mass=cfg["mass"], ls=ls, m5=cfg["m5"], mobius_scale=cfg["scale"])
+    inv = q.InverterDomainWall(gf=gf, fa=fa, solver="mixed16", reliable_delta=args.reliable_delta)
+    inv.set_stop_rsd(1e-8)
+    inv.set_max_num_iter(1000)
+    del U, gf
+    src_h = np.zeros((V, 12, 12), dtype=np.complex128)
+    src_h[0] = np.eye(12)
+    src = torch.from_numpy(src_h).cuda()
+    del src_h
+    sol = torch.zeros_like(src)
+    it = ctypes.c_long()
+    L = _lib.load()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.qlat_dwf_invert_prop_device(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src), ctypes.byref(it)),
+               "qlat_dwf_invert_prop_device")
+    torch.cuda.synchronize()
+    t_prop = time.perf_counter() - t0
+    st = inv.stats()
+    log(f"mixed16 propagator: {t_prop:.2f} s, iters {st['total_iter']}, reliable {st['reliable_updates']}, "
+        f"true residual {st['true_residual']:.2e}, tile {st['rhs_tile']}")
+    del src, sol
+    torch.cuda.empty_cache()
+    rng = np.random.default_rng(cfg["seed"] + 2)
+    phi = torch.from_numpy(rng.standard_normal((V // 2, ls, 12)) + 1j * rng.standard_normal((V // 2, ls, 12))).cuda()
+    force = torch.zeros((V, 4, 3, 3), dtype=torch.complex128, device="cuda")
+    S = ctypes.c_double()
+    S_lo = ctypes.c_double()
+    itf = ctypes.c_long()
+    inv.set_solver("fp64")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.qlat_dwf_fermion_force_device(inv.cdata, _lib.dev_ptr(phi), 1e-10, _lib.dev_ptr(force), None,
+                                               ctypes.byref(S), ctypes.byref(S_lo), ctypes.byref(itf)),
+               "qlat_dwf_fermion_force_device")
+    torch.cuda.synchronize()
+    t_force = time.perf_counter() - t0
+    herm = (force + force.conj().transpose(-1, -2)).abs().max().item()
+    trace = force.diagonal(dim1=-2, dim2=-1).sum(-1).abs().max().item()
+    log(f"force: {t_force:.2f} s, CG iters {itf.value}, S {S.value:.6e}, anti-hermiticity {herm:.1e}, trace {trace:.1e}")
+    line = {
+        "metric": "32^3x64 Ls16 single-GPU share: propagator time-to-1e-8 and force time",
+        "value": t_prop, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp32",
+        "data": "synthetic: seeded exp(i*0.25*H) links, antiperiodic t, point sources, Gaussian pseudofermion",
+        "config": {"workload": "12-column point-source propagator, 32^3x64, Ls=16, Moebius b=1.5 c=0.5, M5=1.8, "
+                               "mass=0.004, mixed16 multi-RHS, stop 1e-8; then one fermion force (FP64 CG 1e-10)",
+                   "rhs_tile": st["rhs_tile"]},
+        "propagator": {"seconds": t_prop, "iters": st["total_iter"], "reliable_updates": st["reliable_updates"],
+                       "true_residual": st["true_residual"]},
+        "force": {"seconds": t_force, "cg_iters": itf.value, "action": S.value},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +282,11 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the solve would shard by propagator column or by 4D domain).", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "C":
+        return bench_config_c(args)
+    if args.config == "E":
+        return bench_config_e(args)
 
     import numpy as np
     import torch
