@@ -195,6 +195,13 @@ struct qlat_dwf_inverter {
   int64_t part_cap = 0;      // doubles in part
   double* red = nullptr;     // device reduction results
   double* red_host = nullptr;  // pinned
+  // graph-captured multi-RHS CG: iterations per graph launch (QLAT_DWF_GRAPH,
+  // 0 = one host round trip per iteration) and the device / pinned CG state.
+  // Off by default: measured at most 2.5% faster (8^3x16) and 1-5% slower
+  // from 12^3x24 up, where a host round trip is a small part of an iteration.
+  int graph_iters = 0;
+  qdwf::CgState* cg_dev = nullptr;
+  qdwf::CgState* cg_host = nullptr;
 
   // multi-RHS: FP32 stacks of `mcap` fields and per-column FP64 fields
   int rhs_tile = 0;  // columns per multi-RHS solve (QLAT_DWF_RHS_TILE; 0 = autotuned / 12)
@@ -827,8 +834,7 @@ long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
     check_launch();
     return dev_norm2(I, r64);
   };
-  double pap[qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs], beta[qdwf::kMaxRhs],
-      step_alpha[qdwf::kMaxRhs];
+  double pap[qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs], step_alpha[qdwf::kMaxRhs];
   long it = 0;
   while (!active.empty() && it < cap) {
     const qdwf::RhsSet cs = colset(active);
@@ -911,6 +917,7 @@ struct ThreePhase {
   void *R, *X;
   void *P, *AP, *T, *B, *Y;
   Inv* colinv[qdwf::kMaxRhs];  // the inverter (mass) of each stack column
+  qdwf::CgState* cg = nullptr;  // set while a graph iteration is captured
   ThreePhase(Inv& inv, int ncol, int storage_, int sb_ = 0)
       : I(inv), storage(storage_), sb(sb_ ? sb_ : inv.tp_sb)
   {
@@ -1013,6 +1020,28 @@ struct ThreePhase {
     qdwf::launch_cg_step_s(a, storage, I.stream);
     check_launch();
   }
+  // the merged step with its scalars and column modes read from the device
+  // CG state (graph-captured iterations)
+  void step_dev(const std::vector<int>& cols)
+  {
+    qdwf::CgStepArgs<float> a{};
+    a.geo = I.geo;
+    a.ls = I.ls;
+    a.cg = cg;
+    a.ap = AP;
+    a.cols = colset(cols);
+    a.xr_fstride = fr;
+    a.x_fstride = fx;
+    a.sb = 32;
+    a.mat = nullptr;
+    a.real_mat = mats(a.cols, MAT_K, a.mats);
+    a.x = X;
+    a.p = P;
+    a.r = R;
+    a.t = T;
+    qdwf::launch_cg_step_s(a, storage, I.stream);
+    check_launch();
+  }
   void hopm5(void* out, const void* in, int eo_out, bool dag, int mat, float alpha, const void* x, float beta,
              const void* dotp, const qdwf::RhsSet& cs, bool dotr = false)
   {
@@ -1033,6 +1062,7 @@ struct ThreePhase {
     a.links = I.l32;
     a.sb = sb;
     a.site_major = I.site_major;
+    a.cg = cg;
     a.mat = nullptr;
     if (mat >= 0) {
       a.real_mat = mats(cs, mat, a.mats);
@@ -1160,6 +1190,110 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
   double dots[3 * qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs];
   const bool merged = tp.merged();
   long it = 0;
+  if (merged && I.graph_iters > 0 && !active.empty()) {
+    // The same iteration with its scalars on the device: K iterations
+    // (hermop + reductions, k_cg_scalars, step) are captured once per active
+    // set and replayed as one graph; the host reads the CG state after each
+    // replay.  A column that reaches its reliable-update threshold adds
+    // alpha p to x and is frozen on the device; the host then does exactly
+    // the FP64 update of the loop below.
+    if (!I.cg_dev) {
+      QDWF_HIP(hipMalloc((void**)&I.cg_dev, sizeof(qdwf::CgState)));
+      QDWF_HIP(hipHostMalloc((void**)&I.cg_host, sizeof(qdwf::CgState), hipHostMallocDefault));
+    }
+    qdwf::CgState& hs = *I.cg_host;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> cap_cols;
+    int cap_k = 0;
+    auto drop = [&] {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+    };
+    try {
+      while (!active.empty() && it < cap) {
+        const int n = (int)active.size();
+        hs = qdwf::CgState{};
+        hs.target = target;
+        hs.d2 = d2;
+        for (int i = 0; i < n; ++i) {
+          hs.rr[i] = rr[active[i]];
+          hs.maxrr[i] = maxrr[active[i]];
+        }
+        QDWF_HIP(hipMemcpyAsync(I.cg_dev, &hs, sizeof hs, hipMemcpyHostToDevice, I.stream));
+        const int k = (int)std::min<long>(I.graph_iters, cap - it);
+        if (!exec || cap_cols != active || cap_k != k) {
+          drop();
+          const qdwf::RhsSet cs = tp.colset(active);
+          const qlat_dwf_solve_stats saved = I.stats;
+          QDWF_HIP(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
+          hipGraph_t g = nullptr;
+          try {
+            tp.cg = I.cg_dev;
+            for (int j = 0; j < k; ++j) {
+              tp.hermop_dot(cs, 1);
+              qdwf::launch_cg_scalars(I.cg_dev, I.red, n, I.stream);
+              check_launch();
+              tp.step_dev(active);
+            }
+            tp.cg = nullptr;
+          } catch (...) {
+            tp.cg = nullptr;
+            (void)hipStreamEndCapture(I.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          QDWF_HIP(hipStreamEndCapture(I.stream, &g));
+          const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          QDWF_HIP(e);
+          I.stats = saved;
+          cap_cols = active;
+          cap_k = k;
+        }
+        QDWF_HIP(hipGraphLaunch(exec, I.stream));
+        QDWF_HIP(hipMemcpyAsync(&hs, I.cg_dev, sizeof hs, hipMemcpyDeviceToHost, I.stream));
+        sync(I);
+        it += k;
+        std::vector<int> keep, ru;
+        std::vector<double> ru_beta;
+        for (int i = 0; i < n; ++i) {
+          const int c = active[i];
+          iters[c] += hs.iters[i];
+          I.stats.hermop_low += hs.iters[i];
+          if (hs.bad[i])
+            throw std::runtime_error(storage ? "CG breakdown: non-finite residual (FP16 storage, multi)"
+                                             : "CG breakdown: non-finite residual (FP32 three-phase, multi)");
+          if (hs.mode[i] == 0) {
+            rr[c] = hs.rr[i];
+            keep.push_back(c);
+            continue;
+          }
+          // reliable update (x += alpha p was done by the frozen step)
+          tp.add_x(x64[c], c, bn[c]);
+          tp.zero_x(c, 1);
+          const double rr_true = true_residual(c);
+          I.stats.reliable_updates += 1;
+          if (rr_true <= target) {
+            const double rel = std::sqrt(rr_true);
+            if (rel > I.stats.true_residual) I.stats.true_residual = rel;
+            continue;  // column retired
+          }
+          tp.set_r(r64, c, 1.0 / bn[c]);
+          ru.push_back(c);
+          ru_beta.push_back(rr_true / hs.rr[i]);  // hs.rr: |r|^2 before the frozen iteration
+          rr[c] = maxrr[c] = rr_true;
+          keep.push_back(c);
+        }
+        // p = r + beta p, t = K p for the updated columns (x, r unchanged)
+        if (!ru.empty()) tp.step(ru, zero.data(), ru_beta.data(), true, zero.data());
+        active.swap(keep);
+      }
+    } catch (...) {
+      drop();
+      throw;
+    }
+    drop();
+  }
   while (!active.empty() && it < cap) {
     const qdwf::RhsSet cs = tp.colset(active);
     tp.hermop_dot(cs, 1);
@@ -1700,7 +1834,7 @@ void check_batch(qlat_dwf_inverter* const* invs, int n)
     const Inv& A = *invs[0];
     const Inv& B = *invs[j];
     QDWF_REQUIRE(B.device == A.device, "batched inverters must share one device");
-    QDWF_REQUIRE(B.ls == A.ls && B.geo.by == A.geo.by, "batched inverters must share Ls and the site order");
+    QDWF_REQUIRE(B.ls == A.ls && B.geo.by == A.geo.by && B.geo.bz == A.geo.bz, "batched inverters must share Ls and the site order");
     for (int mu = 0; mu < 4; ++mu) QDWF_REQUIRE(B.geo.n[mu] == A.geo.n[mu], "batched inverters must share the lattice");
   }
   QDWF_HIP(hipSetDevice(invs[0]->device));
@@ -1909,6 +2043,8 @@ void free_all(Inv* I)
   f(I->part);
   f(I->red);
   if (I->red_host) (void)hipHostFree(I->red_host);
+  f(I->cg_dev);
+  if (I->cg_host) (void)hipHostFree(I->cg_host);
   if (I->stream) (void)hipStreamDestroy(I->stream);
   delete I;
 }
@@ -1948,22 +2084,31 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     QDWF_HIP(hipStreamCreate(&I->stream));
     for (int i = 0; i < 4; ++i) I->geo.n[i] = total_site[i];
     I->geo.xh = total_site[0] / 2;
-    // device site order: tiles of `by` rows in y (dwf_common.h); by = Y is
-    // the reference order.  QLAT_DWF_TILE_Y overrides the default.
-    // default: the divisor of Y closest to 6 rows (measured best at 24^3x64)
-    I->geo.by = total_site[1];
-    for (int by = 1; by <= total_site[1]; ++by)
-      if (total_site[1] % by == 0 && std::abs(by - 6) < std::abs(I->geo.by - 6)) I->geo.by = by;
-    if (const char* e = std::getenv("QLAT_DWF_TILE_Y")) {
-      const int by = std::atoi(e);
-      if (by > 0 && total_site[1] % by == 0) I->geo.by = by;
-    }
+    // device site order: tiles of `by` rows in y and `bz` planes in z, each
+    // swept in t (dwf_common.h); by = Y, bz = Z is the reference order.
+    // Defaults (measured at 24^3x64): by = the divisor of Y closest to 6,
+    // bz = Z (z tiles cut L2 misses of the FP16 hop by 20% without making
+    // it faster, and slow the FP32/FP64 hops by ~7%).
+    // QLAT_DWF_TILE_Y / QLAT_DWF_TILE_Z override the defaults.
+    auto tile_of = [](int n, int pref, const char* env) {
+      int b = n;
+      for (int d = 1; d <= n; ++d)
+        if (n % d == 0 && std::abs(d - pref) < std::abs(b - pref)) b = d;
+      if (const char* e = std::getenv(env)) {
+        const int v = std::atoi(e);
+        if (v > 0 && n % v == 0) b = v;
+      }
+      return b;
+    };
+    I->geo.by = tile_of(total_site[1], 6, "QLAT_DWF_TILE_Y");
+    I->geo.bz = tile_of(total_site[2], total_site[2], "QLAT_DWF_TILE_Z");
     I->geo.vh = V / 2;
     I->geo.stride = (I->geo.vh + qdwf::kStrideAlign - 1) / qdwf::kStrideAlign * qdwf::kStrideAlign;
     I->geo.dxh = qdwf::make_fastdiv((unsigned)I->geo.xh);
     I->geo.dby = qdwf::make_fastdiv((unsigned)I->geo.by);
-    I->geo.dnz = qdwf::make_fastdiv((unsigned)total_site[2]);
+    I->geo.dbz = qdwf::make_fastdiv((unsigned)I->geo.bz);
     I->geo.dnt = qdwf::make_fastdiv((unsigned)total_site[3]);
+    I->geo.dnzt = qdwf::make_fastdiv((unsigned)(total_site[2] / I->geo.bz));
     I->ls = fa->ls;
     I->mass = fa->mass;
     I->m5 = fa->m5;
@@ -1997,6 +2142,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_TP_SB")) I->tp_sb = std::atoi(e) == 16 ? 16 : 32;
     if (const char* e = std::getenv("QLAT_DWF_SITE_MAJOR")) I->site_major = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_FP16_MODE")) I->fp16_mode = std::max(1, std::min(2, std::atoi(e)));
+    if (const char* e = std::getenv("QLAT_DWF_GRAPH")) I->graph_iters = std::max(0, std::min(64, std::atoi(e)));
     {
       // partial sums per column: BLAS grids, one per block of the fused
       // kernels (32 sites) and of the three-phase kernels (16 sites, up to

@@ -78,10 +78,12 @@ __device__ __forceinline__ cplx<T> cneg(cplx<T> a)
 // lexicographically with x fastest and a half-site index is full_index/2
 // (qlat core.h:232-261).  On the device the half sites of one parity are
 // ordered as
-//     d = xh + X/2 * (yi + by * (z + Z * (t + T * yt))),  y = yt*by + yi,
-// i.e. the lattice is cut into tiles of `by` rows in y and each tile is
-// swept in t.  A t neighbour is then X/2*by*Z half sites away instead of
-// X/2*Y*Z, which keeps it in the L2 of the XCD that reads it.  by = Y is
+//     d = xh + X/2 * (yi + by * (zi + bz * (t + T * (zt + Z/bz * yt)))),
+//     y = yt*by + yi,  z = zt*bz + zi,
+// i.e. the lattice is cut into tiles of `by` rows in y and `bz` planes in z
+// and each tile is swept in t.  A t neighbour is then X/2*by*bz half sites
+// away instead of X/2*Y*Z (shorter reuse distance in the L2); the y and z
+// neighbours across a tile face are read from far away.  by = Y, bz = Z is
 // the reference order.  Layout conversion at the boundary applies the
 // permutation (pack/unpack kernels).
 // Unsigned division n / d for n < 2^31 by a divisor fixed at set-up, as a
@@ -109,9 +111,10 @@ struct LatGeo {
   int n[4];       // X Y Z T
   int xh;         // X/2
   int by;         // tile height in y (divides Y)
+  int bz;         // tile depth in z (divides Z)
   int64_t vh;     // V/2
   int64_t stride; // vh rounded up to kStrideAlign
-  FastDiv dxh, dby, dnz, dnt;  // divisions by xh, by, Z, T (set with the extents)
+  FastDiv dxh, dby, dbz, dnt, dnzt;  // divisions by xh, by, bz, T, Z/bz (set with the extents)
 };
 
 // Full coordinate of the device half-site d on parity eo (2 even, 1 odd).
@@ -120,16 +123,19 @@ struct LatGeo {
 __device__ __forceinline__ void half_coord(const LatGeo& g, int64_t d64, int eo, int x[4])
 {
   const unsigned d = (unsigned)d64;
-  const unsigned xh = (unsigned)g.xh, by = (unsigned)g.by, nz = (unsigned)g.n[2], nt = (unsigned)g.n[3];
+  const unsigned xh = (unsigned)g.xh, by = (unsigned)g.by, bz = (unsigned)g.bz, nt = (unsigned)g.n[3];
+  const unsigned nzt = (unsigned)(g.n[2] / g.bz);
   unsigned r = fdiv(d, g.dxh);
   const unsigned xr = d - r * xh;
   unsigned q = fdiv(r, g.dby);
   const unsigned yi = r - q * by;
-  r = fdiv(q, g.dnz);
-  x[2] = (int)(q - r * nz);
+  r = fdiv(q, g.dbz);
+  const unsigned zi = q - r * bz;
   q = fdiv(r, g.dnt);
   x[3] = (int)(r - q * nt);
-  x[1] = (int)(q * by + yi);
+  r = fdiv(q, g.dnzt);
+  x[2] = (int)((q - r * nzt) * bz + zi);
+  x[1] = (int)(r * by + yi);
   const int odd = (eo == 1) ? 1 : 0;
   x[0] = 2 * (int)xr + ((x[1] + x[2] + x[3] + odd) & 1);
 }
@@ -137,10 +143,12 @@ __device__ __forceinline__ void half_coord(const LatGeo& g, int64_t d64, int eo,
 // Device half-site index of full coordinate x (0 <= x[mu] < n[mu]).
 __device__ __forceinline__ int64_t half_index(const LatGeo& g, const int x[4])
 {
-  const unsigned by = (unsigned)g.by;
+  const unsigned by = (unsigned)g.by, bz = (unsigned)g.bz;
+  const unsigned nzt = (unsigned)(g.n[2] / g.bz);
   const unsigned y = (unsigned)x[1], yt = fdiv(y, g.dby), yi = y - yt * by;
+  const unsigned z = (unsigned)x[2], zt = fdiv(z, g.dbz), zi = z - zt * bz;
   return (int64_t)((unsigned)(x[0] >> 1) +
-                   (unsigned)g.xh * (yi + by * ((unsigned)x[2] + (unsigned)g.n[2] * ((unsigned)x[3] + (unsigned)g.n[3] * yt))));
+                   (unsigned)g.xh * (yi + by * (zi + bz * ((unsigned)x[3] + (unsigned)g.n[3] * (zt + nzt * yt)))));
 }
 
 // Slot index of (site h, fifth coordinate s, 16-byte slot) in a blocked

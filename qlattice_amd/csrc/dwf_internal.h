@@ -51,6 +51,8 @@ struct M5Args {
   RhsSet cols;        // blockIdx.z = column index i
 };
 
+struct CgState;
+
 // x += alpha p (x optional); p = r + beta p; t = Mat p
 template <class T>
 struct CgStepArgs {
@@ -74,6 +76,31 @@ struct CgStepArgs {
   int64_t x_fstride;   // of x
   const cplx<T>* mats[kMaxRhs];
   int sb;
+  // device-scalar form (graph-captured iterations): when cg is not null the
+  // per-column alpha, alpha_r, beta and mode are read from it instead
+  CgState* cg;
+};
+
+// Device copy of the multi-RHS CG scalars, indexed by column slot i (the
+// i-th column of the launch).  k_cg_scalars turns the hermop reductions
+// into alpha, beta and |r|^2, so a CG iteration needs no host round trip
+// and can be captured in a graph.  mode: 0 iterating; 2 the column reached
+// its reliable-update threshold this iteration (the step only adds alpha p
+// to x); 1 frozen (every kernel skips the column until the host has done
+// the FP64 update and reset the mode).
+struct CgState {
+  double rr[kMaxRhs];       // |r|^2 after the last completed step
+  double maxrr[kMaxRhs];    // |r|^2 at the last reliable update
+  double alpha[kMaxRhs];    // alpha of the last iteration
+  double rr_new[kMaxRhs];   // |r - alpha Ap|^2 of the last iteration
+  double alpha_x[kMaxRhs];  // step coefficients of the next step
+  double alpha_r[kMaxRhs];
+  double beta[kMaxRhs];
+  double target;            // stop_rsd^2 (normalised columns)
+  double d2;                // reliable-update factor delta^2
+  int mode[kMaxRhs];
+  int iters[kMaxRhs];       // iterations done since the host last reset them
+  int bad[kMaxRhs];         // non-finite alpha or residual
 };
 
 // FP32 fused hop + s-matrix: out = alpha * Mat (H in) + beta * x
@@ -101,6 +128,7 @@ struct HopM5Args {
   const cplx<float>* mats[kMaxRhs];
   int sb;
   int site_major;
+  const CgState* cg;  // optional: blocks of columns with cg->mode != 0 do nothing
 };
 
 // r_i -= alpha_i ap_i with per-column partial |r_i|^2 (part[i*nblk + block])
@@ -153,6 +181,9 @@ int64_t hop_m5s_blocks(const LatGeo& g, int sb);
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st);
 // storage: 0 all FP32; 1 p, t (ap) FP16 and x, r FP32; 2 all FP16
 bool launch_cg_step_s(const CgStepArgs<float>& a, int storage, hipStream_t st);
+// alpha, beta, |r|^2 and modes of the n column slots from the merged hermop
+// reductions red[i] = Re <p, Ap>, red[n + i] = Re <r, Ap>, red[2n + i] = |Ap|^2
+void launch_cg_scalars(CgState* cg, const double* red, int n, hipStream_t st);
 int launch_axpy_norm_site(const AxpyNormSiteArgs& a, int storage, hipStream_t st);
 void launch_axpy_site(double alpha, const void* p, void* x, int ls, int64_t n5, int storage, hipStream_t st);
 // FP64 blocked field <-> FP16 column over n5 = stride * ls 5D sites

@@ -481,14 +481,16 @@ k_hop_m5s(HopM5Args a)
 {
   typedef typename SpinorIO<ST>::slot_t slot_t;
   __shared__ float2 sl[72][SB];
-  const int64_t lbc = xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
+  // block numbers are below 2^32: 32-bit divisions
+  const unsigned lbc = (unsigned)xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
   const int ncol = a.cols.n;
-  const int64_t nblk = (int64_t)gridDim.x / ncol;
+  const unsigned nblk = gridDim.x / (unsigned)ncol;
   // column-fastest (the columns of a site block back to back: link reuse)
   // or site-fastest (one column's sites back to back: neighbour reuse)
-  const int64_t lb = a.site_major ? lbc % nblk : lbc / ncol;
-  const int ci = a.site_major ? (int)(lbc / nblk) : (int)(lbc - lb * ncol);
-  const int64_t h0 = lb * SB;
+  const unsigned lb = a.site_major ? lbc % nblk : lbc / (unsigned)ncol;
+  const int ci = a.site_major ? (int)(lbc / nblk) : (int)(lbc - lb * (unsigned)ncol);
+  if (a.cg && a.cg->mode[ci] != 0) return;  // frozen column (the whole block)
+  const int64_t h0 = (int64_t)lb * SB;
   const int pslot = (a.eo_out == 2) ? 0 : 1;
   const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
   for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) sl[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
@@ -557,9 +559,10 @@ k_hop_m5s(HopM5Args a)
     __shared__ cplx<float> km[2][LSMAX][LSMAX];
     {
       const cplx<float>* __restrict__ M = pick(a.mats, ci);
-      for (int i = threadIdx.x; i < 2 * ls * ls; i += blockDim.x) {
-        const int chi = i / (ls * ls), rem = i - chi * ls * ls, r = rem / ls;
-        km[chi][r][rem - r * ls] = M[i];
+      // compile-time divisors: (chi, r, t) of entry i of the LDS copy
+      for (int i = threadIdx.x; i < 2 * LSMAX * LSMAX; i += blockDim.x) {
+        const int chi = i / (LSMAX * LSMAX), r = (i / LSMAX) % LSMAX, t = i % LSMAX;
+        if (r < ls && t < ls) km[chi][r][t] = M[(chi * ls + r) * ls + t];
       }
     }
     if (active && row < ls) {
@@ -733,15 +736,39 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
   const int col = pick(a.cols.rhs, ci);
   const int64_t coff = (int64_t)col * a.cols.fstride;
   const int64_t roff = (int64_t)col * a.xr_fstride, xoff = (int64_t)col * a.x_fstride;
+  // scalars from the kernel arguments, or from the device CG state
+  // (block-uniform: the whole block returns together)
+  float al, be, ar;
+  if (a.cg) {
+    const int mode = a.cg->mode[ci];
+    if (mode == 1) return;
+    al = (float)a.cg->alpha_x[ci];
+    if (mode == 2) {  // reliable-update point: only x += alpha p
+      if (UPDX && active && row < ls) {
+        cplx<float> pv[12], xv[12];
+        SpinorIO<ST>::load((const slot_t*)a.p + coff, ls, row, h, pv);
+        xslot_t* x = (xslot_t*)a.x + xoff;
+        SpinorIO<SX>::load(x, ls, row, h, xv);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + al * pv[k].re, xv[k].im + al * pv[k].im);
+        SpinorIO<SX>::store(x, ls, row, h, xv);
+      }
+      return;
+    }
+    be = (float)a.cg->beta[ci];
+    ar = (float)a.cg->alpha_r[ci];
+  } else {
+    al = (float)pick(a.alpha, ci);
+    be = (float)pick(a.beta, ci);
+    ar = (float)pick(a.alpha_r, ci);
+  }
   if (active && row < ls) {
-    const float al = (float)pick(a.alpha, ci), be = (float)pick(a.beta, ci);
     cplx<float> pv[12], rv[12];
     SpinorIO<ST>::load((const slot_t*)a.p + coff, ls, row, h, pv);
     SpinorIO<SR>::load((const rslot_t*)a.r + roff, ls, row, h, rv);
     if (a.ap) {  // merged form: r -= alpha_r ap
       cplx<float> av[12];
       SpinorIO<ST>::load((const slot_t*)a.ap + coff, ls, row, h, av);
-      const float ar = (float)pick(a.alpha_r, ci);
 #pragma unroll
       for (int k = 0; k < 12; ++k) rv[k] = cmk<float>(rv[k].re - ar * av[k].re, rv[k].im - ar * av[k].im);
       store_q<SR>((rslot_t*)a.r + roff, ls, row, h, rv);
@@ -806,6 +833,47 @@ bool launch_cg_step_s(const CgStepArgs<float>& a, int storage, hipStream_t st)
   else if (storage == 1) launch_cg_step_s_st<Half16, float, float>(a, st);
   else launch_cg_step_s_st<float, float, float>(a, st);
   return true;
+}
+
+// The host's merged-CG scalar recurrence (alpha = |r|^2 / Re <p,Ap>,
+// |r - alpha Ap|^2 = |r|^2 - 2 alpha Re <r,Ap> + alpha^2 |Ap|^2, beta =
+// |r_new|^2 / |r|^2, reliable-update test |r_new|^2 <= delta^2 max|r|^2 or
+// <= target) on the device, one thread per column slot.  No FMA contraction:
+// the operations and their order are the host's, so both give the same
+// doubles.
+__global__ void __launch_bounds__(64) k_cg_scalars(CgState* __restrict__ cg, const double* __restrict__ red, int n)
+{
+#pragma clang fp contract(off)
+  const int i = threadIdx.x;
+  if (i >= n) return;
+  if (cg->mode[i] != 0) {  // reached the threshold last iteration: frozen from now on
+    cg->mode[i] = 1;
+    return;
+  }
+  const double rr = cg->rr[i];
+  const double alpha = rr / red[i];
+  const double rr_new = fmax(0.0, rr - 2.0 * alpha * red[n + i] + alpha * alpha * red[2 * n + i]);
+  cg->alpha[i] = alpha;
+  cg->rr_new[i] = rr_new;
+  cg->iters[i] += 1;
+  if (!isfinite(rr_new) || !isfinite(alpha)) {
+    cg->bad[i] = 1;
+    cg->mode[i] = 1;
+    return;
+  }
+  cg->alpha_x[i] = alpha;
+  if (rr_new <= cg->d2 * cg->maxrr[i] || rr_new <= cg->target) {
+    cg->mode[i] = 2;
+    return;
+  }
+  cg->alpha_r[i] = alpha;
+  cg->beta[i] = rr_new / rr;
+  cg->rr[i] = rr_new;
+}
+
+void launch_cg_scalars(CgState* cg, const double* red, int n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(64), 0, st, cg, red, n);
 }
 
 // r_i -= alpha_i ap_i over 5D sites (ap in storage ST, r in SR) with

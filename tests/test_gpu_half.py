@@ -158,3 +158,41 @@ def test_three_phase_multi_rhs_propagator(monkeypatch, prop_case, solver, tile, 
     assert not np.any(sol.data[:, :, 9:])
     for j in range(9):
         assert rel(sol.data[:, :, j], exact[:, :, j]) < 1e-6
+
+
+@pytest.mark.parametrize("solver", ["mixed16", "mixed"])
+def test_graph_replay_matches_eager_loop(monkeypatch, prop_case, solver):
+    """Graph-captured CG iterations (QLAT_DWF_GRAPH = K iterations per
+    replay, scalars on the device) against the loop with one host round trip
+    per iteration (QLAT_DWF_GRAPH=0): the same iteration and reliable-update
+    counts up to the timing of the FP64 updates, every column at the FP64
+    true residual and on the oracle's solution, replayed many times with the
+    FP64 updates between replays."""
+    print(f"\n[gpu] graph replay vs eager loop ({solver}) 4^3x8 Ls8", flush=True)
+    monkeypatch.setenv("QLAT_DWF_RHS_TILE", "12")
+    monkeypatch.setenv("QLAT_DWF_THREE_PHASE", "1")
+    site, U, src, exact = prop_case
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    fa = q.FermionAction(mass=0.05, ls=8, m5=1.8, mobius_scale=2.0)
+    out = {}
+    for k in ("0", "1", "3", "16"):
+        monkeypatch.setenv("QLAT_DWF_GRAPH", k)
+        inv = q.InverterDomainWall(gf=gf, fa=fa, solver=solver)
+        inv.set_stop_rsd(1e-9)
+        p = q.Prop(geo)
+        p.data = src
+        sol = inv * p
+        out[k] = (sol.data.copy(), inv.stats())
+        st = out[k][1]
+        assert st["true_residual"] <= 1e-9, k
+        assert not np.any(sol.data[:, :, 9:])
+        for j in range(9):
+            assert rel(sol.data[:, :, j], exact[:, :, j]) < 1e-6, (k, j)
+    base_sol, base = out["0"]
+    for k in ("1", "3", "16"):
+        sol, st = out[k]
+        assert abs(st["total_iter"] - base["total_iter"]) <= 0.02 * base["total_iter"], (k, st, base)
+        assert abs(st["reliable_updates"] - base["reliable_updates"]) <= 1, (k, st, base)
+        assert rel(sol, base_sol) < 1e-7, k

@@ -259,12 +259,14 @@ def test_mixed_precision_solver_reaches_tolerance(monkeypatch, three):
 
 
 def test_tiled_device_order_matches_oracle(monkeypatch):
-    """The device site order with y tiles (QLAT_DWF_TILE_Y) is a layout
-    choice only: every operator and the solve must be unchanged."""
+    """The device site order with y and z tiles (QLAT_DWF_TILE_Y,
+    QLAT_DWF_TILE_Z) is a layout choice only: every operator and the solve
+    must be unchanged."""
     site = (4, 8, 4, 8)
     U = rand_gauge(site, 0.4, 31)
-    for by in ("2", "4"):
+    for by, bz in (("2", "2"), ("4", "1"), ("8", "2")):
         monkeypatch.setenv("QLAT_DWF_TILE_Y", by)
+        monkeypatch.setenv("QLAT_DWF_TILE_Z", bz)
         for spt in ("1", "4", "-4", "-104"):
             monkeypatch.setenv("QLAT_DWF_HOP_SPT32", spt)
             monkeypatch.setenv("QLAT_DWF_HOP_SPT64", spt if spt != "4" else "2")
@@ -272,7 +274,7 @@ def test_tiled_device_order_matches_oracle(monkeypatch):
             x = rand_field((geo.total_volume() // 2, 6, 12), 400)
             for name, fn, _ in HALF_OPS:
                 got = fn(None, half(geo, 6, 2, x), inv)
-                assert rel(got.data, orc.apply_half(name, x, 2)) < 1e-12, (by, spt, name)
+                assert rel(got.data, orc.apply_half(name, x, 2)) < 1e-12, (by, bz, spt, name)
             y = rand_field((geo.total_volume(), 6, 12), 401)
             assert rel(q.multiply_m(None, full(geo, 6, y), inv).data, orc.apply_full("m", y)) < 1e-12
             inv.set_solver("mixed")
@@ -281,7 +283,7 @@ def test_tiled_device_order_matches_oracle(monkeypatch):
             sol = inv * src
             ref, _ = orc.invert_4d(src.data)
             assert inv.stats()["true_residual"] <= 1e-8
-            assert rel(sol.data, ref) < 1e-6, (by, spt)
+            assert rel(sol.data, ref) < 1e-6, (by, bz, spt)
 
 
 @pytest.mark.parametrize("fused", ["1", "0"])
