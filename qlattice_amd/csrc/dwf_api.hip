@@ -176,6 +176,7 @@ struct qlat_dwf_inverter {
   bool three_phase32 = true;   // FP32 solver on the three-phase kernels (QLAT_DWF_THREE_PHASE=0: fused
                                // FP32 kernels, single s-matrix pass per hop)
   int tp_sb = 32;              // sites per block of the three-phase kernels (QLAT_DWF_TP_SB)
+  int step_sb = 32;            // sites per block of the three-phase CG step (QLAT_DWF_STEP_SB)
   int site_major = 0;          // three-phase hop block order (QLAT_DWF_SITE_MAJOR)
   int fp16_mode = 1;           // FP16 solver storage mode 1 or 2 (ThreePhase; QLAT_DWF_FP16_MODE)
   TileChoice tuned[4] = {};    // autotuned tile per storage mode of the three-phase solver
@@ -1010,14 +1011,14 @@ struct ThreePhase {
     a.cols = colset(cols);
     a.xr_fstride = fr;
     a.x_fstride = fx;
-    a.sb = 32;
+    a.sb = I.step_sb;
     a.mat = nullptr;
     a.real_mat = mats(a.cols, MAT_K, a.mats);
     a.x = updx ? X : nullptr;
     a.p = P;
     a.r = R;
     a.t = T;
-    qdwf::launch_cg_step_s(a, storage, I.stream);
+    if (!qdwf::launch_cg_step_s(a, storage, I.stream)) throw ArgError("the three-phase CG step needs an even ls in [6, 16]");
     check_launch();
   }
   // the merged step with its scalars and column modes read from the device
@@ -1032,14 +1033,14 @@ struct ThreePhase {
     a.cols = colset(cols);
     a.xr_fstride = fr;
     a.x_fstride = fx;
-    a.sb = 32;
+    a.sb = I.step_sb;
     a.mat = nullptr;
     a.real_mat = mats(a.cols, MAT_K, a.mats);
     a.x = X;
     a.p = P;
     a.r = R;
     a.t = T;
-    qdwf::launch_cg_step_s(a, storage, I.stream);
+    if (!qdwf::launch_cg_step_s(a, storage, I.stream)) throw ArgError("the three-phase CG step needs an even ls in [6, 16]");
     check_launch();
   }
   void hopm5(void* out, const void* in, int eo_out, bool dag, int mat, float alpha, const void* x, float beta,
@@ -2142,6 +2143,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_TP_SB")) I->tp_sb = std::atoi(e) == 16 ? 16 : 32;
     if (const char* e = std::getenv("QLAT_DWF_SITE_MAJOR")) I->site_major = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_FP16_MODE")) I->fp16_mode = std::max(1, std::min(2, std::atoi(e)));
+    if (const char* e = std::getenv("QLAT_DWF_STEP_SB")) I->step_sb = std::atoi(e) == 16 ? 16 : 32;
     if (const char* e = std::getenv("QLAT_DWF_GRAPH")) I->graph_iters = std::max(0, std::min(64, std::atoi(e)));
     {
       // partial sums per column: BLAS grids, one per block of the fused

@@ -422,48 +422,54 @@ bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st)
 // CG dot products from the stored values.  SMAT = false skips the LDS round
 // trip (plain hop, out = alpha H in + beta x; selected by a null mats[0]).
 
-// Thread (site, kp < 6) replaces its slot column hr[.][kp][site] by Mat
-// (per chirality) applied along s (the CG step's t = K p).  Every thread of
-// the block calls it; the caller has synchronised after writing hr.
-template <int SB, int LSMAX, bool REAL>
-__device__ __forceinline__ void smat_lds(float4 (*hr)[6][SB], const cplx<float>* __restrict__ mat, int ls,
-                                         int site, int row, bool active)
+// The block's s-matrices (both chiralities, [2][ls][ls] on the device) into
+// LDS; every thread of the block takes part, the caller synchronises.
+template <int LSMAX>
+__device__ __forceinline__ void stage_km(cplx<float> (*km)[LSMAX][LSMAX], const cplx<float>* __restrict__ M, int ls)
 {
-  const bool mine = active && row < 6;
-  float4 v[LSMAX];
-  if (mine) {
-#pragma unroll
-    for (int t = 0; t < LSMAX; ++t)
-      if (t < ls) v[t] = hr[t][row][site];
+  // compile-time divisors: (chi, r, t) of entry i of the LDS copy
+  for (int i = threadIdx.x; i < 2 * LSMAX * LSMAX; i += blockDim.x) {
+    const int chi = i / (LSMAX * LSMAX), r = (i / LSMAX) % LSMAX, t = i % LSMAX;
+    if (r < ls && t < ls) km[chi][r][t] = M[(chi * ls + r) * ls + t];
   }
-  __syncthreads();
-  if (mine) {
-    const cplx<float>* __restrict__ M = mat + (row < 3 ? 0 : ls * ls);
+}
+
+// Row s of the s-matrix product for one site: v = sum_t K[chi][s][t] h(t),
+// h(t) = hr[t][.][site] (6 float4 = 12 complex; chirality 0 for complex
+// 0..5, 1 for 6..11).  Thread (site, s) forms its own output row.
+template <int SB, int LSMAX, bool REAL>
+__device__ __forceinline__ void smat_row(const float4 (*hr)[6][SB], const cplx<float> (*km)[LSMAX][LSMAX], int ls,
+                                         int site, int s, cplx<float> v[12])
+{
+  float4 o[6];
 #pragma unroll
-    for (int s = 0; s < LSMAX; ++s) {
-      if (s < ls) {
-        float r0 = 0.f, i0 = 0.f, r1 = 0.f, i1 = 0.f;
+  for (int k = 0; k < 6; ++k) o[k] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int t = 0; t < LSMAX; ++t) {
-          if (t < ls) {
-            const cplx<float> m = M[s * ls + t];
-            r0 += m.re * v[t].x;
-            i0 += m.re * v[t].y;
-            r1 += m.re * v[t].z;
-            i1 += m.re * v[t].w;
-            if (!REAL) {
-              r0 -= m.im * v[t].y;
-              i0 += m.im * v[t].x;
-              r1 -= m.im * v[t].w;
-              i1 += m.im * v[t].z;
-            }
-          }
+  for (int t = 0; t < LSMAX; ++t) {
+    if (t < ls) {
+      const cplx<float> k0 = km[0][s][t], k1 = km[1][s][t];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const cplx<float> m = k < 3 ? k0 : k1;
+        const float4 q = hr[t][k][site];
+        o[k].x += m.re * q.x;
+        o[k].y += m.re * q.y;
+        o[k].z += m.re * q.z;
+        o[k].w += m.re * q.w;
+        if (!REAL) {
+          o[k].x -= m.im * q.y;
+          o[k].y += m.im * q.x;
+          o[k].z -= m.im * q.w;
+          o[k].w += m.im * q.z;
         }
-        hr[s][row][site] = make_float4(r0, i0, r1, i1);
       }
     }
   }
-  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    v[2 * k] = cmk<float>(o[k].x, o[k].y);
+    v[2 * k + 1] = cmk<float>(o[k].z, o[k].w);
+  }
 }
 
 // streaming store that also returns the stored values
@@ -557,14 +563,7 @@ k_hop_m5s(HopM5Args a)
     __shared__ float4 hr[LSMAX][6][SB];
     // the block's s-matrices (both chiralities) in LDS
     __shared__ cplx<float> km[2][LSMAX][LSMAX];
-    {
-      const cplx<float>* __restrict__ M = pick(a.mats, ci);
-      // compile-time divisors: (chi, r, t) of entry i of the LDS copy
-      for (int i = threadIdx.x; i < 2 * LSMAX * LSMAX; i += blockDim.x) {
-        const int chi = i / (LSMAX * LSMAX), r = (i / LSMAX) % LSMAX, t = i % LSMAX;
-        if (r < ls && t < ls) km[chi][r][t] = M[(chi * ls + r) * ls + t];
-      }
-    }
+    stage_km<LSMAX>(km, pick(a.mats, ci), ls);
     if (active && row < ls) {
 #pragma unroll
       for (int k = 0; k < 6; ++k)
@@ -573,37 +572,7 @@ k_hop_m5s(HopM5Args a)
     }
     __syncthreads();
     // thread (site, s) forms its own output row s for all 12 components
-    if (active && row < ls) {
-      float4 o[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) o[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int t = 0; t < LSMAX; ++t) {
-        if (t < ls) {
-          const cplx<float> k0 = km[0][row][t], k1 = km[1][row][t];
-#pragma unroll
-          for (int k = 0; k < 6; ++k) {
-            const cplx<float> m = k < 3 ? k0 : k1;
-            const float4 q = hr[t][k][site];
-            o[k].x += m.re * q.x;
-            o[k].y += m.re * q.y;
-            o[k].z += m.re * q.z;
-            o[k].w += m.re * q.w;
-            if (!REAL) {
-              o[k].x -= m.im * q.y;
-              o[k].y += m.im * q.x;
-              o[k].z -= m.im * q.w;
-              o[k].w += m.im * q.z;
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        v[2 * k] = cmk<float>(o[k].x, o[k].y);
-        v[2 * k + 1] = cmk<float>(o[k].z, o[k].w);
-      }
-    }
+    if (active && row < ls) smat_row<SB, LSMAX, REAL>(hr, km, ls, site, row, v);
   } else {
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = cmk<float>(sc * acc[k].re, sc * acc[k].im);
@@ -788,16 +757,13 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
     for (int k = 0; k < 6; ++k)
       hr[row][k][site] = make_float4(pv[2 * k].re, pv[2 * k].im, pv[2 * k + 1].re, pv[2 * k + 1].im);
   }
+  __shared__ cplx<float> km[2][LSMAX][LSMAX];
+  stage_km<LSMAX>(km, pick(a.mats, ci), ls);
   __syncthreads();
-  smat_lds<SB, LSMAX, REAL>(hr, pick(a.mats, ci), ls, site, row, active);
+  // t = Mat p: thread (site, s) forms its own row s
   if (active && row < ls) {
     cplx<float> v[12];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const float4 q = hr[row][k][site];
-      v[2 * k] = cmk<float>(q.x, q.y);
-      v[2 * k + 1] = cmk<float>(q.z, q.w);
-    }
+    smat_row<SB, LSMAX, REAL>(hr, km, ls, site, row, v);
     store_q<ST>((slot_t*)a.t + coff, ls, row, h, v);
   }
 }
@@ -817,12 +783,20 @@ static void launch_cg_step_s_ls(const CgStepArgs<float>& a, hipStream_t st)
   }
 }
 
+template <class ST, class SR, class SX, int SB>
+static void launch_cg_step_s_sb(const CgStepArgs<float>& a, hipStream_t st)
+{
+  if (a.ls <= 8) launch_cg_step_s_ls<ST, SR, SX, SB, 8>(a, st);
+  else if (a.ls <= 12) launch_cg_step_s_ls<ST, SR, SX, SB, 12>(a, st);
+  else launch_cg_step_s_ls<ST, SR, SX, SB, 16>(a, st);
+}
+
+// 16 sites per block when asked for and Ls allows whole waves, else 32
 template <class ST, class SR, class SX>
 static void launch_cg_step_s_st(const CgStepArgs<float>& a, hipStream_t st)
 {
-  if (a.ls <= 8) launch_cg_step_s_ls<ST, SR, SX, 32, 8>(a, st);
-  else if (a.ls <= 12) launch_cg_step_s_ls<ST, SR, SX, 32, 12>(a, st);
-  else launch_cg_step_s_ls<ST, SR, SX, 32, 16>(a, st);
+  if (a.sb == 16 && three_phase_ok(a.ls, 16)) launch_cg_step_s_sb<ST, SR, SX, 16>(a, st);
+  else launch_cg_step_s_sb<ST, SR, SX, 32>(a, st);
 }
 
 // storage: 0 all FP32; 1 p, t FP16, r and x FP32; 2 all FP16
