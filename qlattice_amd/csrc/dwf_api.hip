@@ -177,6 +177,7 @@ struct qlat_dwf_inverter {
                                // FP32 kernels, single s-matrix pass per hop)
   int tp_sb = 32;              // sites per block of the three-phase kernels (QLAT_DWF_TP_SB)
   int step_sb = 32;            // sites per block of the three-phase CG step (QLAT_DWF_STEP_SB)
+  bool x_defer = true;         // x updated every other CG step, two directions at once (QLAT_DWF_XDEFER)
   int site_major = 0;          // three-phase hop block order (QLAT_DWF_SITE_MAJOR)
   int fp16_mode = 1;           // FP16 solver storage mode 1 or 2 (ThreePhase; QLAT_DWF_FP16_MODE)
   TileChoice tuned[4] = {};    // autotuned tile per storage mode of the three-phase solver
@@ -207,8 +208,8 @@ struct qlat_dwf_inverter {
   // multi-RHS: FP32 stacks of `mcap` fields and per-column FP64 fields
   int rhs_tile = 0;  // columns per multi-RHS solve (QLAT_DWF_RHS_TILE; 0 = autotuned / 12)
   int mcap = 0;
-  float4* m32[8] = {};
-  uint4* mh[7] = {};            // FP16 (per-site scale) stacks of the three-phase solver
+  float4* m32[9] = {};
+  uint4* mh[8] = {};            // FP16 (per-site scale) stacks of the three-phase solver
   std::vector<double2*> col64;  // 3 per column: b, x, out_e_p
 
   qlat_dwf_solve_stats stats{};
@@ -250,7 +251,7 @@ float4* V(Inv& I, int id)
   }
   return I.w32[id];
 }
-enum Multi32 { M_R, M_P, M_AP, M_X, M_T, M_A, M_B, M_Y };
+enum Multi32 { M_R, M_P, M_AP, M_X, M_T, M_A, M_B, M_Y, M_P2 };
 
 // FP32 stack buffer `id` with room for `ncol` fields.
 float4* MV(Inv& I, int id, int ncol)
@@ -277,7 +278,7 @@ float4* MV(Inv& I, int id, int ncol)
 }
 
 // FP16 stacks (16-byte units per field: fieldh_units)
-enum MultiH { H_P, H_AP, H_T, H_B, H_Y, H_R, H_X };
+enum MultiH { H_P, H_AP, H_T, H_B, H_Y, H_R, H_X, H_P2 };
 int64_t fieldh_units(const Inv& I) { return (I.geo.stride >> 6) * I.ls * 208; }
 uint4* MH(Inv& I, int id, int ncol)
 {
@@ -919,6 +920,11 @@ struct ThreePhase {
   void *P, *AP, *T, *B, *Y;
   Inv* colinv[qdwf::kMaxRhs];  // the inverter (mass) of each stack column
   qdwf::CgState* cg = nullptr;  // set while a graph iteration is captured
+  // p is double-buffered: P = Pbuf[cur] is the current direction, and a
+  // flipping step writes the next one into the other buffer, whose old
+  // contents (the previous direction) it can still add to x first
+  void* Pbuf[2] = {nullptr, nullptr};
+  int cur = 0;
   ThreePhase(Inv& inv, int ncol, int storage_, int sb_ = 0)
       : I(inv), storage(storage_), sb(sb_ ? sb_ : inv.tp_sb)
   {
@@ -934,17 +940,20 @@ struct ThreePhase {
     X = xh() ? (void*)MH(I, H_X, ncol) : (void*)MV(I, M_X, ncol);
     if (half()) {
       P = MH(I, H_P, ncol);
+      Pbuf[1] = MH(I, H_P2, ncol);
       AP = MH(I, H_AP, ncol);
       T = MH(I, H_T, ncol);
       B = MH(I, H_B, ncol);
       Y = MH(I, H_Y, ncol);
     } else {
       P = MV(I, M_P, ncol);
+      Pbuf[1] = MV(I, M_P2, ncol);
       AP = MV(I, M_AP, ncol);
       T = MV(I, M_T, ncol);
       B = MV(I, M_B, ncol);
       Y = MV(I, M_Y, ncol);
     }
+    Pbuf[0] = P;
   }
   bool half() const { return storage >= 1; }
   bool rxh() const { return storage == 2; }
@@ -971,7 +980,11 @@ struct ThreePhase {
     check_launch();
   }
   void zero_x(int c0, int n) { QDWF_HIP(hipMemsetAsync(xat(c0), 0, (size_t)fx * xslot_bytes() * n, I.stream)); }
-  void zero_p(int n) { QDWF_HIP(hipMemsetAsync(P, 0, (size_t)fh * slot_bytes() * n, I.stream)); }
+  void zero_p(int n)
+  {
+    for (void* b : Pbuf) QDWF_HIP(hipMemsetAsync(b, 0, (size_t)fh * slot_bytes() * n, I.stream));
+  }
+  void* other() const { return Pbuf[cur ^ 1]; }
   qdwf::RhsSet colset(const std::vector<int>& act) const
   {
     qdwf::RhsSet cs;
@@ -995,9 +1008,11 @@ struct ThreePhase {
   // and |Ap|^2, so |r - alpha Ap|^2 is known before r is updated, and the
   // update of r moves into the CG step.
   bool merged() const { return !rxh(); }
-  // [r -= alpha_r ap (alpha_r not null)]; x += alpha p (updx); p = r + beta p; t = K p
+  // [r -= alpha_r ap (alpha_r not null)]; x += alpha p (updx); p = r + beta p; t = K p.
+  // flip: the new p goes into the other buffer, and x also gets alpha_prev
+  // times that buffer's previous direction.
   void step(const std::vector<int>& cols, const double* alpha, const double* beta, bool updx,
-            const double* alpha_r = nullptr)
+            const double* alpha_r = nullptr, const double* alpha_prev = nullptr, bool flip = false)
   {
     qdwf::CgStepArgs<float> a{};
     a.geo = I.geo;
@@ -1006,7 +1021,9 @@ struct ThreePhase {
       a.alpha[i] = i < (int)cols.size() ? alpha[i] : 0.0;
       a.beta[i] = i < (int)cols.size() ? beta[i] : 0.0;
       a.alpha_r[i] = (alpha_r && i < (int)cols.size()) ? alpha_r[i] : 0.0;
+      a.alpha_prev[i] = (flip && alpha_prev && i < (int)cols.size()) ? alpha_prev[i] : 0.0;
     }
+    a.p_out = flip ? other() : nullptr;
     a.ap = alpha_r ? AP : nullptr;
     a.cols = colset(cols);
     a.xr_fstride = fr;
@@ -1020,6 +1037,10 @@ struct ThreePhase {
     a.t = T;
     if (!qdwf::launch_cg_step_s(a, storage, I.stream)) throw ArgError("the three-phase CG step needs an even ls in [6, 16]");
     check_launch();
+    if (flip) {
+      cur ^= 1;
+      P = Pbuf[cur];
+    }
   }
   // the merged step with its scalars and column modes read from the device
   // CG state (graph-captured iterations)
@@ -1163,6 +1184,7 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
   double2* ax64 = W(I, W_CG_AP);
   std::vector<double> b2(ncol), bn(ncol), rr(ncol), maxrr(ncol);
   std::vector<long> iters(ncol, 0);
+  std::vector<double> pend(ncol, 0.0);  // alpha of a direction not yet added to x
   std::vector<int> active;
   const double target = stop_rsd * stop_rsd;
   for (int c = 0; c < ncol; ++c) {
@@ -1312,7 +1334,7 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
     }
     ++it;
     std::vector<int> keep;
-    std::vector<double> ka, kb, kr;
+    std::vector<double> ka, kb, kr, kq;
     for (int i = 0; i < cs.n; ++i) {
       const int c = active[i];
       iters[c] += 1;
@@ -1320,7 +1342,11 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
         throw std::runtime_error(storage ? "CG breakdown: non-finite residual (FP16 storage, multi)"
                                          : "CG breakdown: non-finite residual (FP32 three-phase, multi)");
       double step_alpha = alpha[i];
+      double xa = alpha[i], xq = 0.0;  // x += xa p + xq (previous p)
       if (rr_new[i] <= d2 * maxrr[c] || rr_new[i] <= target) {
+        if (pend[c] != 0.0) tp.axpy_to_x(pend[c], tp.other(), c);  // the deferred previous step
+        pend[c] = 0.0;
+        xa = 0.0;
         tp.axpy_to_x(alpha[i], tp.P, c);
         tp.add_x(x64[c], c, bn[c]);
         tp.zero_x(c, 1);
@@ -1335,19 +1361,31 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
         rr_new[i] = rr_true;
         maxrr[c] = rr_true;
         step_alpha = 0.0;
+      } else if (I.x_defer) {
+        // x gets alpha p every other step, two directions at once (p is
+        // double-buffered, so the previous one is still there)
+        if (pend[c] == 0.0) {
+          pend[c] = alpha[i];
+          xa = 0.0;
+        } else {
+          xq = pend[c];
+          pend[c] = 0.0;
+        }
       }
       keep.push_back(c);
-      ka.push_back(step_alpha);
+      ka.push_back(xa);
+      kq.push_back(xq);
       kb.push_back(rr_new[i] / rr[c]);
       kr.push_back(step_alpha);  // r was replaced by the true residual when 0
       rr[c] = rr_new[i];
     }
-    if (!keep.empty()) tp.step(keep, ka.data(), kb.data(), true, merged ? kr.data() : nullptr);
+    if (!keep.empty()) tp.step(keep, ka.data(), kb.data(), true, merged ? kr.data() : nullptr, kq.data(), true);
     active.swap(keep);
   }
   long total = 0;
   for (int c = 0; c < ncol; ++c) total += iters[c];
   for (int c : active) {  // cap reached: fold in what we have
+    if (pend[c] != 0.0) tp.axpy_to_x(pend[c], tp.other(), c);
     tp.add_x(x64[c], c, bn[c]);
     const double rel = std::sqrt(true_residual(c));
     if (rel > I.stats.true_residual) I.stats.true_residual = rel;
@@ -2143,6 +2181,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_TP_SB")) I->tp_sb = std::atoi(e) == 16 ? 16 : 32;
     if (const char* e = std::getenv("QLAT_DWF_SITE_MAJOR")) I->site_major = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_FP16_MODE")) I->fp16_mode = std::max(1, std::min(2, std::atoi(e)));
+    if (const char* e = std::getenv("QLAT_DWF_XDEFER")) I->x_defer = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_STEP_SB")) I->step_sb = std::atoi(e) == 16 ? 16 : 32;
     if (const char* e = std::getenv("QLAT_DWF_GRAPH")) I->graph_iters = std::max(0, std::min(64, std::atoi(e)));
     {

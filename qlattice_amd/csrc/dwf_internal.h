@@ -76,6 +76,12 @@ struct CgStepArgs {
   int64_t x_fstride;   // of x
   const cplx<T>* mats[kMaxRhs];
   int sb;
+  // double-buffered p (three-phase variant): when p_out is not null the new
+  // p goes there, and x += alpha p + alpha_prev q with q the p_out field's
+  // old contents (the previous iteration's p), read before it is replaced;
+  // x is not touched for a column whose two coefficients are zero
+  void* p_out;
+  double alpha_prev[kMaxRhs];
   // device-scalar form (graph-captured iterations): when cg is not null the
   // per-column alpha, alpha_r, beta and mode are read from it instead
   CgState* cg;

@@ -707,7 +707,7 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
   const int64_t roff = (int64_t)col * a.xr_fstride, xoff = (int64_t)col * a.x_fstride;
   // scalars from the kernel arguments, or from the device CG state
   // (block-uniform: the whole block returns together)
-  float al, be, ar;
+  float al, be, ar, aq = 0.f;
   if (a.cg) {
     const int mode = a.cg->mode[ci];
     if (mode == 1) return;
@@ -730,7 +730,9 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
     al = (float)pick(a.alpha, ci);
     be = (float)pick(a.beta, ci);
     ar = (float)pick(a.alpha_r, ci);
+    aq = (float)pick(a.alpha_prev, ci);
   }
+  slot_t* pout = (slot_t*)(a.p_out ? a.p_out : a.p) + coff;  // may be a.p
   if (active && row < ls) {
     cplx<float> pv[12], rv[12];
     SpinorIO<ST>::load((const slot_t*)a.p + coff, ls, row, h, pv);
@@ -742,17 +744,23 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
       for (int k = 0; k < 12; ++k) rv[k] = cmk<float>(rv[k].re - ar * av[k].re, rv[k].im - ar * av[k].im);
       store_q<SR>((rslot_t*)a.r + roff, ls, row, h, rv);
     }
-    if (UPDX) {
+    if (UPDX && (al != 0.f || aq != 0.f)) {  // block-uniform
       cplx<float> xv[12];
       xslot_t* x = (xslot_t*)a.x + xoff;
       SpinorIO<SX>::load(x, ls, row, h, xv);
+      if (aq != 0.f) {  // the previous iteration's deferred alpha q (q in p_out, not yet replaced)
+        cplx<float> qv[12];
+        SpinorIO<ST>::load(pout, ls, row, h, qv);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + aq * qv[k].re, xv[k].im + aq * qv[k].im);
+      }
 #pragma unroll
       for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + al * pv[k].re, xv[k].im + al * pv[k].im);
       SpinorIO<SX>::store(x, ls, row, h, xv);
     }
 #pragma unroll
     for (int k = 0; k < 12; ++k) pv[k] = cmk<float>(be * pv[k].re + rv[k].re, be * pv[k].im + rv[k].im);
-    store_q<ST>((slot_t*)a.p + coff, ls, row, h, pv);
+    store_q<ST>(pout, ls, row, h, pv);
 #pragma unroll
     for (int k = 0; k < 6; ++k)
       hr[row][k][site] = make_float4(pv[2 * k].re, pv[2 * k].im, pv[2 * k + 1].re, pv[2 * k + 1].im);
