@@ -2672,7 +2672,8 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
       QDWF_HIP(hipMemsetAsync(tp->R, 0x3f, (size_t)tp->fr * tp->rslot_bytes() * tile, I.stream));
       tp->zero_x(0, tile);
     }
-    const std::vector<double> al(qdwf::kMaxRhs, 0.0), be(qdwf::kMaxRhs, 0.5);
+    // nonzero alpha: the step then updates x as well (it skips x for alpha 0)
+    const std::vector<double> al(qdwf::kMaxRhs, 1e-3), be(qdwf::kMaxRhs, 0.5);
     auto run = [&]() {
       if (which == 5) {
         tp->hopm5(tp->B, tp->T, 2, false, MAT_K, 1.f, nullptr, 0.f, nullptr, tp->colset(cols));

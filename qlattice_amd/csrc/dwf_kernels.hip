@@ -1617,17 +1617,32 @@ __global__ void __launch_bounds__(kBlasBlock) k_axpy_norm_multi(AxpyNormArgs<S> 
 }
 
 // out[row] = sum of part[row*nblk + 0 .. nblk)
-__global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ part, int nblk, double* out)
+// One 1024-thread block per row (4 independent partial sums per thread, so
+// several loads are in flight); the summation order depends only on nblk.
+__global__ void __launch_bounds__(1024) k_reduce_rows(const double* __restrict__ part, int nblk, double* out)
 {
-  __shared__ double sh[4];
+  __shared__ double sh[16];
   const double* p = part + (int64_t)blockIdx.x * nblk;
-  double v = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += 256) v += p[b];
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int b = threadIdx.x;
+  for (; b + 3 * 1024 < nblk; b += 4 * 1024) {
+    v0 += p[b];
+    v1 += p[b + 1024];
+    v2 += p[b + 2 * 1024];
+    v3 += p[b + 3 * 1024];
+  }
+  for (; b < nblk; b += 1024) v0 += p[b];
+  double v = (v0 + v1) + (v2 + v3);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) out[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) t += sh[w];
+    out[blockIdx.x] = t;
+  }
 }
 
 // Sum nblk partial rows of nv values (nv <= 2) with one 256-thread block;
@@ -1758,14 +1773,14 @@ void launch_axpy_norm(double a, const S* ap, S* r, int64_t n, double* part, doub
 }
 void launch_reduce_rows(const double* part, int nblk, int nrows, double* out, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(256), 0, st, part, nblk, out);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(1024), 0, st, part, nblk, out);
 }
 template <class S>
 int launch_axpy_norm_multi(const AxpyNormArgs<S>& a, double* out, hipStream_t st)
 {
   const int g = blas_grid(a.n);
   hipLaunchKernelGGL(k_axpy_norm_multi<S>, dim3(g, a.cols.n), dim3(kBlasBlock), 0, st, a);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(a.cols.n), dim3(256), 0, st, a.part, g, out);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(a.cols.n), dim3(1024), 0, st, a.part, g, out);
   return g;
 }
 template int launch_axpy_norm_multi<double2>(const AxpyNormArgs<double2>&, double*, hipStream_t);
