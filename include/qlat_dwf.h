@@ -186,6 +186,41 @@ int qlat_dwf_invert_4d(qlat_dwf_inverter* inv, double* out, const double* in, lo
  * (inverter.py:96-109 -> cqlat/inverter.cpp:23-36). */
 int qlat_dwf_invert_prop(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter);
 
+/* ------------------------------------------------------------- sources
+ * Propagator sources built on the device (qcd-prop.cpp:61-150, :260-307,
+ * :465-530).  Every kind is a unit spin-colour matrix times a site factor
+ * f(x): column cs of the propagator is the fermion field with f(x) in
+ * component cs (set_propagator_col_from_fermion_field, qcd-prop.h:55-68).
+ * mom = lmom * 2 pi / total_site (lattice_mom_mult, field.h:14-22) and
+ * phase = sum_mu mom[mu] * x[mu]. */
+#define QLAT_DWF_SRC_POINT 0 /* f = value at x = xg (set_point_src, qcd-prop.cpp:61-89) */
+#define QLAT_DWF_SRC_WALL 1  /* f = exp(i phase) on t = tslice, lmom[3] = 0 (set_wall_src,
+                                set_tslice_mom_src: qcd-prop.cpp:117-150, :465-499) */
+#define QLAT_DWF_SRC_MOM 2   /* f = exp(i phase) everywhere (set_mom_src, qcd-prop.cpp:275-307;
+                                set_volume_src, :501-530, requires lmom[3] = 0) */
+#define QLAT_DWF_SRC_U1 3    /* f = fu1(x), V complex values in reference site order
+                                (set_rand_vol_u1_src, qcd-prop.cpp:260-273) */
+
+typedef struct {
+  int kind;
+  int xg[4];          /* POINT */
+  int tslice;         /* WALL */
+  double lmom[4];     /* WALL, MOM */
+  double value[2];    /* POINT: re, im */
+  const double* fu1;  /* U1: host pointer for the host entry points, device pointer for _device */
+} qlat_dwf_source;
+
+/* The source propagator [V][12][12] on a lattice of total_site
+ * (set_point_src, set_wall_src, set_mom_src, set_rand_vol_u1_src;
+ * propagator.pyx:175-234 mk_point_src, mk_wall_src, mk_rand_vol_u1_src). */
+int qlat_dwf_make_source(const int total_site[4], const qlat_dwf_source* src, double* prop);
+
+/* set_point_src_propagator / set_wall_src_propagator /
+ * set_mom_src_propagator / set_tslice_mom_src_propagator (qcd-prop.h:202-290,
+ * :403-420): the source is built on the device and its 12 columns are solved
+ * together (invert(Prop), as qlat_dwf_invert_prop); sol is [V][12][12]. */
+int qlat_dwf_source_propagator(qlat_dwf_inverter* inv, const qlat_dwf_source* src, double* sol, long* iter);
+
 /* find_max_eigen_value_hermop_sym2 (dslash.h:586-588; dslash.cpp:1349-1369)
  * from a caller-supplied odd half-field start vector. */
 int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,
@@ -242,6 +277,9 @@ int qlat_dwf_inverter_set_gauge(qlat_dwf_inverter* inv, const double* gauge, int
 int qlat_dwf_multiply_hermop_sym2_device(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
 int qlat_dwf_invert_5d_device(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
 int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter);
+int qlat_dwf_make_source_device(const int total_site[4], const qlat_dwf_source* src, double* prop);
+int qlat_dwf_source_propagator_device(qlat_dwf_inverter* inv, const qlat_dwf_source* src, double* sol,
+                                      long* iter);
 
 /* --------------------------------------------------------- measurement
  * Average duration (ms) of one launch of a kernel, timed with HIP events

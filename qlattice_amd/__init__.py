@@ -3,8 +3,9 @@
 A drop-in for Qlattice's even-odd Moebius domain-wall path
 (qlat/qlat/lib/dslash.cpp, qlat/qlat/include/qlat/dslash.h) with the same
 Python surface as ``qlat`` for it: ``FermionAction``, ``InverterDomainWall``,
-``GaugeField``, ``Prop``, ``mk_point_src``, and the ``multiply_*`` /
-``invert`` / ``cg_with_herm_sym_2`` functions.  All arithmetic runs in the
+``GaugeField``, ``Prop``, the point / wall / momentum / U(1) sources and
+their ``*_src_propagator`` solves, and the ``multiply_*`` / ``invert`` /
+``cg_with_herm_sym_2`` functions.  All arithmetic runs in the
 hand-written HIP library ``libqlat_dwf_hip.so`` (C ABI: include/qlat_dwf.h).
 """
 
@@ -13,6 +14,7 @@ from .fields import (
     Coordinate,
     CoordinateD,
     FermionField4d,
+    FieldComplexD,
     FermionField5d,
     GaugeField,
     Geometry,
@@ -20,7 +22,6 @@ from .fields import (
     fermion_field_4d_from_5d,
     fermion_field_5d_from_4d,
     get_half_fermion,
-    mk_point_src,
     set_half_fermion,
     set_point_src_fermion_field,
 )
@@ -52,5 +53,20 @@ from .inverter import (
 )
 from ._lib import QlatDwfError, load as load_library
 from .force import fermion_force, pseudofermion_action, set_gauge
+from .sources import (
+    mk_point_src,
+    mk_rand_vol_u1_src,
+    mk_wall_src,
+    set_mom_src,
+    set_mom_src_propagator,
+    set_point_src,
+    set_point_src_propagator,
+    set_rand_vol_u1_src,
+    set_tslice_mom_src,
+    set_tslice_mom_src_propagator,
+    set_volume_src,
+    set_wall_src,
+    set_wall_src_propagator,
+)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -49,6 +49,10 @@ EXPORTED = [
     "qlat_dwf_invert_5d",
     "qlat_dwf_invert_4d",
     "qlat_dwf_invert_prop",
+    "qlat_dwf_make_source",
+    "qlat_dwf_make_source_device",
+    "qlat_dwf_source_propagator",
+    "qlat_dwf_source_propagator_device",
     "qlat_dwf_find_max_eigen_value_hermop_sym2",
     "qlat_dwf_multiply_hermop_sym2_device",
     "qlat_dwf_invert_5d_device",
@@ -90,6 +94,20 @@ class SolveStatsC(ctypes.Structure):
         ("sites_per_block", ctypes.c_int),
     ]
 
+
+class SourceC(ctypes.Structure):
+    """qlat_dwf_source (include/qlat_dwf.h)."""
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("xg", ctypes.c_int * 4),
+        ("tslice", ctypes.c_int),
+        ("lmom", ctypes.c_double * 4),
+        ("value", ctypes.c_double * 2),
+        ("fu1", ctypes.c_void_p),
+    ]
+
+
+SRC_POINT, SRC_WALL, SRC_MOM, SRC_U1 = 0, 1, 2, 3
 
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -149,6 +167,11 @@ def load():
     for name in ("qlat_dwf_invert_5d_batch", "qlat_dwf_invert_5d_batch_device"):
         getattr(L, name).argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_dp),
                                      ctypes.POINTER(_dp), _lp]
+    _ip4 = ctypes.POINTER(ctypes.c_int)
+    L.qlat_dwf_make_source.argtypes = [_ip4, ctypes.POINTER(SourceC), _dp]
+    L.qlat_dwf_make_source_device.argtypes = [_ip4, ctypes.POINTER(SourceC), _dp]
+    L.qlat_dwf_source_propagator.argtypes = [_vp, ctypes.POINTER(SourceC), _dp, _lp]
+    L.qlat_dwf_source_propagator_device.argtypes = [_vp, ctypes.POINTER(SourceC), _dp, _lp]
     for name in EXPORTED:
         if name not in ("qlat_dwf_last_error", "qlat_dwf_abi_version"):
             getattr(L, name).restype = ctypes.c_int

@@ -2421,6 +2421,120 @@ int qlat_dwf_invert_prop(qlat_dwf_inverter* inv, double* sol, const double* src,
   });
 }
 
+namespace {
+
+// Kernel arguments of a source (validated as the reference does: a wall
+// source needs lmom[3] = 0, qcd-prop.cpp:120; a point outside the lattice
+// leaves the source zero, :70-72).  fu1 is a device pointer.
+qdwf::SrcArgs src_args(const int n[4], const qlat_dwf_source& s, const double2* fu1)
+{
+  QDWF_REQUIRE(s.kind >= QLAT_DWF_SRC_POINT && s.kind <= QLAT_DWF_SRC_U1, "unknown source kind");
+  qdwf::SrcArgs a{};
+  for (int mu = 0; mu < 4; ++mu) {
+    QDWF_REQUIRE(n[mu] > 0, "total_site must be positive");
+    a.n[mu] = n[mu];
+    a.xg[mu] = s.xg[mu];
+    // lattice_mom_mult (field.h:14-17): 2 pi / total_site, with the
+    // reference's double pi (qlat-utils core.h:41)
+    constexpr double kPi = 3.141592653589793;
+    a.mom[mu] = s.lmom[mu] * (2 * kPi / (double)n[mu]);
+  }
+  a.kind = s.kind;
+  a.tslice = s.tslice;
+  a.value_re = s.value[0];
+  a.value_im = s.value[1];
+  if (s.kind == QLAT_DWF_SRC_WALL) QDWF_REQUIRE(s.lmom[3] == 0.0, "wall source: lmom[3] must be 0");
+  if (s.kind == QLAT_DWF_SRC_U1) {
+    QDWF_REQUIRE(fu1, "U(1) source: null fu1");
+    a.fu1 = fu1;
+  }
+  return a;
+}
+
+size_t prop_bytes_of(const int n[4]) { return (size_t)n[0] * n[1] * n[2] * n[3] * 144 * sizeof(double2); }
+
+// Build the source into the device propagator `prop` on the inverter's
+// stream; a host fu1 goes through staging buffer 1 first.
+void make_source_dev(Inv& I, const qlat_dwf_source& s, double2* prop, bool host_fu1)
+{
+  const double2* fu1 = nullptr;
+  if (s.kind == QLAT_DWF_SRC_U1) {
+    QDWF_REQUIRE(s.fu1, "U(1) source: null fu1");
+    fu1 = host_fu1 ? upload(I, 1, s.fu1, (size_t)I.geo.vh * 2 * sizeof(double2)) : (const double2*)s.fu1;
+  }
+  qdwf::launch_make_src(prop, src_args(I.geo.n, s, fu1), I.stream);
+  check_launch();
+}
+
+}  // namespace
+
+int qlat_dwf_make_source(const int total_site[4], const qlat_dwf_source* src, double* prop)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(total_site && src && prop, "invalid argument");
+    const size_t bytes = prop_bytes_of(total_site);
+    const size_t vbytes = bytes / 144;
+    double2* d = (double2*)dev_alloc(bytes + (src->kind == QLAT_DWF_SRC_U1 ? vbytes : 0));
+    try {
+      double2* fu1 = nullptr;
+      if (src->kind == QLAT_DWF_SRC_U1 && src->fu1) {
+        fu1 = d + bytes / sizeof(double2);
+        QDWF_HIP(hipMemcpy(fu1, src->fu1, vbytes, hipMemcpyHostToDevice));
+      }
+      qdwf::launch_make_src(d, src_args(total_site, *src, fu1), nullptr);
+      check_launch();
+      QDWF_HIP(hipMemcpy(prop, d, bytes, hipMemcpyDeviceToHost));
+    } catch (...) {
+      (void)hipFree(d);
+      throw;
+    }
+    QDWF_HIP(hipFree(d));
+  });
+}
+
+int qlat_dwf_make_source_device(const int total_site[4], const qlat_dwf_source* src, double* prop)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(total_site && src && prop, "invalid argument");
+    qdwf::launch_make_src((double2*)prop, src_args(total_site, *src, (const double2*)src->fu1), nullptr);
+    check_launch();
+    QDWF_HIP(hipDeviceSynchronize());
+  });
+}
+
+int qlat_dwf_source_propagator(qlat_dwf_inverter* inv, const qlat_dwf_source* src, double* sol, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(src && sol && iter, "invalid argument");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = prop_bytes(*inv);
+    double2* d_src = stage(*inv, 0, bytes);
+    make_source_dev(*inv, *src, d_src, true);  // fu1 staged in buffer 1 before it holds the solution
+    double2* d_sol = stage(*inv, 1, bytes);
+    *iter = invert_prop_dev(*inv, d_sol, d_src, true);
+    download(*inv, sol, d_sol, bytes);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_source_propagator_device(qlat_dwf_inverter* inv, const qlat_dwf_source* src, double* sol,
+                                      long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(src && sol && iter, "invalid argument");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    double2* d_src = stage(*inv, 0, prop_bytes(*inv));
+    make_source_dev(*inv, *src, d_src, false);
+    *iter = invert_prop_dev(*inv, (double2*)sol, d_src, true);
+    sync(*inv);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
 int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,
                                               double* result)
 {

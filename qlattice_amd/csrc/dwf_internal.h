@@ -187,6 +187,20 @@ int64_t hop_m5s_blocks(const LatGeo& g, int sb);
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st);
 // storage: 0 all FP32; 1 p, t (ap) FP16 and x, r FP32; 2 all FP16
 bool launch_cg_step_s(const CgStepArgs<float>& a, int storage, hipStream_t st);
+// Unit spin-colour propagator source times a site factor f(x), written in
+// the reference layout [V][12][12] (kind: 0 point, 1 wall, 2 momentum,
+// 3 U(1) field; mom in radians per lattice unit).
+struct SrcArgs {
+  int n[4];
+  int kind;
+  int xg[4];
+  int tslice;
+  double mom[4];
+  double value_re, value_im;
+  const double2* fu1;  // [V] complex (kind 3)
+};
+void launch_make_src(double2* prop, const SrcArgs& a, hipStream_t st);
+
 // alpha, beta, |r|^2 and modes of the n column slots from the merged hermop
 // reductions red[i] = Re <p, Ap>, red[n + i] = Re <r, Ap>, red[2n + i] = |Ap|^2
 void launch_cg_scalars(CgState* cg, const double* red, int n, hipStream_t st);

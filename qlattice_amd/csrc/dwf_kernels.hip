@@ -858,6 +858,53 @@ void launch_cg_scalars(CgState* cg, const double* red, int n, hipStream_t st)
   hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(64), 0, st, cg, red, n);
 }
 
+// Source propagator: thread (site, row) writes row `row` of the site's
+// 12x12 matrix, f(x) on the diagonal (set_point_src, set_wall_src,
+// set_mom_src, set_rand_vol_u1_src: qcd-prop.cpp:61-150, :260-307).  The
+// phase is summed over mu = 0..3 in order, as the reference loop does.
+__global__ void __launch_bounds__(kBlasBlock) k_make_src(double2* __restrict__ prop, SrcArgs a)
+{
+#pragma clang fp contract(off)
+  const int64_t vol = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.n[3];
+  const int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x;
+  if (i >= vol * 12) return;
+  const int64_t site = i / 12;
+  const int row = (int)(i - site * 12);
+  int x[4];
+  int64_t r = site;
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    const int64_t q = r / a.n[mu];
+    x[mu] = (int)(r - q * a.n[mu]);
+    r = q;
+  }
+  x[3] = (int)r;
+  double2 f = make_double2(0.0, 0.0);
+  if (a.kind == 0) {
+    if (x[0] == a.xg[0] && x[1] == a.xg[1] && x[2] == a.xg[2] && x[3] == a.xg[3]) f = make_double2(a.value_re, a.value_im);
+  } else if (a.kind == 1 || a.kind == 2) {
+    if (a.kind == 2 || x[3] == a.tslice) {
+      double phase = 0.0;
+#pragma unroll
+      for (int mu = 0; mu < 4; ++mu) phase += a.mom[mu] * x[mu];
+      double sn, cs;
+      sincos(phase, &sn, &cs);
+      f = make_double2(cs, sn);
+    }
+  } else {
+    f = a.fu1[site];
+  }
+  double2* out = prop + i * 12;
+#pragma unroll
+  for (int c = 0; c < 12; ++c) out[c] = (c == row) ? f : make_double2(0.0, 0.0);
+}
+
+void launch_make_src(double2* prop, const SrcArgs& a, hipStream_t st)
+{
+  const int64_t n = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.n[3] * 12;
+  hipLaunchKernelGGL(k_make_src, dim3((unsigned)((n + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st, prop, a);
+}
+
 // r_i -= alpha_i ap_i over 5D sites (ap in storage ST, r in SR) with
 // per-column partial |r_i|^2 of the stored r (part[i*gridDim.x + block]).
 template <class ST, class SR>

@@ -2,7 +2,7 @@
 
 These mirror the parts of qlat's Python surface that the domain-wall path
 consumes (``q.Coordinate``, ``q.Geometry``, ``q.GaugeField``, ``q.Prop``,
-``q.FermionField4d``, ``q.mk_point_src``).  Fields are numpy arrays in the
+``q.FermionField4d``, ``q.FieldComplexD``).  Fields are numpy arrays in the
 reference's array-of-structures complex128 layout, so data moves to and from
 the HIP library without reordering on the host:
 
@@ -187,6 +187,16 @@ class Prop(_Field):
         return self
 
 
+class FieldComplexD(_Field):
+    """FieldM<ComplexD, multiplicity> (field.h): (V, multiplicity) complex128,
+    e.g. the U(1) noise of set_rand_vol_u1_src."""
+
+    def __init__(self, geo=None, multiplicity=1):
+        super().__init__(geo, None)
+        if geo is not None:
+            self.data = np.zeros((geo.total_volume(), multiplicity), dtype=np.complex128)
+
+
 def get_half_fermion(ff, eo):
     """Half field of a full 5D field (dslash.cpp:542-561)."""
     assert ff.eo == 0
@@ -207,13 +217,6 @@ def set_point_src_fermion_field(ff, xg, cs, value=1.0):
     ff.data[...] = 0
     ff.data[ff.geo.index_from_coordinate(xg), cs] = value
     return ff
-
-
-def mk_point_src(geo, xg, value=1.0):
-    """Point-source propagator (propagator.pyx:201-204 -> qcd-prop.cpp:75-89)."""
-    p = Prop(geo)
-    p.data[geo.index_from_coordinate(xg)] = value * np.eye(12)
-    return p
 
 
 def fermion_field_5d_from_4d(ff4d, ls, upper=0, lower=None):

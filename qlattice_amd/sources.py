@@ -1,0 +1,149 @@
+"""Propagator sources and source propagators (qlat qcd-prop.cpp:61-530,
+qcd-prop.h:202-290, :397-420; Python surface propagator.pyx:175-234).
+
+Every source is a unit spin-colour matrix times a site factor f(x), built on
+the device by the HIP library (``qlat_dwf_make_source``):
+
+  point        f = value at x = xg                       set_point_src
+  wall         f = exp(i mom.x) on t = tslice            set_wall_src, set_tslice_mom_src
+  momentum     f = exp(i mom.x) on every site            set_mom_src, set_volume_src
+  U(1) volume  f = fu1(x)                                set_rand_vol_u1_src
+
+with mom = lmom * 2 pi / total_site (lattice_mom_mult, field.h:14-22).  The
+``*_src_propagator`` functions build the source on the device and solve its 12
+columns as one multi-RHS problem, so the source never crosses PCIe.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .fields import Coordinate, CoordinateD, FieldComplexD, Prop
+
+
+def _source(kind, xg=(0, 0, 0, 0), tslice=0, lmom=None, value=1.0, fu1=None):
+    s = _lib.SourceC()
+    s.kind = kind
+    for mu in range(4):
+        s.xg[mu] = int(xg[mu])
+        s.lmom[mu] = float(0.0 if lmom is None else lmom[mu])
+    s.tslice = int(tslice)
+    v = complex(value)
+    s.value[0], s.value[1] = v.real, v.imag
+    keep = None
+    if fu1 is not None:
+        keep = np.ascontiguousarray(fu1, dtype=np.complex128)
+        s.fu1 = keep.ctypes.data
+    return s, keep
+
+
+def _fill(prop, geo, s):
+    assert prop.data.shape == (geo.total_volume(), 12, 12)
+    if prop.data.dtype != np.complex128 or not prop.data.flags.c_contiguous:
+        prop.data = np.zeros((geo.total_volume(), 12, 12), dtype=np.complex128)
+    ts = (ctypes.c_int * 4)(*geo.total_site())
+    _lib.check(_lib.load().qlat_dwf_make_source(ts, ctypes.byref(s), _lib.ptr(prop.data)),
+               "qlat_dwf_make_source")
+    return prop
+
+
+def _prop_for(prop, geo):
+    if prop.geo is None:
+        prop.geo = geo
+    if prop.data is None or prop.data.shape != (geo.total_volume(), 12, 12):
+        prop.data = np.zeros((geo.total_volume(), 12, 12), dtype=np.complex128)
+    return prop
+
+
+# ------------------------------------------------------------------ sources
+
+def set_point_src(prop, geo, xg, value=1.0):
+    """qcd-prop.cpp:75-89: value * unit matrix at xg (zero if xg is off the lattice)."""
+    s, _ = _source(_lib.SRC_POINT, xg=Coordinate(xg), value=value)
+    return _fill(_prop_for(prop, geo), geo, s)
+
+
+def set_wall_src(prop, geo, tslice, lmom=None):
+    """qcd-prop.cpp:117-150: exp(i mom.x) unit matrix on t = tslice (lmom[3] must be 0)."""
+    s, _ = _source(_lib.SRC_WALL, tslice=tslice, lmom=CoordinateD(lmom) if lmom is not None else None)
+    return _fill(_prop_for(prop, geo), geo, s)
+
+
+def set_tslice_mom_src(prop, geo, tslice, lmom):
+    """qcd-prop.cpp:465-499 (the same field as set_wall_src)."""
+    return set_wall_src(prop, geo, tslice, lmom)
+
+
+def set_mom_src(prop, geo, lmom):
+    """qcd-prop.cpp:275-307: exp(i mom.x) unit matrix on every site."""
+    s, _ = _source(_lib.SRC_MOM, lmom=CoordinateD(lmom))
+    return _fill(_prop_for(prop, geo), geo, s)
+
+
+def set_volume_src(prop, geo, lmom):
+    """qcd-prop.cpp:501-530: set_mom_src with lmom[3] = 0 required."""
+    if CoordinateD(lmom)[3] != 0:
+        raise ValueError("set_volume_src: lmom[3] must be 0 (qcd-prop.cpp:505)")
+    return set_mom_src(prop, geo, lmom)
+
+
+def set_rand_vol_u1_src(prop, fu1):
+    """qcd-prop.cpp:260-273: fu1(x) unit matrix (fu1: FieldComplexD with multiplicity 1)."""
+    if fu1.data.shape[1:] not in ((), (1,)):
+        raise ValueError("set_rand_vol_u1_src: fu1.multiplicity must be 1")
+    s, keep = _source(_lib.SRC_U1, fu1=fu1.data.reshape(-1))
+    out = _fill(_prop_for(prop, fu1.geo), fu1.geo, s)
+    del keep
+    return out
+
+
+def mk_point_src(geo, xg, value=1.0):
+    """propagator.pyx:201-205."""
+    return set_point_src(Prop(geo), geo, xg, value)
+
+
+def mk_wall_src(geo, tslice, lmom=None):
+    """propagator.pyx:207-211."""
+    return set_wall_src(Prop(geo), geo, tslice, lmom)
+
+
+def mk_rand_vol_u1_src(fu1):
+    """propagator.pyx:227-235."""
+    return set_rand_vol_u1_src(Prop(fu1.geo), fu1)
+
+
+# ------------------------------------------------------- source propagators
+
+def _solve(prop, inv, s):
+    geo = inv.geo()
+    _prop_for(prop, geo)
+    if prop.data.dtype != np.complex128 or not prop.data.flags.c_contiguous:
+        prop.data = np.zeros((geo.total_volume(), 12, 12), dtype=np.complex128)
+    it = ctypes.c_long(0)
+    _lib.check(_lib.load().qlat_dwf_source_propagator(inv.cdata, ctypes.byref(s), _lib.ptr(prop.data),
+                                                      ctypes.byref(it)), "qlat_dwf_source_propagator")
+    return prop
+
+
+def set_point_src_propagator(prop, inv, xg, value=1.0):
+    """qcd-prop.h:202-218: the solution for set_point_src, 12 columns solved together."""
+    s, _ = _source(_lib.SRC_POINT, xg=Coordinate(xg), value=value)
+    return _solve(prop, inv, s)
+
+
+def set_wall_src_propagator(prop, inv, tslice, lmom=None):
+    """qcd-prop.h:226-243."""
+    s, _ = _source(_lib.SRC_WALL, tslice=tslice, lmom=CoordinateD(lmom) if lmom is not None else None)
+    return _solve(prop, inv, s)
+
+
+def set_tslice_mom_src_propagator(prop, inv, tslice, lmom):
+    """qcd-prop.h:403-420."""
+    return set_wall_src_propagator(prop, inv, tslice, lmom)
+
+
+def set_mom_src_propagator(prop, inv, lmom):
+    """qcd-prop.h:273-290."""
+    s, _ = _source(_lib.SRC_MOM, lmom=CoordinateD(lmom))
+    return _solve(prop, inv, s)

@@ -48,3 +48,9 @@ ZMOBIUS_OMEGA = [
     complex(0.0365221637144842, -0.03343945161367745),
     complex(0.0365221637144842, 0.03343945161367745),
 ]
+
+
+def qlat_check(v, ref, eps):
+    """The reference's golden-value test: 2|v - ref| / (|v| + |ref|) <= eps
+    (qlat_utils/utils.py:455-545)."""
+    return 2 * abs(v - ref) / (abs(v) + abs(ref)) <= eps

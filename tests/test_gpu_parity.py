@@ -15,15 +15,11 @@ import pytest
 
 import qlattice_amd as q
 from oracle import oracle as O
-from tests.helpers import ZMOBIUS_OMEGA, point_src, rand_field, rand_gauge, rel, unit_gauge
+from tests.helpers import ZMOBIUS_OMEGA, point_src, qlat_check, rand_field, rand_gauge, rel, unit_gauge
 
 pytestmark = pytest.mark.gpu
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "free_invert.json")))
-
-
-def qlat_check(v, ref, eps):
-    return 2 * abs(v - ref) / (abs(v) + abs(ref)) <= eps
 
 
 def make_pair(site, U, *, mass, ls, m5, mobius_scale=1.0, omega=None):

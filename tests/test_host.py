@@ -80,11 +80,11 @@ def test_twist_boundary_flips_last_time_slice_only():
     assert np.allclose(W[t == 3, 3], -U[t == 3, 3])
 
 
-def test_point_source_and_chiral_embedding():
+def test_point_source_fermion_field_and_chiral_embedding():
     geo = q.Geometry((2, 2, 2, 2))
-    p = q.mk_point_src(geo, q.Coordinate([1, 0, 1, 0]))
+    ff = q.set_point_src_fermion_field(q.FermionField4d(geo), q.Coordinate([1, 0, 1, 0]), 7, 2.0)
     i = geo.index_from_coordinate((1, 0, 1, 0))
-    assert np.array_equal(p.data[i], np.eye(12)) and p.qnorm() == 12
+    assert ff.data[i, 7] == 2.0 and ff.qnorm() == 4.0
     f4 = q.FermionField4d(geo)
     f4.data[:] = np.arange(12)[None, :] + 1
     f5 = q.fermion_field_5d_from_4d(f4, 6)
