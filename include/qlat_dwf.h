@@ -221,6 +221,31 @@ int qlat_dwf_make_source(const int total_site[4], const qlat_dwf_source* src, do
  * together (invert(Prop), as qlat_dwf_invert_prop); sol is [V][12][12]. */
 int qlat_dwf_source_propagator(qlat_dwf_inverter* inv, const qlat_dwf_source* src, double* sol, long* iter);
 
+/* ---------------------------------------------------------- low modes
+ * LowModes (dslash.h:43-85) held by an inverter, as InverterDomainWall::setup
+ * (gf, fa, lm) does (dslash.h:176-186), in the in-memory layout of the
+ * reference's compressed eigen system (compressed-eigen-io.h:175-195,
+ * :332-339), single node:
+ *   blocks of block_site (each entry even and dividing total_site), block b =
+ *   lexicographic index (x fastest) of x / block_site; inside a block the
+ *   half site bidx = (lexicographic index of x % block_site) / 2, and
+ *   element k = (bidx * Ls + s) * 12 + spin*3+colour (convert_half_vector,
+ *   commpressed-eigen-io.cpp:925-947); block_size = block_vol/2 * Ls * 12;
+ *   bases  [n_block][n_basis][block_size]  complex64
+ *   coefs  [n_block][n_vec][n_basis]       complex64
+ *   eigen_values [n_vec] (of multiply_hermop_sym2 on the odd sites).
+ * n_vec = 0 detaches the low modes.  With low modes attached, every restart
+ * cycle of the FP64 solver starts CG from deflate(residual) (dslash.h:496-500). */
+int qlat_dwf_inverter_set_low_modes(qlat_dwf_inverter* inv, const int block_site[4], int n_basis, int n_vec,
+                                    const double* eigen_values, const float* bases, const float* coefs);
+
+/* deflate(FermionField5d& out, const FermionField5d& in, LowModes&)
+ * (dslash.cpp:144-300) on a half field of parity eo: the input is rounded to
+ * FP32 and, as in the reference (whose blocked input buffer is also the
+ * accumulator of the produce step), out = in + sum_i v_i <v_i, in> / lambda_i;
+ * out = 0 when no low modes are attached.  out may equal in. */
+int qlat_dwf_deflate(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
+
 /* find_max_eigen_value_hermop_sym2 (dslash.h:586-588; dslash.cpp:1349-1369)
  * from a caller-supplied odd half-field start vector. */
 int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,

@@ -28,8 +28,10 @@ from .fields import (
 from .inverter import (
     Inverter,
     InverterDomainWall,
+    LowModes,
     bench_kernel,
     cg_with_herm_sym_2,
+    deflate,
     dot_product,
     find_max_eigen_value_hermop_sym2,
     invert,

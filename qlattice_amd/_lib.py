@@ -49,6 +49,8 @@ EXPORTED = [
     "qlat_dwf_invert_5d",
     "qlat_dwf_invert_4d",
     "qlat_dwf_invert_prop",
+    "qlat_dwf_inverter_set_low_modes",
+    "qlat_dwf_deflate",
     "qlat_dwf_make_source",
     "qlat_dwf_make_source_device",
     "qlat_dwf_source_propagator",
@@ -168,6 +170,9 @@ def load():
         getattr(L, name).argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_dp),
                                      ctypes.POINTER(_dp), _lp]
     _ip4 = ctypes.POINTER(ctypes.c_int)
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.qlat_dwf_inverter_set_low_modes.argtypes = [_vp, _ip4, ctypes.c_int, ctypes.c_int, _dp, _fp, _fp]
+    L.qlat_dwf_deflate.argtypes = [_vp, _dp, _dp, ctypes.c_int]
     L.qlat_dwf_make_source.argtypes = [_ip4, ctypes.POINTER(SourceC), _dp]
     L.qlat_dwf_make_source_device.argtypes = [_ip4, ctypes.POINTER(SourceC), _dp]
     L.qlat_dwf_source_propagator.argtypes = [_vp, ctypes.POINTER(SourceC), _dp, _lp]

@@ -205,6 +205,16 @@ struct qlat_dwf_inverter {
   qdwf::CgState* cg_dev = nullptr;
   qdwf::CgState* cg_host = nullptr;
 
+  // low modes (block-compressed eigen system), qlat_dwf_inverter_set_low_modes
+  int lm_bs[4] = {0, 0, 0, 0};
+  int lm_nbasis = 0, lm_nvec = 0, lm_nblock = 0, lm_bve = 0;
+  float2* lm_basis = nullptr;
+  float2* lm_coef = nullptr;
+  double* lm_evals = nullptr;
+  int* lm_hmap[2] = {nullptr, nullptr};  // odd, even
+  double2* lm_part = nullptr;
+  double2* lm_p = nullptr;
+
   // multi-RHS: FP32 stacks of `mcap` fields and per-column FP64 fields
   int rhs_tile = 0;  // columns per multi-RHS solve (QLAT_DWF_RHS_TILE; 0 = autotuned / 12)
   int mcap = 0;
@@ -1474,6 +1484,43 @@ double true_residual_fp64(Inv& I, const double2* x, const double2* b)
 
 // invert_with_cg (dslash.h:440-530) on device halves; in_e/in_o: 5D source
 // halves (FP64), results in out_e/out_o.
+qdwf::DeflArgs defl_args(Inv& I, int eo)
+{
+  qdwf::DeflArgs a{};
+  a.geo = I.geo;
+  a.ls = I.ls;
+  a.eo = eo;
+  for (int mu = 0; mu < 4; ++mu) a.bs[mu] = I.lm_bs[mu];
+  a.nblock = I.lm_nblock;
+  a.block_vol_eo = I.lm_bve;
+  a.bsize = (int64_t)I.lm_bve * I.ls * 12;
+  a.n_basis = I.lm_nbasis;
+  a.n_vec = I.lm_nvec;
+  a.basis = I.lm_basis;
+  a.coef = I.lm_coef;
+  a.evals = I.lm_evals;
+  a.hmap = I.lm_hmap[eo == 1 ? 0 : 1];
+  a.part = I.lm_part;
+  a.p = I.lm_p;
+  return a;
+}
+
+// deflate (dslash.cpp:144-300) of the device half field `in` of parity eo:
+// out = in + sum_i v_i <v_i, in> / lambda_i with FP32 rounding as in the
+// reference, or 0 without low modes.  out may equal in.
+void deflate_dev(Inv& I, double2* out, const double2* in, int eo)
+{
+  if (I.lm_nvec == 0) {
+    zero64(I, out);
+    return;
+  }
+  qdwf::DeflArgs a = defl_args(I, eo);
+  a.in = in;
+  a.out = out;
+  qdwf::launch_deflate(a, I.stream);
+  check_launch();
+}
+
 long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* out_e, double2* out_o)
 {
   double2* dm_e = W(I, W_DM_E);
@@ -1513,9 +1560,12 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
     double qnorm_itmp = qnorm_in_o_p;
     long cycle;
     for (cycle = 1; cycle <= I.max_cycle; ++cycle) {
-      zero64(I, x);
+      // deflated initial guess when low modes are attached (dslash.h:496-500)
+      const bool guess = I.lm_nvec > 0;
+      if (guess) deflate_dev(I, x, itmp, 1);
+      else zero64(I, x);
       const long iter =
-          cg_fp64(I, x, itmp, I.stop_rsd * std::sqrt(qnorm_in_o_p / qnorm_itmp), I.max_num_iter, true);
+          cg_fp64(I, x, itmp, I.stop_rsd * std::sqrt(qnorm_in_o_p / qnorm_itmp), I.max_num_iter, !guess);
       total_iter += iter;
       qdwf::launch_axpy<double2>(1.0, x, out_o_p, n64(I), I.stream);
       check_launch();
@@ -2084,6 +2134,12 @@ void free_all(Inv* I)
   if (I->red_host) (void)hipHostFree(I->red_host);
   f(I->cg_dev);
   if (I->cg_host) (void)hipHostFree(I->cg_host);
+  f(I->lm_basis);
+  f(I->lm_coef);
+  f(I->lm_evals);
+  for (auto* p : I->lm_hmap) f(p);
+  f(I->lm_part);
+  f(I->lm_p);
   if (I->stream) (void)hipStreamDestroy(I->stream);
   delete I;
 }
@@ -2532,6 +2588,81 @@ int qlat_dwf_source_propagator_device(qlat_dwf_inverter* inv, const qlat_dwf_sou
     *iter = invert_prop_dev(*inv, (double2*)sol, d_src, true);
     sync(*inv);
     inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_inverter_set_low_modes(qlat_dwf_inverter* inv, const int block_site[4], int n_basis, int n_vec,
+                                    const double* eigen_values, const float* bases, const float* coefs)
+{
+  return guarded([&] {
+    check_inv(inv);
+    Inv& I = *inv;
+    QDWF_REQUIRE(n_vec >= 0, "n_vec must be >= 0");
+    auto f = [](void*& p) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+    };
+    sync(I);
+    f((void*&)I.lm_basis);
+    f((void*&)I.lm_coef);
+    f((void*&)I.lm_evals);
+    for (auto*& p : I.lm_hmap) f((void*&)p);
+    f((void*&)I.lm_part);
+    f((void*&)I.lm_p);
+    I.lm_nvec = 0;
+    if (n_vec == 0) return;
+    QDWF_REQUIRE(block_site && eigen_values && bases && coefs, "invalid argument");
+    QDWF_REQUIRE(n_basis >= 1 && n_basis <= 2048, "n_basis must be in [1, 2048]");
+    int64_t nblock = 1, bvol = 1;
+    for (int mu = 0; mu < 4; ++mu) {
+      QDWF_REQUIRE(block_site[mu] > 0 && I.geo.n[mu] % block_site[mu] == 0, "block_site must divide total_site");
+      nblock *= I.geo.n[mu] / block_site[mu];
+      bvol *= block_site[mu];
+    }
+    QDWF_REQUIRE(block_site[0] % 2 == 0, "block_site[0] must be even (x pairs of opposite parity)");
+    for (int i = 0; i < n_vec; ++i) QDWF_REQUIRE(eigen_values[i] != 0.0, "eigen values must be non-zero");
+    for (int mu = 0; mu < 4; ++mu) I.lm_bs[mu] = block_site[mu];
+    I.lm_nblock = (int)nblock;
+    I.lm_bve = (int)(bvol / 2);
+    I.lm_nbasis = n_basis;
+    const int64_t bsize = (int64_t)I.lm_bve * I.ls * 12;
+    const size_t nb = (size_t)nblock * n_basis * bsize * sizeof(float2);
+    const size_t nc = (size_t)nblock * n_vec * n_basis * sizeof(float2);
+    I.lm_basis = (float2*)dev_alloc(nb);
+    I.lm_coef = (float2*)dev_alloc(nc);
+    I.lm_evals = (double*)dev_alloc(n_vec * sizeof(double));
+    for (auto*& p : I.lm_hmap) p = (int*)dev_alloc((size_t)nblock * I.lm_bve * sizeof(int));
+    I.lm_part = (double2*)dev_alloc((size_t)nblock * n_vec * sizeof(double2));
+    I.lm_p = (double2*)dev_alloc(n_vec * sizeof(double2));
+    QDWF_HIP(hipMemcpyAsync(I.lm_basis, bases, nb, hipMemcpyHostToDevice, I.stream));
+    QDWF_HIP(hipMemcpyAsync(I.lm_coef, coefs, nc, hipMemcpyHostToDevice, I.stream));
+    QDWF_HIP(hipMemcpyAsync(I.lm_evals, eigen_values, n_vec * sizeof(double), hipMemcpyHostToDevice, I.stream));
+    I.lm_nvec = n_vec;
+    for (int eo = 1; eo <= 2; ++eo) {
+      qdwf::launch_defl_hmap(I.lm_hmap[eo - 1], defl_args(I, eo), I.stream);
+      check_launch();
+    }
+    sync(I);
+  });
+}
+
+int qlat_dwf_deflate(qlat_dwf_inverter* inv, double* out, const double* in, int eo)
+{
+  return guarded([&] {
+    check_inv(inv);
+    check_eo(eo);
+    QDWF_REQUIRE(out && in, "null field pointer");
+    Inv& I = *inv;
+    const size_t bytes = half_aos_bytes(I);
+    const double2* d_in = upload(I, 0, in, bytes);
+    double2* f = W(I, W_TMP);
+    qdwf::launch_pack5(d_in, f, I.geo, I.ls, eo, 0, I.stream);
+    check_launch();
+    deflate_dev(I, f, f, eo);
+    double2* d_out = stage(I, 1, bytes);
+    qdwf::launch_unpack5(f, d_out, I.geo, I.ls, eo, 0, I.stream);
+    check_launch();
+    download(I, out, d_out, bytes);
   });
 }
 

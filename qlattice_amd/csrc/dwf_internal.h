@@ -201,6 +201,30 @@ struct SrcArgs {
 };
 void launch_make_src(double2* prop, const SrcArgs& a, hipStream_t st);
 
+// Block-compressed low modes (deflate, dslash.cpp:144-249).  hmap[b][bidx]
+// is the device half-site of half site bidx of block b; field elements are
+// FP64 SoA half fields of parity eo.
+struct DeflArgs {
+  LatGeo geo;
+  int ls;
+  int eo;
+  int bs[4];         // block_site
+  int nblock;
+  int block_vol_eo;
+  int64_t bsize;     // block_vol_eo * ls * 12
+  int n_basis, n_vec;
+  const float2* basis;  // [nblock][n_basis][bsize]
+  const float2* coef;   // [nblock][n_vec][n_basis]
+  const double* evals;  // [n_vec]
+  const int* hmap;      // [nblock][block_vol_eo]
+  const double2* in;
+  double2* out;
+  double2* part;        // [nblock][n_vec]
+  double2* p;           // [n_vec]
+};
+void launch_defl_hmap(int* hmap, const DeflArgs& a, hipStream_t st);
+void launch_deflate(const DeflArgs& a, hipStream_t st);
+
 // alpha, beta, |r|^2 and modes of the n column slots from the merged hermop
 // reductions red[i] = Re <p, Ap>, red[n + i] = Re <r, Ap>, red[2n + i] = |Ap|^2
 void launch_cg_scalars(CgState* cg, const double* red, int n, hipStream_t st);

@@ -905,6 +905,144 @@ void launch_make_src(double2* prop, const SrcArgs& a, hipStream_t st)
   hipLaunchKernelGGL(k_make_src, dim3((unsigned)((n + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st, prop, a);
 }
 
+// ---------------------------------------------------------------- deflation
+// The three steps of deflate (dslash.cpp:144-249) over blocks of block_site:
+// project the FP32-rounded input on the block bases (FP32 products, FP64
+// sums), inner products with the eigenvectors' coefficients summed over
+// blocks and divided by the eigenvalues, and the produce step, which adds
+// sum_j basis_j c_j (FP32) to the blocked input, as the reference does.
+
+// hmap[b][bidx]: device half-site of the bidx-th half site (parity eo) of
+// block b (convert_half_vector, commpressed-eigen-io.cpp:925-947).
+__global__ void __launch_bounds__(kBlasBlock) k_defl_hmap(int* __restrict__ hmap, DeflArgs a)
+{
+  const int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x;
+  if (i >= (int64_t)a.nblock * a.block_vol_eo) return;
+  const int b = (int)(i / a.block_vol_eo);
+  const int bidx = (int)(i - (int64_t)b * a.block_vol_eo);
+  int bc[4], l[4];
+  int r = b;
+  for (int mu = 0; mu < 4; ++mu) {
+    const int nb = a.geo.n[mu] / a.bs[mu];
+    bc[mu] = r % nb;
+    r /= nb;
+  }
+  r = 2 * bidx;  // lexicographic index in the block; bs[0] even
+  for (int mu = 0; mu < 4; ++mu) {
+    l[mu] = r % a.bs[mu];
+    r /= a.bs[mu];
+  }
+  int x[4];
+  for (int mu = 0; mu < 4; ++mu) x[mu] = bc[mu] * a.bs[mu] + l[mu];
+  const int odd = (x[0] + x[1] + x[2] + x[3]) & 1;
+  if (odd != (a.eo == 1 ? 1 : 0)) x[0] += 1;  // the other site of the x pair
+  hmap[i] = (int)half_index(a.geo, x);
+}
+
+__device__ __forceinline__ unsigned defl_slot(const DeflArgs& a, int b, int64_t k)
+{
+  const int per = a.ls * 12;
+  const int bidx = (int)(k / per);
+  const int rem = (int)(k - (int64_t)bidx * per);
+  const int s = rem / 12;
+  return sidx<12>(a.ls, s, rem - s * 12, a.hmap[(int64_t)b * a.block_vol_eo + bidx]);
+}
+
+// one workgroup per block; n_basis doubles2 of dynamic LDS
+__global__ void __launch_bounds__(kBlasBlock) k_defl_project(DeflArgs a)
+{
+  extern __shared__ double2 cj[];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j = wv; j < a.n_basis; j += kBlasBlock / 64) {
+    const float2* __restrict__ bj = a.basis + ((int64_t)b * a.n_basis + j) * a.bsize;
+    double re = 0.0, im = 0.0;
+    for (int64_t k = lane; k < a.bsize; k += 64) {
+      const float2 bk = bj[k];
+      const double2 vd = a.in[defl_slot(a, b, k)];
+      const float vx = (float)vd.x, vy = (float)vd.y;
+      // conj(b) v in FP32
+      re += (double)(bk.x * vx + bk.y * vy);
+      im += (double)(bk.x * vy - bk.y * vx);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      re += __shfl_down(re, o, 64);
+      im += __shfl_down(im, o, 64);
+    }
+    if (lane == 0) cj[j] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.n_vec; i += kBlasBlock) {
+    const float2* __restrict__ ci = a.coef + ((int64_t)b * a.n_vec + i) * a.n_basis;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < a.n_basis; ++j) {  // conj(coef) c, FP64
+      const double cx = ci[j].x, cy = ci[j].y;
+      re += cx * cj[j].x + cy * cj[j].y;
+      im += cx * cj[j].y - cy * cj[j].x;
+    }
+    a.part[(int64_t)b * a.n_vec + i] = make_double2(re, im);
+  }
+}
+
+// p[i] = sum over blocks (block order) / lambda_i
+__global__ void __launch_bounds__(kBlasBlock) k_defl_reduce(DeflArgs a)
+{
+  for (int i = threadIdx.x; i < a.n_vec; i += kBlasBlock) {
+    double re = 0.0, im = 0.0;
+    for (int b = 0; b < a.nblock; ++b) {
+      const double2 v = a.part[(int64_t)b * a.n_vec + i];
+      re += v.x;
+      im += v.y;
+    }
+    a.p[i] = make_double2(re / a.evals[i], im / a.evals[i]);
+  }
+}
+
+// one workgroup per block; n_basis float2 of dynamic LDS
+__global__ void __launch_bounds__(kBlasBlock) k_defl_produce(DeflArgs a)
+{
+#pragma clang fp contract(off)
+  extern __shared__ float2 cf[];
+  const int b = blockIdx.x;
+  for (int j = threadIdx.x; j < a.n_basis; j += kBlasBlock) {
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < a.n_vec; ++i) {  // coef p, FP64, i in order
+      const float2 c = a.coef[((int64_t)b * a.n_vec + i) * a.n_basis + j];
+      const double2 pi = a.p[i];
+      re += (double)c.x * pi.x - (double)c.y * pi.y;
+      im += (double)c.x * pi.y + (double)c.y * pi.x;
+    }
+    cf[j] = make_float2((float)re, (float)im);
+  }
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < a.bsize; k += kBlasBlock) {
+    const unsigned slot = defl_slot(a, b, k);
+    const double2 vd = a.in[slot];
+    float vx = (float)vd.x, vy = (float)vd.y;
+    for (int j = 0; j < a.n_basis; ++j) {  // v += basis_j c_j, FP32, j in order
+      const float2 bk = a.basis[((int64_t)b * a.n_basis + j) * a.bsize + k];
+      const float2 c = cf[j];
+      vx += bk.x * c.x - bk.y * c.y;
+      vy += bk.x * c.y + bk.y * c.x;
+    }
+    a.out[slot] = make_double2((double)vx, (double)vy);
+  }
+}
+
+void launch_defl_hmap(int* hmap, const DeflArgs& a, hipStream_t st)
+{
+  const int64_t n = (int64_t)a.nblock * a.block_vol_eo;
+  hipLaunchKernelGGL(k_defl_hmap, dim3((unsigned)((n + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st, hmap, a);
+}
+
+void launch_deflate(const DeflArgs& a, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_defl_project, dim3(a.nblock), dim3(kBlasBlock), a.n_basis * sizeof(double2), st, a);
+  hipLaunchKernelGGL(k_defl_reduce, dim3(1), dim3(kBlasBlock), 0, st, a);
+  hipLaunchKernelGGL(k_defl_produce, dim3(a.nblock), dim3(kBlasBlock), a.n_basis * sizeof(float2), st, a);
+}
+
 // r_i -= alpha_i ap_i over 5D sites (ap in storage ST, r in SR) with
 // per-column partial |r_i|^2 of the stored r (part[i*gridDim.x + block]).
 template <class ST, class SR>

@@ -113,6 +113,27 @@ class InverterDomainWall(Inverter):
                    "qlat_dwf_inverter_set_solver")
         self.solver = solver
 
+    def set_low_modes(self, lm):
+        """Attach low modes (InverterDomainWall::setup(gf, fa, lm),
+        dslash.h:176-186); None detaches.  The FP64 solver then starts each
+        restart cycle from deflate(residual) (dslash.h:496-500)."""
+        L = _lib.load()
+        if lm is None:
+            _lib.check(L.qlat_dwf_inverter_set_low_modes(self.cdata, None, 0, 0, None, None, None),
+                       "qlat_dwf_inverter_set_low_modes")
+            self.lm = None
+            return
+        bs = (ctypes.c_int * 4)(*lm.block_site)
+        ev = np.ascontiguousarray(lm.eigen_values, dtype=np.float64)
+        bases = np.ascontiguousarray(lm.bases, dtype=np.complex64)
+        coefs = np.ascontiguousarray(lm.coefs, dtype=np.complex64)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _lib.check(L.qlat_dwf_inverter_set_low_modes(self.cdata, bs, int(lm.n_basis), int(lm.n_vec),
+                                                     ev.ctypes.data_as(_lib._dp), bases.ctypes.data_as(fp),
+                                                     coefs.ctypes.data_as(fp)),
+                   "qlat_dwf_inverter_set_low_modes")
+        self.lm = lm
+
     def set_gauge(self, gf):
         """Replace the links (InverterDomainWall::setup, dslash.h:154-162)."""
         from .force import set_gauge
@@ -232,6 +253,40 @@ def multiply_mpcdag_sym2(out, in_, inv):
 def multiply_hermop_sym2(out, in_, inv):
     """dslash.cpp:1140-1149."""
     return _apply_half("qlat_dwf_multiply_hermop_sym2", out, in_, inv, in_.eo, in_.eo)
+
+
+class LowModes:
+    """LowModes (dslash.h:43-85): a block-compressed eigen system of
+    multiply_hermop_sym2 on the odd sites, in the reference's in-memory
+    layout (compressed-eigen-io.h:175-195):
+
+      block_site    4 even-x block extents dividing total_site
+      eigen_values  (n_vec,) float64
+      bases         (n_block, n_basis, block_vol/2 * Ls * 12) complex64
+      coefs         (n_block, n_vec, n_basis) complex64
+
+    eigenvector i restricted to block b = sum_j coefs[b, i, j] bases[b, j];
+    blocks are ordered lexicographically (x fastest), and inside a block the
+    half site (lexicographic index of x % block_site) / 2 comes first, then
+    s, then spin*3+colour (convert_half_vector, commpressed-eigen-io.cpp:925-947).
+    """
+
+    def __init__(self, block_site, eigen_values, bases, coefs):
+        self.block_site = tuple(int(v) for v in block_site)
+        self.eigen_values = np.asarray(eigen_values, dtype=np.float64)
+        self.bases = np.asarray(bases, dtype=np.complex64)
+        self.coefs = np.asarray(coefs, dtype=np.complex64)
+        self.n_vec = len(self.eigen_values)
+        self.n_basis = self.bases.shape[1]
+        assert self.coefs.shape == (self.bases.shape[0], self.n_vec, self.n_basis)
+
+
+def deflate(out, in_, inv):
+    """deflate(FermionField5d&, const FermionField5d&, LowModes&)
+    (dslash.cpp:251-300) with the inverter's low modes: out = in + sum_i v_i
+    <v_i, in> / lambda_i with the input rounded to FP32 (the reference's
+    produce step accumulates into the blocked input), or 0 without low modes."""
+    return _apply_half("qlat_dwf_deflate", out, in_, inv, in_.eo, in_.eo)
 
 
 def multiply_hermop_sym2_fp32(out, in_, inv):

@@ -1,0 +1,146 @@
+"""Low modes and deflation (dslash.h:43-112, dslash.cpp:144-300, the deflated
+restart cycles of invert_with_cg, dslash.h:496-500).
+
+The low modes are exact: the lowest eigenvectors of the oracle's
+multiply_hermop_sym2 on the odd sites of a 4x4x2x4 Ls4 lattice, from numpy's
+dense Hermitian eigensolver, compressed per 2^4 block by a QR factorisation
+(n_basis = n_vec, so the compression is exact up to complex64 rounding).
+Pins: deflate equals the three steps of the reference restated in float64 on
+the same complex64 data; its deflated part equals sum_i v_i <v_i, in> /
+lambda_i of the exact eigenvectors; the deflated FP64 solve reaches the
+oracle's solution in fewer iterations.
+"""
+
+import numpy as np
+import pytest
+
+import qlattice_amd as q
+from oracle import oracle as O
+from tests.helpers import rand_field, rand_gauge, rel
+
+pytestmark = pytest.mark.gpu
+
+SITE = (4, 4, 2, 4)
+LS = 4
+BS = (2, 2, 2, 2)
+NVEC = 24
+MASS = 0.02
+
+
+def block_order(site, bs, eo=1):
+    """half index (reference order) of element rows: [block][bidx] in the
+    compressed layout (convert_half_vector, commpressed-eigen-io.cpp:925-947)."""
+    nb = [site[m] // bs[m] for m in range(4)]
+    bve = int(np.prod(bs)) // 2
+    rows = []
+    for b in range(int(np.prod(nb))):
+        bc, r = [], b
+        for m in range(4):
+            bc.append(r % nb[m])
+            r //= nb[m]
+        blk = []
+        for bidx in range(bve):
+            r, lc = 2 * bidx, []
+            for m in range(4):
+                lc.append(r % bs[m])
+                r //= bs[m]
+            x = [bc[m] * bs[m] + lc[m] for m in range(4)]
+            if (sum(x) & 1) != (1 if eo == 1 else 0):
+                x[0] += 1
+            full = x[0] + site[0] * (x[1] + site[1] * (x[2] + site[2] * x[3]))
+            blk.append(full // 2)
+        rows.append(blk)
+    return np.array(rows)
+
+
+@pytest.fixture(scope="module")
+def system():
+    print("\n[gpu] deflation: dense eigensystem of hermop 4x4x2x4 Ls4", flush=True)
+    U = rand_gauge(SITE, 0.3, 81)
+    orc = O.OracleInverter(SITE, U, mass=MASS, ls=LS, m5=1.8, mobius_scale=2.0)
+    nh = int(np.prod(SITE)) // 2
+    dim = nh * LS * 12
+    A = np.empty((dim, dim), np.complex128)
+    e = np.zeros(dim, np.complex128)
+    for k in range(dim):
+        e[k] = 1.0
+        A[:, k] = orc.apply_half("hermop_sym2", e.reshape(nh, LS, 12), 1).reshape(-1)
+        e[k] = 0.0
+    A = 0.5 * (A + A.conj().T)
+    w, V = np.linalg.eigh(A)
+    lam, vec = w[:NVEC], V[:, :NVEC]  # (dim, NVEC)
+    # compress per block: eigenvector rows of block b in the layout order
+    rows = block_order(SITE, BS)
+    nblock, bve = rows.shape
+    idx = (rows[:, :, None, None] * LS * 12 + np.arange(LS)[None, None, :, None] * 12
+           + np.arange(12)[None, None, None, :]).reshape(nblock, -1)
+    bases = np.empty((nblock, NVEC, idx.shape[1]), np.complex64)
+    coefs = np.empty((nblock, NVEC, NVEC), np.complex64)
+    for b in range(nblock):
+        Qb, Rb = np.linalg.qr(vec[idx[b]])  # (bsize, NVEC), (NVEC, NVEC)
+        bases[b] = Qb.T
+        coefs[b] = Rb.T  # coefs[b, i, j] = R[j, i]
+    lm = q.LowModes(BS, lam, bases, coefs)
+    return U, orc, lm, lam, vec, idx
+
+
+def make_inv(U):
+    geo = q.Geometry(SITE)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    fa = q.FermionAction(mass=MASS, ls=LS, m5=1.8, mobius_scale=2.0)
+    return geo, q.InverterDomainWall(gf=gf, fa=fa)
+
+
+def restated_deflate(x, lm, idx):
+    """dslash.cpp:144-249 in float64 on the complex64 data (input rounded to
+    complex64; the produce step adds to the blocked input)."""
+    v = x.reshape(-1).astype(np.complex64).astype(np.complex128)
+    B = lm.bases.astype(np.complex128)
+    C = lm.coefs.astype(np.complex128)
+    vb = v[idx]  # (nblock, bsize)
+    c = np.einsum("bjk,bk->bj", B.conj(), vb)
+    p = np.einsum("bij,bj->i", C.conj(), c) / lm.eigen_values
+    c2 = np.einsum("bij,i->bj", C, p)
+    out = v.copy()
+    out[idx] = vb + np.einsum("bjk,bj->bk", B, c2)
+    return out.reshape(x.shape)
+
+
+def test_deflate_matches_reference_steps_and_projector(system):
+    U, orc, lm, lam, vec, idx = system
+    geo, inv = make_inv(U)
+    nh = geo.total_volume() // 2
+    x = rand_field((nh, LS, 12), 82)
+    f = q.FermionField5d(geo, LS, 1)
+    f.data = x
+    zero = q.deflate(None, f, inv).data
+    assert not zero.any()  # no low modes attached: out = 0 (dslash.cpp:147-152)
+    inv.set_low_modes(lm)
+    got = q.deflate(None, f, inv).data
+    assert rel(got, restated_deflate(x, lm, idx)) < 2e-6
+    x32 = x.astype(np.complex64).astype(np.complex128)
+    proj = (vec @ ((vec.conj().T @ x.reshape(-1)) / lam)).reshape(x.shape)
+    assert rel(got - x32, proj) < 1e-5
+    inv.set_low_modes(None)
+    assert not q.deflate(None, f, inv).data.any()
+
+
+def test_deflated_fp64_solve(system):
+    U, orc, lm, lam, vec, idx = system
+    geo, inv = make_inv(U)
+    src = q.FermionField4d(geo)
+    src.data = rand_field((geo.total_volume(), 12), 83)
+    inv.set_stop_rsd(1e-10)
+    inv.set_max_num_iter(5000)
+    plain = inv * src
+    it_plain = inv.stats()["total_iter"]
+    inv.set_low_modes(lm)
+    defl = inv * src
+    it_defl = inv.stats()["total_iter"]
+    orc.set_params(1e-12, 5000, 300)
+    ref, _ = orc.invert_4d(src.data)
+    print(f"[gpu] deflated solve: {it_defl} iterations against {it_plain}; lambda {lam[0]:.3e}..{lam[-1]:.3e}",
+          flush=True)
+    assert rel(defl.data, ref) < 1e-8 and rel(plain.data, ref) < 1e-8
+    assert it_defl < it_plain
