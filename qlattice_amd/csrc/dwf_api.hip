@@ -921,6 +921,43 @@ long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
 // FP32; 1: p, A p, t = K p and the operator temporaries b, y in FP16 with a
 // per-5D-site scale, r and the increment x in FP32; 2: r and x in FP16 too
 // (measured to double the iterations at 24^3x64: an option, not the default).
+qdwf::DeflArgs defl_args(Inv& I, int eo)
+{
+  qdwf::DeflArgs a{};
+  a.geo = I.geo;
+  a.ls = I.ls;
+  a.eo = eo;
+  for (int mu = 0; mu < 4; ++mu) a.bs[mu] = I.lm_bs[mu];
+  a.nblock = I.lm_nblock;
+  a.block_vol_eo = I.lm_bve;
+  a.bsize = (int64_t)I.lm_bve * I.ls * 12;
+  a.n_basis = I.lm_nbasis;
+  a.n_vec = I.lm_nvec;
+  a.basis = I.lm_basis;
+  a.coef = I.lm_coef;
+  a.evals = I.lm_evals;
+  a.hmap = I.lm_hmap[eo == 1 ? 0 : 1];
+  a.part = I.lm_part;
+  a.p = I.lm_p;
+  return a;
+}
+
+// deflate (dslash.cpp:144-300) of the device half field `in` of parity eo:
+// out = in + sum_i v_i <v_i, in> / lambda_i with FP32 rounding as in the
+// reference, or 0 without low modes.  out may equal in.
+void deflate_dev(Inv& I, double2* out, const double2* in, int eo)
+{
+  if (I.lm_nvec == 0) {
+    zero64(I, out);
+    return;
+  }
+  qdwf::DeflArgs a = defl_args(I, eo);
+  a.in = in;
+  a.out = out;
+  qdwf::launch_deflate(a, I.stream);
+  check_launch();
+}
+
 struct ThreePhase {
   Inv& I;
   int storage;
@@ -1197,20 +1234,6 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
   std::vector<double> pend(ncol, 0.0);  // alpha of a direction not yet added to x
   std::vector<int> active;
   const double target = stop_rsd * stop_rsd;
-  for (int c = 0; c < ncol; ++c) {
-    b2[c] = dev_norm2(I, b64[c]);
-    bn[c] = std::sqrt(b2[c]);
-    zero64(I, x64[c]);
-    if (b2[c] == 0.0) continue;  // x = 0 exactly
-    tp.set_r(b64[c], c, 1.0 / bn[c]);  // x = 0: r = b
-    rr[c] = maxrr[c] = 1.0;
-    if (rr[c] > target) active.push_back(c);
-  }
-  tp.zero_p(ncol);
-  tp.zero_x(0, ncol);
-  const std::vector<double> zero(qdwf::kMaxRhs, 0.0);
-  if (!active.empty()) tp.step(active, zero.data(), zero.data(), false);  // p = r, t = K p
-  const double d2 = I.delta * I.delta;
   // |b - A x64|^2 / |b|^2 with r64 = b - A x64
   auto true_residual = [&](int c) {
     op_hermop<double>(*tp.colinv[c], ax64, x64[c], 1);
@@ -1220,6 +1243,37 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
     check_launch();
     return dev_norm2(I, r64) / b2[c];
   };
+  for (int c = 0; c < ncol; ++c) {
+    b2[c] = dev_norm2(I, b64[c]);
+    bn[c] = std::sqrt(b2[c]);
+    zero64(I, x64[c]);
+    if (b2[c] == 0.0) continue;  // x = 0 exactly
+    const Inv& J = *tp.colinv[c];
+    if (J.lm_nvec > 0) {
+      // deflated initial guess (dslash.h:496-500), r = b - A x in FP64
+      qdwf::DeflArgs a = defl_args(const_cast<Inv&>(J), 1);
+      a.in = b64[c];
+      a.out = x64[c];
+      qdwf::launch_deflate(a, I.stream);
+      check_launch();
+      rr[c] = maxrr[c] = true_residual(c);
+      if (rr[c] <= target) {
+        const double rel = std::sqrt(rr[c]);
+        if (rel > I.stats.true_residual) I.stats.true_residual = rel;
+        continue;
+      }
+      tp.set_r(r64, c, 1.0 / bn[c]);
+    } else {
+      tp.set_r(b64[c], c, 1.0 / bn[c]);  // x = 0: r = b
+      rr[c] = maxrr[c] = 1.0;
+    }
+    if (rr[c] > target) active.push_back(c);
+  }
+  tp.zero_p(ncol);
+  tp.zero_x(0, ncol);
+  const std::vector<double> zero(qdwf::kMaxRhs, 0.0);
+  if (!active.empty()) tp.step(active, zero.data(), zero.data(), false);  // p = r, t = K p
+  const double d2 = I.delta * I.delta;
   double dots[3 * qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs];
   const bool merged = tp.merged();
   long it = 0;
@@ -1484,43 +1538,6 @@ double true_residual_fp64(Inv& I, const double2* x, const double2* b)
 
 // invert_with_cg (dslash.h:440-530) on device halves; in_e/in_o: 5D source
 // halves (FP64), results in out_e/out_o.
-qdwf::DeflArgs defl_args(Inv& I, int eo)
-{
-  qdwf::DeflArgs a{};
-  a.geo = I.geo;
-  a.ls = I.ls;
-  a.eo = eo;
-  for (int mu = 0; mu < 4; ++mu) a.bs[mu] = I.lm_bs[mu];
-  a.nblock = I.lm_nblock;
-  a.block_vol_eo = I.lm_bve;
-  a.bsize = (int64_t)I.lm_bve * I.ls * 12;
-  a.n_basis = I.lm_nbasis;
-  a.n_vec = I.lm_nvec;
-  a.basis = I.lm_basis;
-  a.coef = I.lm_coef;
-  a.evals = I.lm_evals;
-  a.hmap = I.lm_hmap[eo == 1 ? 0 : 1];
-  a.part = I.lm_part;
-  a.p = I.lm_p;
-  return a;
-}
-
-// deflate (dslash.cpp:144-300) of the device half field `in` of parity eo:
-// out = in + sum_i v_i <v_i, in> / lambda_i with FP32 rounding as in the
-// reference, or 0 without low modes.  out may equal in.
-void deflate_dev(Inv& I, double2* out, const double2* in, int eo)
-{
-  if (I.lm_nvec == 0) {
-    zero64(I, out);
-    return;
-  }
-  qdwf::DeflArgs a = defl_args(I, eo);
-  a.in = in;
-  a.out = out;
-  qdwf::launch_deflate(a, I.stream);
-  check_launch();
-}
-
 long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* out_e, double2* out_o)
 {
   double2* dm_e = W(I, W_DM_E);

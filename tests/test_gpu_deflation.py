@@ -2,13 +2,14 @@
 restart cycles of invert_with_cg, dslash.h:496-500).
 
 The low modes are exact: the lowest eigenvectors of the oracle's
-multiply_hermop_sym2 on the odd sites of a 4x4x2x4 Ls4 lattice, from numpy's
+multiply_hermop_sym2 on the odd sites of a 4x4x2x4 Ls6 lattice, from numpy's
 dense Hermitian eigensolver, compressed per 2^4 block by a QR factorisation
 (n_basis = n_vec, so the compression is exact up to complex64 rounding).
 Pins: deflate equals the three steps of the reference restated in float64 on
 the same complex64 data; its deflated part equals sum_i v_i <v_i, in> /
 lambda_i of the exact eigenvectors; the deflated FP64 solve reaches the
-oracle's solution in fewer iterations.
+oracle's solution in fewer iterations, and so do the mixed FP32 and FP16-storage
+solvers started from the deflated guess.
 """
 
 import numpy as np
@@ -21,7 +22,7 @@ from tests.helpers import rand_field, rand_gauge, rel
 pytestmark = pytest.mark.gpu
 
 SITE = (4, 4, 2, 4)
-LS = 4
+LS = 6
 BS = (2, 2, 2, 2)
 NVEC = 24
 MASS = 0.02
@@ -55,7 +56,7 @@ def block_order(site, bs, eo=1):
 
 @pytest.fixture(scope="module")
 def system():
-    print("\n[gpu] deflation: dense eigensystem of hermop 4x4x2x4 Ls4", flush=True)
+    print("\n[gpu] deflation: dense eigensystem of hermop 4x4x2x4 Ls6", flush=True)
     U = rand_gauge(SITE, 0.3, 81)
     orc = O.OracleInverter(SITE, U, mass=MASS, ls=LS, m5=1.8, mobius_scale=2.0)
     nh = int(np.prod(SITE)) // 2
@@ -144,3 +145,28 @@ def test_deflated_fp64_solve(system):
           flush=True)
     assert rel(defl.data, ref) < 1e-8 and rel(plain.data, ref) < 1e-8
     assert it_defl < it_plain
+
+
+@pytest.mark.parametrize("solver", ["mixed16", "mixed"])
+def test_deflated_mixed_solve(system, solver):
+    """The mixed solvers start from x = deflate(b) with the FP64 residual
+    b - A x, then iterate as without low modes."""
+    U, orc, lm, lam, vec, idx = system
+    geo, inv = make_inv(U)
+    inv.set_solver(solver)
+    src = q.FermionField4d(geo)
+    src.data = rand_field((geo.total_volume(), 12), 84)
+    inv.set_stop_rsd(1e-9)
+    inv.set_max_num_iter(5000)
+    plain = inv * src
+    st_plain = inv.stats()
+    inv.set_low_modes(lm)
+    defl = inv * src
+    st = inv.stats()
+    orc.set_params(1e-12, 5000, 300)
+    ref, _ = orc.invert_4d(src.data)
+    print(f"[gpu] deflated {solver} solve: {st['total_iter']} iterations against {st_plain['total_iter']}",
+          flush=True)
+    assert st["true_residual"] <= 1e-9 and st_plain["true_residual"] <= 1e-9
+    assert rel(defl.data, ref) < 1e-6 and rel(plain.data, ref) < 1e-6
+    assert st["total_iter"] < st_plain["total_iter"]
