@@ -55,6 +55,15 @@ from .inverter import (
 )
 from ._lib import QlatDwfError, load as load_library
 from .force import fermion_force, pseudofermion_action, set_gauge
+from .gauge_io import (
+    GaugeFieldInfo,
+    gf_avg_link_trace,
+    gf_avg_plaq,
+    load_gauge_field,
+    read_gauge_field_header,
+    save_gauge_field,
+    unitarize,
+)
 from .sources import (
     mk_point_src,
     mk_rand_vol_u1_src,

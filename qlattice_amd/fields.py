@@ -151,6 +151,18 @@ class GaugeField(_Field):
 
         return {"plaq": plaquette(self.data, self.geo.total_site())}
 
+    def save(self, path):
+        """NERSC format (save_gauge_field, qcd.h:143-179); returns bytes written."""
+        from .gauge_io import save_gauge_field
+
+        return save_gauge_field(self, path)
+
+    def load(self, path):
+        """NERSC format (load_gauge_field, qcd.h:181-270); returns bytes read."""
+        from .gauge_io import load_gauge_field
+
+        return load_gauge_field(self, path)
+
 
 class FermionField4d(_Field):
     def __init__(self, geo=None):

@@ -1,0 +1,89 @@
+"""NERSC gauge configuration I/O (qcd.h:34-270) on the CPU: round trips of
+the two-row format, the three-row and little-endian variants, the reference's
+warning on checksum mismatch, and the header's plaquette and link trace
+(pinned by a pure-gauge configuration, whose plaquette is exactly 1)."""
+
+import warnings
+
+import numpy as np
+import pytest
+
+import qlattice_amd as q
+from qlattice_amd.gauge import rand_gauge_links
+
+
+def random_su3(n, rng):
+    z = rng.standard_normal((n, 3, 3)) + 1j * rng.standard_normal((n, 3, 3))
+    qm, r = np.linalg.qr(z)
+    qm = qm * (np.diagonal(r, axis1=-2, axis2=-1) / np.abs(np.diagonal(r, axis1=-2, axis2=-1)))[:, None, :]
+    return qm / np.linalg.det(qm)[:, None, None] ** (1 / 3)
+
+
+def test_round_trip_two_row_format(tmp_path):
+    site = (4, 4, 2, 6)
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo)
+    gf.data = rand_gauge_links(site, 0.5, 3)
+    path = str(tmp_path / "cfg.nersc")
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        n = gf.save(path)
+        assert n == geo.total_volume() * 4 * 6 * 16
+        g2 = q.GaugeField()
+        assert g2.load(path) == n
+    assert g2.geo.total_site() == site
+    assert np.max(np.abs(g2.data - gf.data)) < 1e-13  # third rows rebuilt by unitarize
+    h = q.read_gauge_field_header(path)
+    assert h.datatype == "4D_SU3_GAUGE" and h.floating_point == "IEEE64BIG"
+    assert abs(h.plaq - q.gf_avg_plaq(gf)) < 1e-12 and abs(h.trace - q.gf_avg_link_trace(gf)) < 1e-12
+    assert open(path, "rb").read().startswith(b"BEGIN_HEADER\nHDR_VERSION = 1.0\nDATATYPE = 4D_SU3_GAUGE\n")
+
+
+def test_unit_and_pure_gauge_plaquette():
+    site = (4, 2, 4, 4)
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo).set_unit()
+    assert q.gf_avg_plaq(gf) == pytest.approx(1.0, abs=1e-15)
+    assert q.gf_avg_link_trace(gf) == pytest.approx(1.0, abs=1e-15)
+    # U_mu(x) = g(x) g(x + mu)^+: every plaquette is the identity
+    rng = np.random.default_rng(7)
+    g = random_su3(geo.total_volume(), rng).reshape(site[3], site[2], site[1], site[0], 3, 3)
+    U = np.empty((site[3], site[2], site[1], site[0], 4, 3, 3), np.complex128)
+    for mu in range(4):
+        gp = np.roll(g, -1, axis=3 - mu)
+        U[..., mu, :, :] = g @ np.conj(np.swapaxes(gp, -1, -2))
+    gf.data = U.reshape(-1, 4, 3, 3)
+    assert q.gf_avg_plaq(gf) == pytest.approx(1.0, abs=1e-13)
+    assert abs(q.gf_avg_link_trace(gf)) < 0.5
+
+
+def test_three_row_little_endian_file_and_errors(tmp_path):
+    site = (2, 2, 2, 4)
+    geo = q.Geometry(site)
+    U = rand_gauge_links(site, 0.4, 9)
+    data = U.astype("<c16").tobytes()
+    import zlib
+
+    words = np.frombuffer(data, dtype="<u4")
+    head = ("BEGIN_HEADER\nDATATYPE = 4D_SU3_GAUGE_3x3\n"
+            + "".join(f"DIMENSION_{m + 1} = {site[m]}\n" for m in range(4))
+            + f"CHECKSUM = {int(words.sum(dtype=np.uint64)) & 0xFFFFFFFF:08x}\n"
+            + f"CRC32HASH = {zlib.crc32(data):08x}\nFLOATING_POINT = IEEE64LITTLE\nEND_HEADER\n")
+    path = str(tmp_path / "cfg3x3")
+    with open(path, "wb") as f:
+        f.write(head.encode() + data)
+    gf = q.GaugeField()
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        assert gf.load(path) == len(data)
+    assert np.array_equal(gf.data, U)
+    # a corrupted byte: the reference warns and still loads (qcd.h:206-226)
+    raw = bytearray(open(path, "rb").read())
+    raw[-5] ^= 0x40
+    open(path, "wb").write(bytes(raw))
+    with pytest.warns(UserWarning, match="CHECKSUM"):
+        gf.load(path)
+    bad = str(tmp_path / "bad")
+    open(bad, "wb").write(head.replace("4D_SU3_GAUGE_3x3", "4D_U1").encode() + data)
+    with pytest.raises(ValueError):
+        gf.load(bad)
