@@ -246,6 +246,28 @@ int qlat_dwf_inverter_set_low_modes(qlat_dwf_inverter* inv, const int block_site
  * out = 0 when no low modes are attached.  out may equal in. */
 int qlat_dwf_deflate(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
 
+/* ------------------------------------------------------------------ QED
+ * The QED domain-wall operator and its solvers (qed.h:433-463,
+ * qed.cpp:119-411): full (not even-odd) 5D fields [V][ls][4] complex128 in
+ * reference site order, U(1) links gf1 [V][4] complex128 (any non-zero
+ * complex values: the backward link is 1/gf1(x - mu)), optional Wick phase
+ * factors wick[T] complex128 (NULL: all 1), periodic in every direction.
+ * Stateless: each call moves its operands to the device and back. */
+/* multiply_m_dwf_qed (qed.cpp:263-411); is_dagger selects M^+.  out != in. */
+int qlat_dwf_multiply_m_dwf_qed(const int total_site[4], int ls, const double* gf1, double mass, double m5,
+                                const double* wick, int is_dagger, double* out, const double* in);
+/* cg_with_m_dwf_qed (qed.cpp:178-261): M^+ M out = in (M M^+ when
+ * is_dagger) from the initial guess in out; iter = iterations or
+ * max_num_iter + 1. */
+int qlat_dwf_cg_with_m_dwf_qed(const int total_site[4], int ls, const double* gf1, double mass, double m5,
+                               const double* wick, int is_dagger, double stop_rsd, long max_num_iter, double* out,
+                               const double* in, long* iter);
+/* invert_dwf_qed (qed.cpp:119-150): 4D fields [V][4]; M out = in (M^+ when
+ * is_dagger) through the 5D solve with the chiral 4D <-> 5D embedding. */
+int qlat_dwf_invert_dwf_qed(const int total_site[4], int ls, const double* gf1, double mass, double m5,
+                            const double* wick, int is_dagger, double stop_rsd, long max_num_iter, double* out4d,
+                            const double* in4d, long* iter);
+
 /* find_max_eigen_value_hermop_sym2 (dslash.h:586-588; dslash.cpp:1349-1369)
  * from a caller-supplied odd half-field start vector. */
 int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,

@@ -64,6 +64,7 @@ from .gauge_io import (
     save_gauge_field,
     unitarize,
 )
+from .qed import cg_with_m_dwf_qed, invert_dwf_qed, invert_qed, multiply_m_dwf_qed
 from .sources import (
     mk_point_src,
     mk_rand_vol_u1_src,

@@ -2608,6 +2608,172 @@ int qlat_dwf_source_propagator_device(qlat_dwf_inverter* inv, const qlat_dwf_sou
   });
 }
 
+namespace {
+
+// device memory freed at scope exit
+struct DevBuf {
+  void* p = nullptr;
+  explicit DevBuf(size_t bytes) : p(dev_alloc(bytes)) {}
+  ~DevBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  double2* d() const { return (double2*)p; }
+};
+
+// A QED call's operands in device memory (null stream, synchronous copies).
+struct QedCall {
+  qdwf::QedArgs a{};
+  int64_t vol = 0, n = 0;  // sites, complex entries of a 5D field
+  std::unique_ptr<DevBuf> gf, wick;
+  QedCall(const int total_site[4], int ls, const double* gf1, double mass, double m5, const double* wick_h)
+  {
+    QDWF_REQUIRE(total_site && gf1, "invalid argument");
+    QDWF_REQUIRE(ls > 0, "ls must be positive (qed.cpp:297)");
+    vol = 1;
+    for (int mu = 0; mu < 4; ++mu) {
+      QDWF_REQUIRE(total_site[mu] > 0, "total_site must be positive");
+      a.n[mu] = total_site[mu];
+      vol *= total_site[mu];
+    }
+    n = vol * ls * 4;
+    a.ls = ls;
+    a.mass = mass;
+    a.m5 = m5;
+    gf.reset(new DevBuf(vol * 4 * sizeof(double2)));
+    QDWF_HIP(hipMemcpy(gf->p, gf1, vol * 4 * sizeof(double2), hipMemcpyHostToDevice));
+    a.gf = gf->d();
+    if (wick_h) {
+      wick.reset(new DevBuf(total_site[3] * sizeof(double2)));
+      QDWF_HIP(hipMemcpy(wick->p, wick_h, total_site[3] * sizeof(double2), hipMemcpyHostToDevice));
+      a.wick = wick->d();
+    }
+  }
+  void apply(double2* out, const double2* in, bool dagger) const
+  {
+    qdwf::QedArgs b = a;
+    b.in = in;
+    b.out = out;
+    b.dagger = dagger ? 1 : 0;
+    qdwf::launch_qed_m(b, nullptr);
+    check_launch();
+  }
+};
+
+double qed_norm2(const double2* x, int64_t n, double* part, double* red)
+{
+  qdwf::launch_norm2<double2>(x, n, part, red, nullptr);
+  double v = 0.0;
+  QDWF_HIP(hipMemcpy(&v, red, sizeof(double), hipMemcpyDeviceToHost));
+  return v;
+}
+
+// cg_with_m_dwf_qed (qed.cpp:178-261): M_{!d} M_{d} x = b from the guess in x
+long qed_cg(const QedCall& q, double2* x, const double2* b, bool dagger, double stop_rsd, long max_num_iter)
+{
+  QDWF_REQUIRE(stop_rsd >= 0.0 && max_num_iter >= 0, "invalid stop_rsd or max_num_iter");
+  if (max_num_iter == 0) return 0;
+  const int64_t n = q.n;
+  const size_t fb = n * sizeof(double2);
+  DevBuf r(fb), p(fb), ap(fb), tmp(fb), part(2 * qdwf::kMaxBlasGrid * sizeof(double)), red(2 * sizeof(double));
+  auto A = [&](double2* out, const double2* in) {
+    q.apply(tmp.d(), in, dagger);
+    q.apply(out, tmp.d(), !dagger);
+  };
+  QDWF_HIP(hipMemcpy(r.p, b, fb, hipMemcpyDeviceToDevice));
+  A(ap.d(), x);
+  qdwf::launch_axpy<double2>(-1.0, ap.d(), r.d(), n, nullptr);
+  check_launch();
+  QDWF_HIP(hipMemcpy(p.p, r.p, fb, hipMemcpyDeviceToDevice));
+  const double qnorm_in = qed_norm2(b, n, (double*)part.p, (double*)red.p);
+  double qnorm_r = qed_norm2(r.d(), n, (double*)part.p, (double*)red.p);
+  for (long iter = 1; iter <= max_num_iter; ++iter) {
+    A(ap.d(), p.d());
+    qdwf::launch_dot<double2>(p.d(), ap.d(), n, (double*)part.p, (double*)red.p, nullptr);
+    double pap[2];
+    QDWF_HIP(hipMemcpy(pap, red.p, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    const double alpha = qnorm_r / pap[0];
+    qdwf::launch_axpy<double2>(alpha, p.d(), x, n, nullptr);
+    qdwf::launch_axpy<double2>(-alpha, ap.d(), r.d(), n, nullptr);
+    check_launch();
+    const double new_qnorm_r = qed_norm2(r.d(), n, (double*)part.p, (double*)red.p);
+    if (new_qnorm_r <= qnorm_in * stop_rsd * stop_rsd) return iter;
+    qdwf::launch_xpay<double2>(r.d(), new_qnorm_r / qnorm_r, p.d(), n, nullptr);  // p = r + beta p
+    check_launch();
+    qnorm_r = new_qnorm_r;
+  }
+  return max_num_iter + 1;
+}
+
+}  // namespace
+
+int qlat_dwf_multiply_m_dwf_qed(const int total_site[4], int ls, const double* gf1, double mass, double m5,
+                                const double* wick, int is_dagger, double* out, const double* in)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(out && in, "null field pointer");
+    const QedCall q(total_site, ls, gf1, mass, m5, wick);
+    const size_t fb = q.n * sizeof(double2);
+    DevBuf din(fb), dout(fb);
+    QDWF_HIP(hipMemcpy(din.p, in, fb, hipMemcpyHostToDevice));
+    q.apply(dout.d(), din.d(), is_dagger != 0);
+    QDWF_HIP(hipMemcpy(out, dout.p, fb, hipMemcpyDeviceToHost));
+  });
+}
+
+int qlat_dwf_cg_with_m_dwf_qed(const int total_site[4], int ls, const double* gf1, double mass, double m5,
+                               const double* wick, int is_dagger, double stop_rsd, long max_num_iter, double* out,
+                               const double* in, long* iter)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(out && in && iter, "invalid argument");
+    const QedCall q(total_site, ls, gf1, mass, m5, wick);
+    const size_t fb = q.n * sizeof(double2);
+    DevBuf din(fb), dx(fb);
+    QDWF_HIP(hipMemcpy(din.p, in, fb, hipMemcpyHostToDevice));
+    QDWF_HIP(hipMemcpy(dx.p, out, fb, hipMemcpyHostToDevice));  // the initial guess
+    *iter = qed_cg(q, dx.d(), din.d(), is_dagger != 0, stop_rsd, max_num_iter);
+    QDWF_HIP(hipMemcpy(out, dx.p, fb, hipMemcpyDeviceToHost));
+  });
+}
+
+int qlat_dwf_invert_dwf_qed(const int total_site[4], int ls, const double* gf1, double mass, double m5,
+                            const double* wick, int is_dagger, double stop_rsd, long max_num_iter, double* out4d,
+                            const double* in4d, long* iter)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(out4d && in4d && iter, "invalid argument");
+    const QedCall q(total_site, ls, gf1, mass, m5, wick);
+    const int64_t vol = q.vol;
+    const size_t fb = q.n * sizeof(double2);
+    // fermion_field_5d_from_4d_qed(src5d, in, ls, 0, ls - 1) on the host side of the copy
+    std::vector<double2> h5((size_t)q.n, make_double2(0.0, 0.0));
+    const double2* in = (const double2*)in4d;
+    for (int64_t x = 0; x < vol; ++x) {
+      h5[(x * ls + 0) * 4 + 0] = in[x * 4 + 0];
+      h5[(x * ls + 0) * 4 + 1] = in[x * 4 + 1];
+      h5[(x * ls + ls - 1) * 4 + 2] = in[x * 4 + 2];
+      h5[(x * ls + ls - 1) * 4 + 3] = in[x * 4 + 3];
+    }
+    DevBuf s5(fb), b5(fb), x5(fb);
+    QDWF_HIP(hipMemcpy(s5.p, h5.data(), fb, hipMemcpyHostToDevice));
+    q.apply(b5.d(), s5.d(), is_dagger == 0);  // M_{!d} src (qed.cpp:141-142)
+    QDWF_HIP(hipMemset(x5.p, 0, fb));
+    *iter = qed_cg(q, x5.d(), b5.d(), is_dagger != 0, stop_rsd, max_num_iter);
+    QDWF_HIP(hipMemcpy(h5.data(), x5.p, fb, hipMemcpyDeviceToHost));
+    // fermion_field_4d_from_5d_qed(out, sol5d, ls, ls - 1, 0)
+    double2* out = (double2*)out4d;
+    for (int64_t x = 0; x < vol; ++x) {
+      out[x * 4 + 0] = h5[(x * ls + ls - 1) * 4 + 0];
+      out[x * 4 + 1] = h5[(x * ls + ls - 1) * 4 + 1];
+      out[x * 4 + 2] = h5[(x * ls + 0) * 4 + 2];
+      out[x * 4 + 3] = h5[(x * ls + 0) * 4 + 3];
+    }
+  });
+}
+
 int qlat_dwf_inverter_set_low_modes(qlat_dwf_inverter* inv, const int block_site[4], int n_basis, int n_vec,
                                     const double* eigen_values, const float* bases, const float* coefs)
 {

@@ -201,6 +201,21 @@ struct SrcArgs {
 };
 void launch_make_src(double2* prop, const SrcArgs& a, hipStream_t st);
 
+// QED domain-wall operator (qed.cpp:263-411) on full 5D fields in the
+// reference layout [V][ls][4] complex128, U(1) links gf [V][4], optional
+// Wick phase factors per time slice.
+struct QedArgs {
+  int n[4];
+  int ls;
+  double mass, m5;
+  const double2* gf;
+  const double2* wick;  // [T] or null
+  int dagger;
+  const double2* in;
+  double2* out;
+};
+void launch_qed_m(const QedArgs& a, hipStream_t st);
+
 // Block-compressed low modes (deflate, dslash.cpp:144-249).  hmap[b][bidx]
 // is the device half-site of half site bidx of block b; field elements are
 // FP64 SoA half fields of parity eo.

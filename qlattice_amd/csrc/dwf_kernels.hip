@@ -905,6 +905,121 @@ void launch_make_src(double2* prop, const SrcArgs& a, hipStream_t st)
   hipLaunchKernelGGL(k_make_src, dim3((unsigned)((n + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st, prop, a);
 }
 
+// ------------------------------------------------------------ QED operator
+// multiply_m_dwf_qed (qed.cpp:263-411): thread (site, s) forms out(x, s)
+// from in(x, s'), s' in {s-1, s, s+1, 0, Ls-1}, and the 8 neighbours at s.
+// CPS gammas (matrix.h:426-452) as one non-zero per row: row r of gamma_mu
+// has coefficient kQg[mu][r] (1, i, -1, -i as 0..3) in column kQc[mu][r].
+__constant__ int kQc[4][4] = {{3, 2, 1, 0}, {3, 2, 1, 0}, {2, 3, 0, 1}, {2, 3, 0, 1}};
+__constant__ int kQg[4][4] = {{1, 1, 3, 3}, {2, 0, 0, 2}, {1, 3, 3, 1}, {0, 0, 0, 0}};
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b)
+{
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ double2 zscale(double c, double2 a) { return make_double2(c * a.x, c * a.y); }
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+// i^k * a
+__device__ __forceinline__ double2 zipow(int k, double2 a)
+{
+  return k == 0 ? a : k == 1 ? make_double2(-a.y, a.x) : k == 2 ? make_double2(-a.x, -a.y) : make_double2(a.y, -a.x);
+}
+
+// v += c * iv + cg * (G iv), G = gamma_5 (mu < 0) or CPS gamma_mu
+__device__ __forceinline__ void qed_acc(double2 v[4], double2 c, double2 cg, int mu, const double2* __restrict__ iv)
+{
+  double2 w[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) w[r] = iv[r];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double2 gv = mu < 0 ? (r < 2 ? w[r] : zscale(-1.0, w[r])) : zipow(kQg[mu][r], w[kQc[mu][r]]);
+    const double2 a = zmul(c, w[r]), b = zmul(cg, gv);
+    v[r] = make_double2(v[r].x + a.x + b.x, v[r].y + a.y + b.y);
+  }
+}
+
+__global__ void __launch_bounds__(kBlasBlock) k_qed_m(QedArgs a)
+{
+  const int ls = a.ls;
+  const int64_t vol = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.n[3];
+  const int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x;
+  if (i >= vol * ls) return;
+  const int64_t site = i / ls;
+  const int s = (int)(i - site * ls);
+  int x[4];
+  int64_t r = site;
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    const int64_t q = r / a.n[mu];
+    x[mu] = (int)(r - q * a.n[mu]);
+    r = q;
+  }
+  x[3] = (int)r;
+  const int T = a.n[3];
+  double2 tw = make_double2(1.0, 0.0), twm = make_double2(1.0, 0.0);
+  if (a.wick) {
+    tw = a.wick[x[3]];
+    twm = a.wick[(x[3] - 1 + T) % T];
+  }
+  if (a.dagger) {
+    tw = zconj(tw);
+    twm = zconj(twm);
+  }
+  const double2 one = make_double2(1.0, 0.0);
+  const double2 avg = make_double2(0.5 * tw.x + 0.5 * twm.x, 0.5 * tw.y + 0.5 * twm.y);
+  const double2 itw = zdiv(one, tw), itwm = zdiv(one, twm);
+  const double2 iavg = make_double2(0.5 * itw.x + 0.5 * itwm.x, 0.5 * itw.y + 0.5 * itwm.y);
+  const double df = a.dagger ? -1.0 : 1.0;
+  const double2 zero = make_double2(0.0, 0.0);
+  auto at = [&](int64_t st, int ss) { return a.in + (st * ls + ss) * 4; };
+  double2 v[4] = {zero, zero, zero, zero};
+  // ((4 - m5) avg + inv_avg) in
+  qed_acc(v, make_double2((4.0 - a.m5) * avg.x + iavg.x, (4.0 - a.m5) * avg.y + iavg.y), zero, -1, at(site, s));
+  // fifth dimension (qed.cpp:342-371)
+  if (s < ls - 1) qed_acc(v, zscale(-0.5, avg), zscale(df * 0.5, avg), -1, at(site, s + 1));
+  if (s > 0) qed_acc(v, zscale(-0.5, avg), zscale(-df * 0.5, avg), -1, at(site, s - 1));
+  if (s == ls - 1) qed_acc(v, zscale(0.5 * a.mass, avg), zscale(-df * 0.5 * a.mass, avg), -1, at(site, 0));
+  if (s == 0) qed_acc(v, zscale(0.5 * a.mass, avg), zscale(df * 0.5 * a.mass, avg), -1, at(site, ls - 1));
+  // hopping (qed.cpp:373-408)
+  int64_t stride = 1;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    const int L = a.n[mu];
+    const int64_t sp = site + (x[mu] + 1 == L ? -(int64_t)(L - 1) : 1) * stride;
+    const int64_t sm = site + (x[mu] == 0 ? (int64_t)(L - 1) : -1) * stride;
+    stride *= L;
+    double2 up = a.gf[site * 4 + mu];
+    double2 um = zdiv(one, a.gf[sm * 4 + mu]);
+    if (a.dagger) {
+      up = zconj(zdiv(one, up));
+      um = zconj(zdiv(one, um));
+    }
+    double2 tf1 = avg, tfm1 = avg, tf2 = avg, tfm2 = avg;
+    if (mu == 3) {
+      tf1 = tfm1 = one;
+      tf2 = itw;
+      tfm2 = itwm;
+    }
+    qed_acc(v, zscale(-0.5, zmul(tf2, up)), zscale(df * 0.5, zmul(up, tf1)), mu, at(sp, s));
+    qed_acc(v, zscale(-0.5, zmul(tfm2, um)), zscale(-df * 0.5, zmul(um, tfm1)), mu, at(sm, s));
+  }
+  double2* o = a.out + (site * ls + s) * 4;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o[k] = v[k];
+}
+
+void launch_qed_m(const QedArgs& a, hipStream_t st)
+{
+  const int64_t n = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.n[3] * a.ls;
+  hipLaunchKernelGGL(k_qed_m, dim3((unsigned)((n + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st, a);
+}
+
 // ---------------------------------------------------------------- deflation
 // The three steps of deflate (dslash.cpp:144-249) over blocks of block_site:
 // project the FP32-rounded input on the block bases (FP32 products, FP64
