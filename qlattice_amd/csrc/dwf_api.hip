@@ -2781,6 +2781,19 @@ int qlat_dwf_inverter_set_low_modes(qlat_dwf_inverter* inv, const int block_site
     check_inv(inv);
     Inv& I = *inv;
     QDWF_REQUIRE(n_vec >= 0, "n_vec must be >= 0");
+    // validate before the current low modes are released
+    int64_t nblock = 1, bvol = 1;
+    if (n_vec > 0) {
+      QDWF_REQUIRE(block_site && eigen_values && bases && coefs, "invalid argument");
+      QDWF_REQUIRE(n_basis >= 1 && n_basis <= 2048, "n_basis must be in [1, 2048]");
+      for (int mu = 0; mu < 4; ++mu) {
+        QDWF_REQUIRE(block_site[mu] > 0 && I.geo.n[mu] % block_site[mu] == 0, "block_site must divide total_site");
+        nblock *= I.geo.n[mu] / block_site[mu];
+        bvol *= block_site[mu];
+      }
+      QDWF_REQUIRE(block_site[0] % 2 == 0, "block_site[0] must be even (x pairs of opposite parity)");
+      for (int i = 0; i < n_vec; ++i) QDWF_REQUIRE(eigen_values[i] != 0.0, "eigen values must be non-zero");
+    }
     auto f = [](void*& p) {
       if (p) (void)hipFree(p);
       p = nullptr;
@@ -2794,16 +2807,6 @@ int qlat_dwf_inverter_set_low_modes(qlat_dwf_inverter* inv, const int block_site
     f((void*&)I.lm_p);
     I.lm_nvec = 0;
     if (n_vec == 0) return;
-    QDWF_REQUIRE(block_site && eigen_values && bases && coefs, "invalid argument");
-    QDWF_REQUIRE(n_basis >= 1 && n_basis <= 2048, "n_basis must be in [1, 2048]");
-    int64_t nblock = 1, bvol = 1;
-    for (int mu = 0; mu < 4; ++mu) {
-      QDWF_REQUIRE(block_site[mu] > 0 && I.geo.n[mu] % block_site[mu] == 0, "block_site must divide total_site");
-      nblock *= I.geo.n[mu] / block_site[mu];
-      bvol *= block_site[mu];
-    }
-    QDWF_REQUIRE(block_site[0] % 2 == 0, "block_site[0] must be even (x pairs of opposite parity)");
-    for (int i = 0; i < n_vec; ++i) QDWF_REQUIRE(eigen_values[i] != 0.0, "eigen values must be non-zero");
     for (int mu = 0; mu < 4; ++mu) I.lm_bs[mu] = block_site[mu];
     I.lm_nblock = (int)nblock;
     I.lm_bve = (int)(bvol / 2);
