@@ -246,6 +246,13 @@ int qlat_dwf_inverter_set_low_modes(qlat_dwf_inverter* inv, const int block_site
  * out = 0 when no low modes are attached.  out may equal in. */
 int qlat_dwf_deflate(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
 
+/* free_invert (qcd-prop.h:347-359, free_mom_invert :292-345): the
+ * infinite-Ls domain-wall propagator on the unit gauge field applied to a
+ * propagator [V][12][12] in momentum space (InverterDwfFreeField,
+ * inverter.py:26-71); momtwist may be NULL (no twist).  sol may equal src. */
+int qlat_dwf_free_invert(const int total_site[4], double mass, double m5, const double momtwist[4], double* sol,
+                         const double* src);
+
 /* ------------------------------------------------------------------ QED
  * The QED domain-wall operator and its solvers (qed.h:433-463,
  * qed.cpp:119-411): full (not even-odd) 5D fields [V][ls][4] complex128 in

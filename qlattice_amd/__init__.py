@@ -28,12 +28,14 @@ from .fields import (
 from .inverter import (
     Inverter,
     InverterDomainWall,
+    InverterDwfFreeField,
     LowModes,
     bench_kernel,
     cg_with_herm_sym_2,
     deflate,
     dot_product,
     find_max_eigen_value_hermop_sym2,
+    free_invert,
     invert,
     invert_batch,
     multiply_d_minus,

@@ -2709,6 +2709,37 @@ long qed_cg(const QedCall& q, double2* x, const double2* b, bool dagger, double 
 
 }  // namespace
 
+int qlat_dwf_free_invert(const int total_site[4], double mass, double m5, const double momtwist[4], double* sol,
+                         const double* src)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(total_site && sol && src, "invalid argument");
+    int64_t vol = 1;
+    for (int mu = 0; mu < 4; ++mu) {
+      QDWF_REQUIRE(total_site[mu] > 0, "total_site must be positive");
+      vol *= total_site[mu];
+    }
+    const double tw0[4] = {0.0, 0.0, 0.0, 0.0};
+    const double* tw = momtwist ? momtwist : tw0;
+    const size_t bytes = (size_t)vol * 144 * sizeof(double2);
+    DevBuf a(bytes), b(bytes);
+    QDWF_HIP(hipMemcpy(a.p, src, bytes, hipMemcpyHostToDevice));
+    // fft_complex_field(sol, true): sum_x f(x) exp(-i k x), direction by direction
+    for (int mu = 0; mu < 4; ++mu) {
+      qdwf::launch_dft_dir(mu % 2 ? a.d() : b.d(), mu % 2 ? b.d() : a.d(), total_site, mu, -1, nullptr);
+      check_launch();
+    }
+    // free_mom_invert; the 1 / V of the backward transform applied here (linear)
+    qdwf::launch_free_mom(a.d(), total_site, mass, m5, tw, 1.0 / (double)vol, nullptr);
+    check_launch();
+    for (int mu = 0; mu < 4; ++mu) {
+      qdwf::launch_dft_dir(mu % 2 ? a.d() : b.d(), mu % 2 ? b.d() : a.d(), total_site, mu, +1, nullptr);
+      check_launch();
+    }
+    QDWF_HIP(hipMemcpy(sol, a.p, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
 int qlat_dwf_multiply_m_dwf_qed(const int total_site[4], int ls, const double* gf1, double mass, double m5,
                                 const double* wick, int is_dagger, double* out, const double* in)
 {

@@ -216,6 +216,13 @@ struct QedArgs {
 };
 void launch_qed_m(const QedArgs& a, hipStream_t st);
 
+// Free-field domain-wall propagator (free_invert, qcd-prop.h:292-359) on
+// propagators [V][12][12]: a DFT along direction mu (sign -1 forward,
+// +1 backward, unnormalised), and the momentum-space multiply.
+void launch_dft_dir(double2* out, const double2* in, const int n[4], int mu, int sign, hipStream_t st);
+void launch_free_mom(double2* prop, const int n[4], double mass, double m5, const double momtwist[4], double scale,
+                     hipStream_t st);
+
 // Block-compressed low modes (deflate, dslash.cpp:144-249).  hmap[b][bidx]
 // is the device half-site of half site bidx of block b; field elements are
 // FP64 SoA half fields of parity eo.

@@ -1020,6 +1020,125 @@ void launch_qed_m(const QedArgs& a, hipStream_t st)
   hipLaunchKernelGGL(k_qed_m, dim3((unsigned)((n + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st, a);
 }
 
+// ------------------------------------------------------- free-field DWF
+// out(.., k_mu, ..) = sum_x in(.., x_mu, ..) exp(sign 2 pi i k x / L): thread
+// (site, entry) over the 144 entries of a propagator site; the phase index
+// (k x) mod L keeps the argument of sincos in [0, 2 pi).
+struct DftArgs {
+  int n[4];
+  int mu;
+  int sign;
+};
+__global__ void __launch_bounds__(kBlasBlock) k_dft_dir(double2* __restrict__ out, const double2* __restrict__ in,
+                                                        DftArgs a)
+{
+  const int64_t vol = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.n[3];
+  const int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x;
+  if (i >= vol * 144) return;
+  const int64_t site = i / 144;
+  const int e = (int)(i - site * 144);
+  int64_t stride = 1;
+  for (int m = 0; m < a.mu; ++m) stride *= a.n[m];
+  const int L = a.n[a.mu];
+  const int k = (int)((site / stride) % L);
+  const int64_t base = site - (int64_t)k * stride;
+  const double w = 2.0 * 3.141592653589793 / L;
+  double re = 0.0, im = 0.0;
+  for (int x = 0; x < L; ++x) {
+    const int ph = (int)(((int64_t)k * x) % L);
+    double sn, cs;
+    sincos(w * ph, &sn, &cs);
+    sn *= a.sign;
+    const double2 v = in[(base + (int64_t)x * stride) * 144 + e];
+    re += v.x * cs - v.y * sn;
+    im += v.x * sn + v.y * cs;
+  }
+  out[i] = make_double2(re, im);
+}
+
+void launch_dft_dir(double2* out, const double2* in, const int n[4], int mu, int sign, hipStream_t st)
+{
+  DftArgs a{};
+  for (int m = 0; m < 4; ++m) a.n[m] = n[m];
+  a.mu = mu;
+  a.sign = sign;
+  const int64_t tot = (int64_t)n[0] * n[1] * n[2] * n[3] * 144;
+  hipLaunchKernelGGL(k_dft_dir, dim3((unsigned)((tot + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st, out,
+                     in, a);
+}
+
+// free_mom_invert (qcd-prop.h:292-345): at momentum k the propagator is
+// multiplied by ipgm = lwa (-i sum_i sin(k_i) gamma_i + m lwa) / (p^2 + (m lwa)^2)
+// on the spin index (colour identity), or set to zero where p^2 and lwa are
+// both below 1e-10; every entry is then scaled by `scale`.  Thread (site,
+// colour c, column) rewrites the 4 spin rows of that colour and column.
+struct FreeMomArgs {
+  int n[4];
+  double mass, m5, tw[4], scale;
+};
+__global__ void __launch_bounds__(kBlasBlock) k_free_mom(double2* __restrict__ prop, FreeMomArgs a)
+{
+  const int64_t vol = (int64_t)a.n[0] * a.n[1] * a.n[2] * a.n[3];
+  const int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x;
+  if (i >= vol * 36) return;
+  const int64_t site = i / 36;
+  const int cc = (int)(i - site * 36);
+  const int c = cc / 12, col = cc % 12;
+  int kx[4];
+  int64_t r = site;
+  for (int m = 0; m < 4; ++m) {
+    kx[m] = (int)(r % a.n[m]);
+    r /= a.n[m];
+  }
+  double p2 = 0.0, wp = 1.0 - a.m5, ks[4];
+  for (int m = 0; m < 4; ++m) {
+    const double kk = 2.0 * 3.141592653589793 * (kx[m] + a.tw[m]) / (double)a.n[m];
+    ks[m] = sin(kk);
+    p2 += ks[m] * ks[m];
+    const double sh = sin(kk / 2.0);
+    wp += 2.0 * sh * sh;
+  }
+  const double calpha = (1.0 + wp * wp + p2) / 2.0 / wp;
+  const double alpha = acosh(calpha);
+  const double lwa = 1.0 - wp * exp(-alpha);
+  double2* base = prop + site * 144;
+  double2 v[4];
+  for (int s = 0; s < 4; ++s) v[s] = base[(s * 3 + c) * 12 + col];
+  double2 o[4];
+  if (1.0e-10 > p2 && 1.0e-10 > lwa) {
+    for (int s = 0; s < 4; ++s) o[s] = make_double2(0.0, 0.0);
+  } else {
+    const double f = lwa / (p2 + (a.mass * lwa) * (a.mass * lwa));
+    // pg v with CPS gammas (one non-zero per row), then ipgm v = f (-i pg v + m lwa v)
+    for (int s = 0; s < 4; ++s) {
+      double2 pg = make_double2(0.0, 0.0);
+      for (int m = 0; m < 4; ++m) {
+        const double2 g = zipow(kQg[m][s], v[kQc[m][s]]);
+        pg.x += ks[m] * g.x;
+        pg.y += ks[m] * g.y;
+      }
+      o[s] = make_double2(f * (pg.y + a.mass * lwa * v[s].x), f * (-pg.x + a.mass * lwa * v[s].y));
+    }
+  }
+  for (int s = 0; s < 4; ++s) base[(s * 3 + c) * 12 + col] = make_double2(a.scale * o[s].x, a.scale * o[s].y);
+}
+
+void launch_free_mom(double2* prop, const int n[4], double mass, double m5, const double momtwist[4], double scale,
+                     hipStream_t st)
+{
+  FreeMomArgs a{};
+  for (int m = 0; m < 4; ++m) {
+    a.n[m] = n[m];
+    a.tw[m] = momtwist[m];
+  }
+  a.mass = mass;
+  a.m5 = m5;
+  a.scale = scale;
+  const int64_t tot = (int64_t)n[0] * n[1] * n[2] * n[3] * 36;
+  hipLaunchKernelGGL(k_free_mom, dim3((unsigned)((tot + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st, prop,
+                     a);
+}
+
 // ---------------------------------------------------------------- deflation
 // The three steps of deflate (dslash.cpp:144-249) over blocks of block_site:
 // project the FP32-rounded input on the block bases (FP32 products, FP64

@@ -28,6 +28,41 @@ class Inverter:
     pass
 
 
+def free_invert(prop_src, mass, m5=1.0, momtwist=None):
+    """free_invert (qcd-prop.h:347-359): the infinite-Ls domain-wall
+    propagator of the free field applied to prop_src, on the GPU."""
+    geo = prop_src.geo
+    src = np.ascontiguousarray(prop_src.data, dtype=np.complex128)
+    sol = Prop(geo)
+    ts = (ctypes.c_int * 4)(*geo.total_site())
+    tw = None if momtwist is None else (ctypes.c_double * 4)(*[float(v) for v in momtwist])
+    _lib.check(_lib.load().qlat_dwf_free_invert(ts, float(mass), float(m5), tw, _lib.ptr(sol.data), _lib.ptr(src)),
+               "qlat_dwf_free_invert")
+    return sol
+
+
+class InverterDwfFreeField(Inverter):
+    """inverter.py:26-71: the analytic free-field domain-wall inverter."""
+
+    def __init__(self, *, mass, m5=1.0, momtwist=None, qtimer=None):
+        self.mass = float(mass)
+        self.m5 = float(m5)
+        self.momtwist = momtwist
+        self.timer = qtimer
+
+    def __mul__(self, prop_src):
+        if isinstance(prop_src, Prop):
+            if self.timer is not None:
+                self.timer.start()
+            sol = free_invert(prop_src, self.mass, self.m5, self.momtwist)
+            if self.timer is not None:
+                self.timer.stop()
+            return sol
+        if isinstance(prop_src, list):
+            return [self * p for p in prop_src]
+        raise Exception("InverterDwfFreeField")
+
+
 class InverterDomainWall(Inverter):
     """InverterDomainWall (dslash.h:135-197; inverter.py:75-129).
 
