@@ -57,6 +57,14 @@ from .inverter import (
 )
 from ._lib import QlatDwfError, load as load_library
 from .force import fermion_force, pseudofermion_action, set_gauge
+from .eigen_io import (
+    CompressedEigenSystemInfo,
+    load_compressed_eigen_vectors,
+    load_low_modes,
+    read_compressed_eigen_system_info,
+    read_eigen_values,
+    save_compressed_eigen_vectors,
+)
 from .gauge_io import (
     GaugeFieldInfo,
     gf_avg_link_trace,

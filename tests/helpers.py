@@ -54,3 +54,29 @@ def qlat_check(v, ref, eps):
     """The reference's golden-value test: 2|v - ref| / (|v| + |ref|) <= eps
     (qlat_utils/utils.py:455-545)."""
     return 2 * abs(v - ref) / (abs(v) + abs(ref)) <= eps
+
+
+def block_order(site, bs, eo=1):
+    """half index (reference order) of element rows: [block][bidx] in the
+    compressed layout (convert_half_vector, commpressed-eigen-io.cpp:925-947)."""
+    nb = [site[m] // bs[m] for m in range(4)]
+    bve = int(np.prod(bs)) // 2
+    rows = []
+    for b in range(int(np.prod(nb))):
+        bc, r = [], b
+        for m in range(4):
+            bc.append(r % nb[m])
+            r //= nb[m]
+        blk = []
+        for bidx in range(bve):
+            r, lc = 2 * bidx, []
+            for m in range(4):
+                lc.append(r % bs[m])
+                r //= bs[m]
+            x = [bc[m] * bs[m] + lc[m] for m in range(4)]
+            if (sum(x) & 1) != (1 if eo == 1 else 0):
+                x[0] += 1
+            full = x[0] + site[0] * (x[1] + site[1] * (x[2] + site[2] * x[3]))
+            blk.append(full // 2)
+        rows.append(blk)
+    return np.array(rows)

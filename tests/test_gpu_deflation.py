@@ -17,7 +17,7 @@ import pytest
 
 import qlattice_amd as q
 from oracle import oracle as O
-from tests.helpers import rand_field, rand_gauge, rel
+from tests.helpers import block_order, rand_field, rand_gauge, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -26,32 +26,6 @@ LS = 6
 BS = (2, 2, 2, 2)
 NVEC = 24
 MASS = 0.02
-
-
-def block_order(site, bs, eo=1):
-    """half index (reference order) of element rows: [block][bidx] in the
-    compressed layout (convert_half_vector, commpressed-eigen-io.cpp:925-947)."""
-    nb = [site[m] // bs[m] for m in range(4)]
-    bve = int(np.prod(bs)) // 2
-    rows = []
-    for b in range(int(np.prod(nb))):
-        bc, r = [], b
-        for m in range(4):
-            bc.append(r % nb[m])
-            r //= nb[m]
-        blk = []
-        for bidx in range(bve):
-            r, lc = 2 * bidx, []
-            for m in range(4):
-                lc.append(r % bs[m])
-                r //= bs[m]
-            x = [bc[m] * bs[m] + lc[m] for m in range(4)]
-            if (sum(x) & 1) != (1 if eo == 1 else 0):
-                x[0] += 1
-            full = x[0] + site[0] * (x[1] + site[1] * (x[2] + site[2] * x[3]))
-            blk.append(full // 2)
-        rows.append(blk)
-    return np.array(rows)
 
 
 @pytest.fixture(scope="module")
@@ -170,3 +144,27 @@ def test_deflated_mixed_solve(system, solver):
     assert st["true_residual"] <= 1e-9 and st_plain["true_residual"] <= 1e-9
     assert rel(defl.data, ref) < 1e-6 and rel(plain.data, ref) < 1e-6
     assert st["total_iter"] < st_plain["total_iter"]
+
+
+def test_deflate_with_low_modes_from_files(system, tmp_path):
+    """load_low_modes (dslash.cpp:9-23) of the same system written over two
+    nodes: bit-identical deflation from FP32 files, and within the 16-bit
+    code's resolution when half the basis and the coefficients are 16-bit."""
+    U, orc, lm, lam, vec, idx = system
+    geo, inv = make_inv(U)
+    nh = geo.total_volume() // 2
+    f = q.FermionField5d(geo, LS, 1)
+    f.data = rand_field((nh, LS, 12), 85)
+    inv.set_low_modes(lm)
+    ref = q.deflate(None, f, inv).data
+    q.save_compressed_eigen_vectors(str(tmp_path / "f32"), lm, SITE, LS, (4, 4, 2, 2))
+    inv.set_low_modes(q.load_low_modes(str(tmp_path / "f32")))
+    assert np.array_equal(q.deflate(None, f, inv).data, ref)
+    q.save_compressed_eigen_vectors(str(tmp_path / "f16"), lm, SITE, LS, (4, 4, 2, 2), nkeep_single=NVEC // 2,
+                                    fp16_coef_exp_share_floats=8)
+    inv.set_low_modes(q.load_low_modes(str(tmp_path / "f16")))
+    got = q.deflate(None, f, inv).data
+    x32 = f.data.astype(np.complex64).astype(np.complex128)
+    proj = (vec @ ((vec.conj().T @ f.data.reshape(-1)) / lam)).reshape(f.data.shape)
+    print(f"[gpu] deflation from 16-bit files: rel {rel(got - x32, proj):.2e}", flush=True)
+    assert rel(got - x32, proj) < 1e-3
