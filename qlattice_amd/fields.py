@@ -119,6 +119,31 @@ class _Field:
         self.data *= c
         return self
 
+    def set_rand(self, rng, upper=1.0, lower=0.0):
+        """Field.set_rand (field_types.pyx.in:194-195 -> set_u_rand,
+        core.h:1584-1603): uniform reals in [lower, upper) from rng, site g
+        drawing from rng.newtype(g)."""
+        from .rng import RngState, set_u_rand
+
+        if not isinstance(rng, RngState):
+            raise TypeError("set_rand: rng must be an RngState")
+        if getattr(self, "eo", 0) != 0:
+            raise NotImplementedError("set_rand: only full-lattice fields")
+        set_u_rand(self.data, rng, upper, lower)
+        return self
+
+    def set_rand_g(self, rng, center=0.0, sigma=1.0):
+        """Field.set_rand_g (field_types.pyx.in:198-199 -> set_g_rand,
+        core.h:1619-1640)."""
+        from .rng import RngState, set_g_rand
+
+        if not isinstance(rng, RngState):
+            raise TypeError("set_rand_g: rng must be an RngState")
+        if getattr(self, "eo", 0) != 0:
+            raise NotImplementedError("set_rand_g: only full-lattice fields")
+        set_g_rand(self.data, rng, center, sigma)
+        return self
+
 
 class GaugeField(_Field):
     """GaugeField = FieldM<ColorMatrix, 4> (core.h:1187)."""
@@ -192,10 +217,16 @@ class Prop(_Field):
         if geo is not None:
             self.data = np.zeros((geo.total_volume(), 12, 12), dtype=np.complex128)
 
-    def set_rand(self, seed):
-        rng = np.random.Generator(np.random.PCG64(seed))
-        self.data = (rng.standard_normal(self.data.shape)
-                     + 1j * rng.standard_normal(self.data.shape))
+    def set_rand(self, rng, upper=1.0, lower=0.0):
+        """With an RngState: the reference's set_u_rand.  With an integer:
+        complex Gaussian entries from numpy's PCG64 seeded with it."""
+        from .rng import RngState
+
+        if isinstance(rng, RngState):
+            return super().set_rand(rng, upper, lower)
+        g = np.random.Generator(np.random.PCG64(rng))
+        self.data = (g.standard_normal(self.data.shape)
+                     + 1j * g.standard_normal(self.data.shape))
         return self
 
 

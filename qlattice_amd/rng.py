@@ -1,0 +1,206 @@
+"""RngState and the field fills built on it (qlat-utils rng-state.h:14-93,
+rng-state.cpp:125-283; set_u_rand / set_g_rand, qlat core.h:1584-1640).
+
+The reference's generator is SHA-256 over a message that grows with every
+split: a state holds the SHA-256 midstate of its message (a multiple of 64
+bytes), ``split(s)`` appends ``"[index] {s}"`` (``"[type,index] {s}"`` after
+``newtype``) padded with spaces to a multiple of 64 bytes, and a draw that
+finds no cached value hashes the message followed by ``"[index]"`` (or
+``"[type,index]"``) with standard SHA-256 padding, returns the digest's last
+two 32-bit words (big-endian, high word first) as a 64-bit integer and
+caches words 4-5, 2-3 and 0-1 for the next three draws.  Every draw
+increments the index.  hashlib's SHA-256 object keeps exactly that midstate,
+so the state here carries one.
+
+Host code: fills of test-size fields (a draw per real number); the
+operator and solvers never call it.
+"""
+
+import hashlib
+import math
+import struct
+
+import numpy as np
+
+_ULONG_MAX = (1 << 64) - 1
+_FAC = 1.0 / (256.0 * 256.0 * 256.0 * 256.0) / (256.0 * 256.0 * 256.0 * 256.0)
+
+
+class RngState:
+    """RngState (rng-state.h:31-93): RngState(), RngState(seed: str | int),
+    split(sindex), newtype(type), and the draws rand_gen, u_rand_gen,
+    g_rand_gen as methods."""
+
+    def __init__(self, seed=None):
+        self._sha = hashlib.sha256()
+        self.num_bytes = 0
+        self.type = _ULONG_MAX
+        self.index = 0
+        self.cache = [0, 0, 0]
+        self.cache_avail = 0
+        self.gaussian = 0.0
+        self.gaussian_avail = False
+        if seed is not None:
+            self._split_from(self, str(seed))
+
+    def _split_from(self, rs0, sindex):
+        """split_rng_state (rng-state.cpp:148-177)."""
+        if rs0.type == _ULONG_MAX:
+            data = f"[{rs0.index}] {{{sindex}}}"
+        else:
+            data = f"[{rs0.type},{rs0.index}] {{{sindex}}}"
+        raw = data.encode()
+        n_blocks = (len(raw) - 1) // 64 + 1
+        raw = raw.ljust(n_blocks * 64, b" ")
+        sha = rs0._sha.copy()
+        sha.update(raw)
+        self._sha = sha
+        self.num_bytes = rs0.num_bytes + n_blocks * 64
+        self.type = _ULONG_MAX
+        self.index = 0
+        self.cache = [0, 0, 0]
+        self.cache_avail = 0
+        self.gaussian = 0.0
+        self.gaussian_avail = False
+
+    def copy(self):
+        rs = RngState.__new__(RngState)
+        rs.__dict__.update(self.__dict__)
+        rs._sha = self._sha.copy()
+        rs.cache = list(self.cache)
+        return rs
+
+    def split(self, sindex):
+        """RngState(rs0, sindex): the child stream named str(sindex)."""
+        rs = RngState.__new__(RngState)
+        rs._split_from(self, str(sindex))
+        return rs
+
+    def newtype(self, type_):
+        """A copy whose draws hash "[type,index]" (set_type, rng-state.cpp:20-27)."""
+        if self.type != _ULONG_MAX:
+            raise ValueError("RngState.newtype: the type is already set")
+        type_ = int(type_)
+        if not 0 <= type_ < _ULONG_MAX:
+            raise ValueError("RngState.newtype: type out of range")
+        rs = self.copy()
+        rs.type = type_
+        rs.cache_avail = 0
+        rs.gaussian_avail = False
+        return rs
+
+    def _hash_words(self, index):
+        sha = self._sha.copy()
+        if self.type == _ULONG_MAX:
+            sha.update(f"[{index}]".encode())
+        else:
+            sha.update(f"[{self.type},{index}]".encode())
+        return struct.unpack(">8I", sha.digest())
+
+    def rand_gen(self):
+        """rng-state.cpp:186-212: the next 64-bit integer."""
+        self.index += 1
+        if self.cache_avail > 0:
+            self.cache_avail -= 1
+            r = self.cache[self.cache_avail]
+            self.cache[self.cache_avail] = 0
+            return r
+        w = self._hash_words(self.index)
+        self.cache = [w[0] << 32 | w[1], w[2] << 32 | w[3], w[4] << 32 | w[5]]
+        self.cache_avail = 3
+        return w[6] << 32 | w[7]
+
+    def u_rand_gen(self, upper=1.0, lower=0.0):
+        """rng-state.cpp:214-220: rand_gen * 2^-64 * (upper - lower) + lower."""
+        return float(self.rand_gen()) * _FAC * (upper - lower) + lower
+
+    def g_rand_gen(self, center=0.0, sigma=1.0):
+        """rng-state.cpp:222-258: polar Box-Muller on u_rand_gen(1, -1)."""
+        self.index += 1
+        if self.gaussian_avail:
+            self.gaussian_avail = False
+            return self.gaussian * sigma + center
+        num_try = 1
+        while True:
+            v1 = self.u_rand_gen(1.0, -1.0)
+            v2 = self.u_rand_gen(1.0, -1.0)
+            rsq = v1 * v1 + v2 * v2
+            num_try += 1
+            if not (num_try < 10000 and (rsq >= 1.0 or rsq == 0)):
+                break
+        if num_try > 9999:
+            return 1e10
+        fac = math.sqrt(-2.0 * math.log(rsq) / rsq)
+        self.gaussian = v1 * fac
+        self.gaussian_avail = True
+        return v2 * fac * sigma + center
+
+    def u_rand_u64(self, n):
+        """n consecutive rand_gen values as a uint64 array (the same stream as
+        n calls of rand_gen; whole hashes at a time)."""
+        out = np.empty(n, np.uint64)
+        k = 0
+        while k < n and self.cache_avail > 0:
+            out[k] = self.rand_gen()
+            k += 1
+        while k < n:
+            self.index += 1
+            w = self._hash_words(self.index)
+            vals = [w[6] << 32 | w[7], w[4] << 32 | w[5], w[2] << 32 | w[3], w[0] << 32 | w[1]]
+            take = min(4, n - k)
+            out[k:k + take] = vals[:take]
+            k += take
+            self.index += take - 1
+            self.cache = [vals[3], vals[2], vals[1]]
+            self.cache_avail = 4 - take
+            for j in range(3 - self.cache_avail):  # drawn cache entries are cleared
+                self.cache[2 - j] = 0
+        return out
+
+
+def _site_reals(data):
+    a = np.asarray(data)
+    V = a.shape[0]
+    if np.iscomplexobj(a):
+        return a.view(np.float64).reshape(V, -1)
+    return a.reshape(V, -1)
+
+
+def set_u_rand(data, rs, upper=1.0, lower=-1.0):
+    """set_u_rand (core.h:1584-1603) on a (V, ...) float64 / complex128 array
+    in reference site order: site g draws its reals in memory order from
+    rs.newtype(g); returns data."""
+    reals = _site_reals(data)
+    n = reals.shape[1]
+    scale = upper - lower
+    for g in range(reals.shape[0]):
+        u = rs.newtype(g).u_rand_u64(n).astype(np.float64)
+        reals[g] = u * _FAC * scale + lower
+    return data
+
+
+def set_g_rand(data, rs, center=0.0, sigma=1.0):
+    """set_g_rand (core.h:1619-1640): Gaussian reals, site g from rs.newtype(g)."""
+    reals = _site_reals(data)
+    for g in range(reals.shape[0]):
+        rsi = rs.newtype(g)
+        reals[g] = [rsi.g_rand_gen(center, sigma) for _ in range(reals.shape[1])]
+    return data
+
+
+def get_data_sig(data, rs):
+    """Field.get_data_sig (qlat field_base.pyx:111-146): sum over the
+    (V, ...) array of each number times a real u in [-1, 1), the u field
+    filled by set_u_rand(rs, 1, -1) with one real per number (per complex
+    number for complex data)."""
+    a = np.asarray(data)
+    flat = a.reshape(a.shape[0], -1)
+    fu = np.zeros(flat.shape, np.float64)
+    set_u_rand(fu, rs, 1.0, -1.0)
+    return np.sum(flat * fu)
+
+
+def get_data_sig_arr(data, rs, sig_len):
+    """get_data_sig_arr (qlat_utils utils.py:437-451): get_data_sig with
+    rs.split(str(i)) for i < sig_len."""
+    return np.array([get_data_sig(data, rs.split(f"{i}")) for i in range(sig_len)])

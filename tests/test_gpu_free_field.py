@@ -38,3 +38,33 @@ def test_free_invert_matches_fft_restatement(momtwist):
     assert rel(got.data, ref) < 1e-12
     lst = q.InverterDwfFreeField(mass=0.1, m5=0.9, momtwist=momtwist) * [src]
     assert np.array_equal(lst[0].data, got.data)
+
+
+def test_golden_random_source_on_the_gpu():
+    """examples-py/free-invert.py with its random source
+    Prop.set_rand(RngState("seed").split("src_r")): the analytic propagator
+    (free_invert on the GPU), the Ls 16 domain-wall CG and the zMobius CG
+    solutions reproduce the reference's qnorms."""
+    from tests.helpers import ZMOBIUS_OMEGA
+
+    print("\n[gpu] free-invert random-source goldens 4^3x8", flush=True)
+    geo = q.Geometry((4, 4, 4, 8))
+    src = q.Prop(geo).set_rand(q.RngState("seed").split("src_r"))
+    g = GOLD["random_source"]
+    sol = q.InverterDwfFreeField(mass=0.05, m5=1.0) * src
+    v, eps = g["sol_qnorm_analytic"]
+    assert qlat_check(sol.qnorm(), v, eps), sol.qnorm()
+    gf = q.GaugeField(geo).set_unit()
+    sol1 = q.InverterDomainWall(gf=gf, fa=q.FermionAction(mass=0.05, ls=16, m5=1.0)) * src
+    v, eps = g["sol1_qnorm_dwf_cg"]
+    assert qlat_check(sol1.qnorm(), v, eps), sol1.qnorm()
+    d = sol1.copy()
+    d -= sol
+    assert d.qnorm() < 1e-7  # free-invert.py:40
+    solz = q.InverterDomainWall(gf=gf, fa=q.FermionAction(mass=0.05, ls=12, m5=1.0, omega=ZMOBIUS_OMEGA)) * src
+    v, eps = g["sol_z_qnorm_zmobius_cg"]
+    assert qlat_check(solz.qnorm(), v, eps), solz.qnorm()
+    d = solz.copy()
+    d -= sol
+    v, eps = g["sol_z_diff_qnorm"]
+    assert qlat_check(d.qnorm(), v, eps), d.qnorm()

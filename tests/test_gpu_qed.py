@@ -68,3 +68,53 @@ def test_cg_and_invert_match_restatement(data):
     sol_sp, _ = q.invert_qed(sp, u, 0.1, 1.0, LS, SITE, None, False, 1e-10, 3000)
     col2, _ = q.invert_dwf_qed(sp[:, :, 2], u, 0.1, 1.0, LS, SITE, None, False, 1e-10, 3000)
     assert rel(sol_sp[:, :, 2], col2) < 1e-14
+
+
+class _GpuBackend:
+    """The GPU entry points behind examples-py/dslash-qed.py's calls."""
+
+    @staticmethod
+    def mul(f5, gf, mass, m5, ls, dagger):
+        from tests.qed_script import SITE
+
+        return q.multiply_m_dwf_qed(f5, gf, mass, m5, ls, SITE, None, dagger)
+
+    @staticmethod
+    def cg(f5, gf, mass, m5, ls, dagger, max_iter):
+        from tests.qed_script import SITE
+
+        return q.cg_with_m_dwf_qed(f5, gf, mass, m5, ls, SITE, None, dagger, max_num_iter=max_iter)[0]
+
+    @staticmethod
+    def inv4d(f4, gf, mass, m5, ls, dagger, max_iter):
+        from tests.qed_script import SITE
+
+        return q.invert_dwf_qed(f4, gf, mass, m5, ls, SITE, None, dagger, max_num_iter=max_iter)[0]
+
+    @staticmethod
+    def invqed(sp, gf, mass, m5, ls, wick, dagger, max_iter):
+        from tests.qed_script import SITE
+
+        return q.invert_qed(sp, gf, mass, m5, ls, SITE, wick, dagger, max_num_iter=max_iter)[0]
+
+    @staticmethod
+    def free(sp, mass, m5):
+        from tests.qed_script import SITE, V
+
+        prop = q.Prop(q.Geometry(SITE))
+        prop.data = np.einsum("vab,cd->vacbd", sp, np.eye(3)).reshape(V, 12, 12)
+        out = q.free_invert(prop, mass, m5).data.reshape(V, 4, 3, 4, 3)
+        return np.ascontiguousarray(out[:, :, 0, :, 0])
+
+
+@pytest.mark.parametrize("blocks", [(1,), (2,), (3, 4), (5,)])
+def test_dslash_qed_goldens_gpu(blocks):
+    """examples-py/dslash-qed.py's recorded data signatures
+    (dslash-qed.log.json) from the GPU operator and solvers, with the
+    script's fields drawn from the restated RngState."""
+    from tests import qed_script as S
+
+    print(f"\n[gpu] dslash-qed goldens, script blocks {blocks}", flush=True)
+    out = S.replay(_GpuBackend, blocks)
+    bad = S.compare(out, blocks)
+    assert out and not bad, bad
