@@ -713,10 +713,11 @@ long cg_fp64(Inv& I, double2* x, const double2* b, double stop_rsd, long max_num
   return max_num_iter + 1;
 }
 
-// Mixed-precision CG on A x = b (odd half fields): the Krylov recurrence
-// runs in FP32; whenever the iterated residual has dropped by `delta` since
-// the last replacement (and at convergence) the solution increment is
-// folded into the FP64 x and the residual is recomputed as b - A x in FP64.
+// Mixed-precision CG on A x = b (odd half fields) from the x64 it is given:
+// the Krylov recurrence runs in FP32; whenever the iterated residual has
+// dropped by `delta` since the last replacement (and at convergence) the
+// solution increment is folded into the FP64 x and the residual is
+// recomputed as b - A x in FP64.
 // Returns iterations; stops when |b - A x| <= stop_rsd |b| in FP64.
 long cg_mixed(Inv& I, double2* x64, const double2* b64, double stop_rsd, long max_total_iter, double* true_rel)
 {
@@ -798,6 +799,8 @@ long cg_mixed(Inv& I, double2* x64, const double2* b64, double stop_rsd, long ma
 // FP64 reliable updates (residual recomputed as b - A x in FP64 when its
 // iterated residual dropped by delta, and at convergence) and retires when
 // its FP64 true residual reaches stop_rsd.  Returns the summed iterations.
+void deflate_dev(Inv& I, double2* out, const double2* in, int eo);
+
 long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
 {
   const int64_t fs = n32(I), nd = n64(I);
@@ -814,14 +817,34 @@ long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
   std::vector<double> b2(ncol), target(ncol), rr(ncol), maxrr(ncol);
   std::vector<long> iters(ncol, 0);
   std::vector<int> active;
+  auto true_residual = [&](int c) {
+    op_hermop<double>(I, ax64, C64(I, c, 1), 1);
+    I.stats.hermop_fp64 += 1;
+    copy64(I, r64, C64(I, c, 0));
+    qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
+    check_launch();
+    return dev_norm2(I, r64);
+  };
   for (int c = 0; c < ncol; ++c) {
     b2[c] = dev_norm2(I, C64(I, c, 0));
     target[c] = stop_rsd * stop_rsd * b2[c];
     zero64(I, C64(I, c, 1));
-    // x = 0: r = b exactly
-    qdwf::launch_d2f(C64(I, c, 0), R + c * fs, I.ls, I.geo.stride, I.stream);
-    check_launch();
-    rr[c] = maxrr[c] = b2[c];
+    if (I.lm_nvec > 0 && b2[c] > 0.0) {
+      // deflated initial guess (dslash.h:496-500) in the FP64 x, r = b - A x
+      deflate_dev(I, C64(I, c, 1), C64(I, c, 0), 1);
+      rr[c] = maxrr[c] = true_residual(c);
+      qdwf::launch_d2f(r64, R + c * fs, I.ls, I.geo.stride, I.stream);
+      check_launch();
+      if (rr[c] <= target[c]) {
+        const double rel = std::sqrt(rr[c] / b2[c]);
+        if (rel > I.stats.true_residual) I.stats.true_residual = rel;
+      }
+    } else {
+      // x = 0: r = b exactly
+      qdwf::launch_d2f(C64(I, c, 0), R + c * fs, I.ls, I.geo.stride, I.stream);
+      check_launch();
+      rr[c] = maxrr[c] = b2[c];
+    }
     if (rr[c] > target[c]) active.push_back(c);
   }
   QDWF_HIP(hipMemcpyAsync(P, R, field32_bytes(I) * ncol, hipMemcpyDeviceToDevice, I.stream));
@@ -838,14 +861,6 @@ long cg_mixed_multi(Inv& I, int ncol, double stop_rsd, long cap)
     if (all.n) m5<float>(I, T, P, MAT_K, 1.f, nullptr, 0.f, nullptr, &all);
   }
   const double d2 = I.delta * I.delta;
-  auto true_residual = [&](int c) {
-    op_hermop<double>(I, ax64, C64(I, c, 1), 1);
-    I.stats.hermop_fp64 += 1;
-    copy64(I, r64, C64(I, c, 0));
-    qdwf::launch_axpy<double2>(-1.0, ax64, r64, nd, I.stream);
-    check_launch();
-    return dev_norm2(I, r64);
-  };
   double pap[qdwf::kMaxRhs], rr_new[qdwf::kMaxRhs], alpha[qdwf::kMaxRhs], step_alpha[qdwf::kMaxRhs];
   long it = 0;
   while (!active.empty() && it < cap) {
@@ -1605,6 +1620,9 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
     I.stats.cycles += 1;
   } else {
     const long cap = I.max_num_iter * I.max_cycle;
+    // deflated initial guess when low modes are attached (dslash.h:496-500);
+    // cg_mixed starts from the x it is given
+    if (I.lm_nvec > 0) deflate_dev(I, out_o_p, in_o_p, 1);
     total_iter = cg_mixed(I, out_o_p, in_o_p, I.stop_rsd, cap, &true_rel);
     I.stats.cycles += 1;
     I.stats.spinor_bits = 32;

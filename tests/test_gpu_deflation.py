@@ -121,13 +121,16 @@ def test_deflated_fp64_solve(system):
     assert it_defl < it_plain
 
 
-@pytest.mark.parametrize("solver", ["mixed16", "mixed"])
-def test_deflated_mixed_solve(system, solver):
+@pytest.mark.parametrize("solver", ["mixed16", "mixed", "mixed-fused"])
+def test_deflated_mixed_solve(system, solver, monkeypatch):
     """The mixed solvers start from x = deflate(b) with the FP64 residual
-    b - A x, then iterate as without low modes."""
+    b - A x, then iterate as without low modes ("mixed-fused": the FP32
+    solver on the fused kernels, the path taken for Ls outside [6, 16])."""
     U, orc, lm, lam, vec, idx = system
+    if solver == "mixed-fused":
+        monkeypatch.setenv("QLAT_DWF_THREE_PHASE", "0")
     geo, inv = make_inv(U)
-    inv.set_solver(solver)
+    inv.set_solver("mixed" if solver == "mixed-fused" else solver)
     src = q.FermionField4d(geo)
     src.data = rand_field((geo.total_volume(), 12), 84)
     inv.set_stop_rsd(1e-9)
@@ -168,3 +171,27 @@ def test_deflate_with_low_modes_from_files(system, tmp_path):
     proj = (vec @ ((vec.conj().T @ f.data.reshape(-1)) / lam)).reshape(f.data.shape)
     print(f"[gpu] deflation from 16-bit files: rel {rel(got - x32, proj):.2e}", flush=True)
     assert rel(got - x32, proj) < 1e-3
+
+
+def test_deflated_fused_propagator(system, monkeypatch):
+    """The multi-RHS FP32 solver on the fused kernels (cg_mixed_multi) starts
+    every column from x = deflate(b) too: a 12-column propagator reaches the
+    oracle's solution in fewer iterations with the low modes attached."""
+    U, orc, lm, lam, vec, idx = system
+    monkeypatch.setenv("QLAT_DWF_THREE_PHASE", "0")
+    geo, inv = make_inv(U)
+    inv.set_solver("mixed")
+    inv.set_stop_rsd(1e-9)
+    inv.set_max_num_iter(5000)
+    src = q.Prop(geo).set_rand(86)
+    plain = inv * src
+    it_plain = inv.stats()["total_iter"]
+    inv.set_low_modes(lm)
+    defl = inv * src
+    st = inv.stats()
+    orc.set_params(1e-12, 5000, 300)
+    ref, _ = orc.invert_prop(src.data)
+    print(f"[gpu] deflated fused propagator: {st['total_iter']} iterations against {it_plain}", flush=True)
+    assert st["true_residual"] <= 1e-9
+    assert rel(defl.data, ref) < 1e-6 and rel(plain.data, ref) < 1e-6
+    assert st["total_iter"] < it_plain
