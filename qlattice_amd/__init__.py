@@ -84,6 +84,7 @@ from .sources import (
     set_mom_src_propagator,
     set_point_src,
     set_point_src_propagator,
+    set_rand_vol_u1,
     set_rand_vol_u1_src,
     set_tslice_mom_src,
     set_tslice_mom_src_propagator,

@@ -88,6 +88,31 @@ def set_volume_src(prop, geo, lmom):
     return set_mom_src(prop, geo, lmom)
 
 
+PI = 3.141592653589793  # qlat-utils core.h:41
+
+
+def set_rand_vol_u1(fu1, geo, multiplicity, rs):
+    """qcd-prop.cpp:236-258 (propagator.pyx:183-189): fu1 = exp(i phase) per
+    site and component, the phases u_rand_gen(PI, -PI) of rs.newtype(g) at
+    site g in order.  Host code (one SHA-256 per four phases)."""
+    import math
+
+    from .rng import RngState
+
+    if not isinstance(rs, RngState):
+        raise TypeError("set_rand_vol_u1: rs must be an RngState")
+    V = geo.total_volume()
+    data = np.empty((V, int(multiplicity)), np.complex128)
+    for g in range(V):
+        rst = rs.newtype(g)
+        for m in range(int(multiplicity)):
+            ph = rst.u_rand_gen(PI, -PI)
+            data[g, m] = complex(math.cos(ph), math.sin(ph))
+    fu1.geo = geo
+    fu1.data = data
+    return fu1
+
+
 def set_rand_vol_u1_src(prop, fu1):
     """qcd-prop.cpp:260-273: fu1(x) unit matrix (fu1: FieldComplexD with multiplicity 1)."""
     if fu1.data.shape[1:] not in ((), (1,)):

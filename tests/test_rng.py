@@ -109,3 +109,18 @@ def test_golden_random_source_cg(random_source):
                       ("sol_z_diff_qnorm", np.vdot(solz - sol, solz - sol).real)]:
         v, eps = g[name]
         assert qlat_check(val, v, eps), (name, val, v)
+
+
+@pytest.mark.parametrize("mult", [1, 12])
+def test_set_rand_vol_u1(mult):
+    """qcd-prop.cpp:236-258: unit-modulus noise whose phase at site g,
+    component m is the m-th u_rand_gen(PI, -PI) of rs.newtype(g)."""
+    geo = q.Geometry((2, 2, 2, 4))
+    rs = q.RngState("u1")
+    fu1 = q.set_rand_vol_u1(q.FieldComplexD(geo, mult), geo, mult, rs.split("noise"))
+    assert fu1.data.shape == (32, mult)
+    assert np.allclose(np.abs(fu1.data), 1.0, atol=1e-15, rtol=0)
+    for g in (0, 7, 31):
+        rst = rs.split("noise").newtype(g)
+        ph = [rst.u_rand_gen(np.pi, -np.pi) for _ in range(mult)]
+        assert np.allclose(np.angle(fu1.data[g]), ph, atol=1e-14, rtol=0)
