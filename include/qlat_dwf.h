@@ -33,6 +33,8 @@
 #ifndef QLAT_DWF_H
 #define QLAT_DWF_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -252,6 +254,15 @@ int qlat_dwf_deflate(qlat_dwf_inverter* inv, double* out, const double* in, int 
  * inverter.py:26-71); momtwist may be NULL (no twist).  sol may equal src. */
 int qlat_dwf_free_invert(const int total_site[4], double mass, double m5, const double momtwist[4], double* sol,
                          const double* src);
+
+/* set_u_rand (qlat core.h:1584-1603, RngState rng-state.cpp:148-220) on the
+ * GPU: out[g * n_per_site + m] = u * 2^-64 * (upper - lower) + lower for
+ * sites g < n_sites and m < n_per_site, u the m-th rand_gen draw of
+ * rs.newtype(g), where rs is described by the SHA-256 midstate of its
+ * message (midstate, num_bytes: a multiple of 64) and its index (index0).
+ * One SHA-256 block per four draws; out is host memory. */
+int qlat_dwf_rng_fill_uniform(const uint32_t midstate[8], uint64_t num_bytes, uint64_t index0, int64_t n_sites,
+                              int n_per_site, double upper, double lower, double* out);
 
 /* ------------------------------------------------------------------ QED
  * The QED domain-wall operator and its solvers (qed.h:433-463,

@@ -223,6 +223,21 @@ void launch_dft_dir(double2* out, const double2* in, const int n[4], int mu, int
 void launch_free_mom(double2* prop, const int n[4], double mass, double m5, const double momtwist[4], double scale,
                      hipStream_t st);
 
+// set_u_rand (core.h:1584-1603) on the device: out[g][m] for sites g <
+// n_sites and reals m < n_per_site is u * 2^-64 * scale + lower, u the m-th
+// 64-bit draw of the RngState of site g (the state's SHA-256 midstate `mid`
+// over its num_bytes-byte message, type g, draws from index0 + 1).
+struct RngFillArgs {
+  uint32_t mid[8];
+  uint64_t num_bytes;
+  uint64_t index0;
+  int64_t n_sites;
+  int n_per_site;
+  double scale, lower;
+  double* out;
+};
+void launch_rng_fill_u(const RngFillArgs& a, hipStream_t st);
+
 // Block-compressed low modes (deflate, dslash.cpp:144-249).  hmap[b][bidx]
 // is the device half-site of half site bidx of block b; field elements are
 // FP64 SoA half fields of parity eo.

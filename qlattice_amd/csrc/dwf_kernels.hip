@@ -2368,4 +2368,126 @@ void launch_pack_gauge(const double2* gauge, double2* l64, float2* l32, const La
   hipLaunchKernelGGL(k_pack_gauge, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, gauge, l64, l32, g);
 }
 
+
+// ------------------------------------------------------------ RngState fill
+// SHA-256 (FIPS 180-4): the round constants and one compression of a
+// 64-byte block onto a state.
+__constant__ uint32_t kSha256K[64] = {
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+
+__device__ __forceinline__ uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+__device__ void sha256_compress(uint32_t h[8], uint32_t w[16])
+{
+  uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], k = h[7];
+  for (int i = 0; i < 64; ++i) {
+    uint32_t wi;
+    if (i < 16) {
+      wi = w[i];
+    } else {  // message schedule kept in a rolling window of 16 words
+      const uint32_t w15 = w[(i - 15) & 15], w2 = w[(i - 2) & 15];
+      const uint32_t s0 = rotr32(w15, 7) ^ rotr32(w15, 18) ^ (w15 >> 3);
+      const uint32_t s1 = rotr32(w2, 17) ^ rotr32(w2, 19) ^ (w2 >> 10);
+      wi = w[i & 15] + s0 + w[(i - 7) & 15] + s1;
+      w[i & 15] = wi;
+    }
+    const uint32_t S1 = rotr32(e, 6) ^ rotr32(e, 11) ^ rotr32(e, 25);
+    const uint32_t ch = (e & f) ^ (~e & g);
+    const uint32_t t1 = k + S1 + ch + kSha256K[i] + wi;
+    const uint32_t S0 = rotr32(a, 2) ^ rotr32(a, 13) ^ rotr32(a, 22);
+    const uint32_t mj = (a & b) ^ (a & c) ^ (b & c);
+    const uint32_t t2 = S0 + mj;
+    k = g;
+    g = f;
+    f = e;
+    e = d + t1;
+    d = c;
+    c = b;
+    b = a;
+    a = t1 + t2;
+  }
+  h[0] += a;
+  h[1] += b;
+  h[2] += c;
+  h[3] += d;
+  h[4] += e;
+  h[5] += f;
+  h[6] += g;
+  h[7] += k;
+}
+
+// decimal digits of v into buf (at most 20), returns the count
+__device__ int u64_digits(char* buf, uint64_t v)
+{
+  char tmp[20];
+  int n = 0;
+  do {
+    tmp[n++] = (char)('0' + v % 10);
+    v /= 10;
+  } while (v);
+  for (int i = 0; i < n; ++i) buf[i] = tmp[n - 1 - i];
+  return n;
+}
+
+// One thread per (site g, hash k): the draws at indices index0 + 4k + 1 ..
+// + 4 of the state newtype(g) hash "[g,index0 + 4k + 1]" once (rand_gen,
+// rng-state.cpp:186-212: digest words 6-7 first, then the cached 4-5, 2-3,
+// 0-1).  The final block holds the text, 0x80, zeros and the message length
+// in bits (the text is at most 43 bytes, so one block always suffices).
+__global__ void k_rng_fill_u(RngFillArgs a)
+{
+#pragma clang fp contract(off)
+  const int64_t hps = (a.n_per_site + 3) / 4;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_sites * hps) return;
+  const int64_t g = t / hps, kh = t - g * hps;
+  char text[48];
+  int len = 0;
+  text[len++] = '[';
+  len += u64_digits(text + len, (uint64_t)g);
+  text[len++] = ',';
+  len += u64_digits(text + len, a.index0 + 1 + 4 * (uint64_t)kh);
+  text[len++] = ']';
+  uint32_t w[16];
+  for (int i = 0; i < 16; ++i) {
+    uint32_t v = 0;
+    for (int j = 0; j < 4; ++j) {
+      const int b = 4 * i + j;
+      const uint32_t byte = b < len ? (uint32_t)(uint8_t)text[b] : (b == len ? 0x80u : 0u);
+      v = (v << 8) | byte;
+    }
+    w[i] = v;
+  }
+  const uint64_t bits = (a.num_bytes + (uint64_t)len) * 8;
+  w[14] = (uint32_t)(bits >> 32);
+  w[15] = (uint32_t)bits;
+  uint32_t h[8];
+  for (int i = 0; i < 8; ++i) h[i] = a.mid[i];
+  sha256_compress(h, w);
+  const double fac = 1.0 / 18446744073709551616.0;  // 2^-64
+  double* out = a.out + g * a.n_per_site;
+  for (int j = 0; j < 4; ++j) {
+    const int64_t m = 4 * kh + j;
+    if (m >= a.n_per_site) break;
+    const int hi = 6 - 2 * j;
+    const uint64_t u = ((uint64_t)h[hi] << 32) | h[hi + 1];
+    out[m] = (double)u * fac * a.scale + a.lower;
+  }
+}
+
+void launch_rng_fill_u(const RngFillArgs& a, hipStream_t st)
+{
+  const int64_t tot = a.n_sites * ((a.n_per_site + 3) / 4);
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(k_rng_fill_u, dim3((unsigned)((tot + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0, st,
+                     a);
+}
+
 }  // namespace qdwf

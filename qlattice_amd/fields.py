@@ -119,17 +119,17 @@ class _Field:
         self.data *= c
         return self
 
-    def set_rand(self, rng, upper=1.0, lower=0.0):
+    def set_rand(self, rng, upper=1.0, lower=0.0, device=False):
         """Field.set_rand (field_types.pyx.in:194-195 -> set_u_rand,
         core.h:1584-1603): uniform reals in [lower, upper) from rng, site g
-        drawing from rng.newtype(g)."""
-        from .rng import RngState, set_u_rand
+        drawing from rng.newtype(g); device=True draws them on the GPU."""
+        from .rng import RngState, set_u_rand, set_u_rand_device
 
         if not isinstance(rng, RngState):
             raise TypeError("set_rand: rng must be an RngState")
         if getattr(self, "eo", 0) != 0:
             raise NotImplementedError("set_rand: only full-lattice fields")
-        set_u_rand(self.data, rng, upper, lower)
+        (set_u_rand_device if device else set_u_rand)(self.data, rng, upper, lower)
         return self
 
     def set_rand_g(self, rng, center=0.0, sigma=1.0):
@@ -217,13 +217,14 @@ class Prop(_Field):
         if geo is not None:
             self.data = np.zeros((geo.total_volume(), 12, 12), dtype=np.complex128)
 
-    def set_rand(self, rng, upper=1.0, lower=0.0):
-        """With an RngState: the reference's set_u_rand.  With an integer:
-        complex Gaussian entries from numpy's PCG64 seeded with it."""
+    def set_rand(self, rng, upper=1.0, lower=0.0, device=False):
+        """With an RngState: the reference's set_u_rand (on the GPU with
+        device=True).  With an integer: complex Gaussian entries from numpy's
+        PCG64 seeded with it."""
         from .rng import RngState
 
         if isinstance(rng, RngState):
-            return super().set_rand(rng, upper, lower)
+            return super().set_rand(rng, upper, lower, device)
         g = np.random.Generator(np.random.PCG64(rng))
         self.data = (g.standard_normal(self.data.shape)
                      + 1j * g.standard_normal(self.data.shape))

@@ -33,6 +33,7 @@ class RngState:
 
     def __init__(self, seed=None):
         self._sha = hashlib.sha256()
+        self._msg = b""
         self.num_bytes = 0
         self.type = _ULONG_MAX
         self.index = 0
@@ -55,6 +56,7 @@ class RngState:
         sha = rs0._sha.copy()
         sha.update(raw)
         self._sha = sha
+        self._msg = rs0._msg + raw
         self.num_bytes = rs0.num_bytes + n_blocks * 64
         self.type = _ULONG_MAX
         self.index = 0
@@ -156,6 +158,67 @@ class RngState:
             for j in range(3 - self.cache_avail):  # drawn cache entries are cleared
                 self.cache[2 - j] = 0
         return out
+
+
+_SHA256_K = (
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2)
+_SHA256_H0 = (0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a, 0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19)
+
+
+def sha256_compress(h, block):
+    """One SHA-256 compression (FIPS 180-4) of a 64-byte block onto the
+    state h (8 words); returns the new state."""
+    m = 0xFFFFFFFF
+
+    def rotr(x, n):
+        return ((x >> n) | (x << (32 - n))) & m
+
+    w = list(struct.unpack(">16I", block))
+    for i in range(16, 64):
+        s0 = rotr(w[i - 15], 7) ^ rotr(w[i - 15], 18) ^ (w[i - 15] >> 3)
+        s1 = rotr(w[i - 2], 17) ^ rotr(w[i - 2], 19) ^ (w[i - 2] >> 10)
+        w.append((w[i - 16] + s0 + w[i - 7] + s1) & m)
+    a, b, c, d, e, f, g, k = h
+    for i in range(64):
+        t1 = (k + (rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25)) + ((e & f) ^ (~e & g)) + _SHA256_K[i] + w[i]) & m
+        t2 = ((rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22)) + ((a & b) ^ (a & c) ^ (b & c))) & m
+        a, b, c, d, e, f, g, k = (t1 + t2) & m, a, b, c, (d + t1) & m, e, f, g
+    return tuple((x + y) & m for x, y in zip(h, (a, b, c, d, e, f, g, k)))
+
+
+def midstate(rs):
+    """The SHA-256 state of rs after its message (a whole number of blocks)."""
+    h = _SHA256_H0
+    for i in range(0, len(rs._msg), 64):
+        h = sha256_compress(h, rs._msg[i:i + 64])
+    return h
+
+
+def set_u_rand_device(data, rs, upper=1.0, lower=-1.0):
+    """set_u_rand on the GPU (qlat_dwf_rng_fill_uniform): the same numbers
+    as set_u_rand, one SHA-256 block per four draws on the device."""
+    import ctypes
+
+    from . import _lib
+
+    if rs.type != _ULONG_MAX:
+        raise ValueError("set_u_rand: newtype needs a state without a type")
+    reals = _site_reals(data)
+    V, n = reals.shape
+    out = np.empty((V, n), np.float64)
+    mid = (ctypes.c_uint32 * 8)(*midstate(rs))
+    _lib.check(_lib.load().qlat_dwf_rng_fill_uniform(mid, len(rs._msg), rs.index, V, n, float(upper), float(lower),
+                                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "qlat_dwf_rng_fill_uniform")
+    reals[...] = out
+    return data
 
 
 def _site_reals(data):

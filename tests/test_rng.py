@@ -124,3 +124,17 @@ def test_set_rand_vol_u1(mult):
         rst = rs.split("noise").newtype(g)
         ph = [rst.u_rand_gen(np.pi, -np.pi) for _ in range(mult)]
         assert np.allclose(np.angle(fu1.data[g]), ph, atol=1e-14, rtol=0)
+
+
+def test_midstate_matches_hashlib():
+    """The host SHA-256 compression (used to hand a state to the GPU fill)
+    continued with a final block equals hashlib's digest of the message."""
+    from qlattice_amd import rng as R
+
+    rs = q.RngState("seed").split("a" * 70).split(3)
+    tail = b"[12,5]"
+    bits = (len(rs._msg) + len(tail)) * 8
+    block = tail + b"\x80" + bytes(55 - len(tail)) + bits.to_bytes(8, "big")
+    h = R.sha256_compress(R.midstate(rs), block)
+    assert h == struct.unpack(">8I", hashlib.sha256(rs._msg + tail).digest())
+    assert len(rs._msg) == rs.num_bytes == 64 * 4
