@@ -138,3 +138,22 @@ def test_midstate_matches_hashlib():
     h = R.sha256_compress(R.midstate(rs), block)
     assert h == struct.unpack(">8I", hashlib.sha256(rs._msg + tail).digest())
     assert len(rs._msg) == rs.num_bytes == 64 * 4
+
+
+def test_device_fill_rejects_bad_arguments():
+    """qlat_dwf_rng_fill_uniform validates its arguments before any device
+    work (no GPU needed to see the errors)."""
+    import ctypes
+
+    from qlattice_amd import _lib
+
+    L = _lib.load()
+    mid = (ctypes.c_uint32 * 8)()
+    out = np.zeros(8)
+    dp = out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert L.qlat_dwf_rng_fill_uniform(mid, 64, 0, 8, 0, 1.0, 0.0, dp) != 0
+    assert b"n_per_site" in L.qlat_dwf_last_error()
+    assert L.qlat_dwf_rng_fill_uniform(mid, 63, 0, 8, 1, 1.0, 0.0, dp) != 0
+    assert b"64-byte" in L.qlat_dwf_last_error()
+    assert L.qlat_dwf_rng_fill_uniform(None, 64, 0, 8, 1, 1.0, 0.0, dp) != 0
+    assert L.qlat_dwf_rng_fill_uniform(mid, 64, 0, 0, 1, 1.0, 0.0, dp) == 0  # nothing to fill
