@@ -264,6 +264,14 @@ int qlat_dwf_free_invert(const int total_site[4], double mass, double m5, const 
 int qlat_dwf_rng_fill_uniform(const uint32_t midstate[8], uint64_t num_bytes, uint64_t index0, int64_t n_sites,
                               int n_per_site, double upper, double lower, double* out);
 
+/* set_g_rand (qlat core.h:1619-1640, g_rand_gen rng-state.cpp:222-258) on
+ * the GPU with the same state description: out[g * n_per_site + m] is the
+ * m-th g_rand_gen(center, sigma) of rs.newtype(g), one thread per site.  The
+ * logarithm of the polar method is the device's, so values can differ from
+ * the host's in the last bit. */
+int qlat_dwf_rng_fill_gaussian(const uint32_t midstate[8], uint64_t num_bytes, uint64_t index0, int64_t n_sites,
+                               int n_per_site, double center, double sigma, double* out);
+
 /* ------------------------------------------------------------------ QED
  * The QED domain-wall operator and its solvers (qed.h:433-463,
  * qed.cpp:119-411): full (not even-odd) 5D fields [V][ls][4] complex128 in

@@ -75,7 +75,7 @@ from .gauge_io import (
     unitarize,
 )
 from .qed import cg_with_m_dwf_qed, invert_dwf_qed, invert_qed, multiply_m_dwf_qed
-from .rng import RngState, get_data_sig, get_data_sig_arr, set_g_rand, set_u_rand, set_u_rand_device
+from .rng import RngState, get_data_sig, get_data_sig_arr, set_g_rand, set_g_rand_device, set_u_rand, set_u_rand_device
 from .sources import (
     mk_point_src,
     mk_rand_vol_u1_src,

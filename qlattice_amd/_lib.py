@@ -51,6 +51,7 @@ EXPORTED = [
     "qlat_dwf_invert_prop",
     "qlat_dwf_free_invert",
     "qlat_dwf_rng_fill_uniform",
+    "qlat_dwf_rng_fill_gaussian",
     "qlat_dwf_multiply_m_dwf_qed",
     "qlat_dwf_cg_with_m_dwf_qed",
     "qlat_dwf_invert_dwf_qed",
@@ -179,8 +180,9 @@ def load():
     L.qlat_dwf_inverter_set_low_modes.argtypes = [_vp, _ip4, ctypes.c_int, ctypes.c_int, _dp, _fp, _fp]
     L.qlat_dwf_deflate.argtypes = [_vp, _dp, _dp, ctypes.c_int]
     L.qlat_dwf_free_invert.argtypes = [_ip4, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
-    L.qlat_dwf_rng_fill_uniform.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64, ctypes.c_uint64,
-                                            ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+    for name in ("qlat_dwf_rng_fill_uniform", "qlat_dwf_rng_fill_gaussian"):
+        getattr(L, name).argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64, ctypes.c_uint64,
+                                     ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
     _qed = [_ip4, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int]
     L.qlat_dwf_multiply_m_dwf_qed.argtypes = _qed + [_dp, _dp]
     L.qlat_dwf_cg_with_m_dwf_qed.argtypes = _qed + [ctypes.c_double, ctypes.c_long, _dp, _dp, _lp]

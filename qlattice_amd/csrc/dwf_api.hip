@@ -2784,6 +2784,32 @@ int qlat_dwf_rng_fill_uniform(const uint32_t midstate[8], uint64_t num_bytes, ui
   });
 }
 
+int qlat_dwf_rng_fill_gaussian(const uint32_t midstate[8], uint64_t num_bytes, uint64_t index0, int64_t n_sites,
+                               int n_per_site, double center, double sigma, double* out)
+{
+  return guarded([&] {
+    QDWF_REQUIRE(midstate && out, "invalid argument");
+    QDWF_REQUIRE(n_sites >= 0 && n_per_site >= 1, "need n_sites >= 0 and n_per_site >= 1");
+    QDWF_REQUIRE(num_bytes % 64 == 0, "the RngState message is a whole number of 64-byte blocks");
+    QDWF_REQUIRE(n_sites <= (int64_t)1 << 40 && n_per_site <= (1 << 20), "field too large");
+    if (n_sites == 0) return;
+    qdwf::RngFillArgs a{};
+    for (int i = 0; i < 8; ++i) a.mid[i] = midstate[i];
+    a.num_bytes = num_bytes;
+    a.index0 = index0;
+    a.n_sites = n_sites;
+    a.n_per_site = n_per_site;
+    a.scale = sigma;
+    a.lower = center;
+    const size_t bytes = (size_t)n_sites * n_per_site * sizeof(double);
+    DevBuf d(bytes);
+    a.out = (double*)d.p;
+    qdwf::launch_rng_fill_g(a, nullptr);
+    check_launch();
+    QDWF_HIP(hipMemcpy(out, d.p, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
 int qlat_dwf_multiply_m_dwf_qed(const int total_site[4], int ls, const double* gf1, double mass, double m5,
                                 const double* wick, int is_dagger, double* out, const double* in)
 {

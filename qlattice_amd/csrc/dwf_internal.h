@@ -237,6 +237,9 @@ struct RngFillArgs {
   double* out;
 };
 void launch_rng_fill_u(const RngFillArgs& a, hipStream_t st);
+// set_g_rand (core.h:1619-1640): out[g][m] = g_rand_gen(center, sigma) of the
+// same per-site state, one thread per site (scale = sigma, lower = center).
+void launch_rng_fill_g(const RngFillArgs& a, hipStream_t st);
 
 // Block-compressed low modes (deflate, dslash.cpp:144-249).  hmap[b][bidx]
 // is the device half-site of half site bidx of block b; field elements are

@@ -2490,4 +2490,107 @@ void launch_rng_fill_u(const RngFillArgs& a, hipStream_t st)
                      a);
 }
 
+// The RngState of one site as the reference keeps it (rng-state.h:31-40):
+// index, three cached 64-bit words and a spare Gaussian; draws as rand_gen,
+// u_rand_gen and g_rand_gen (rng-state.cpp:186-258).
+struct SiteRng {
+  const uint32_t* mid;
+  uint64_t num_bytes, type, index;
+  uint64_t cache[3];
+  int avail;
+  double gauss;
+  bool gavail;
+};
+
+__device__ uint64_t site_rand(SiteRng& r)
+{
+  r.index += 1;
+  if (r.avail > 0) {
+    r.avail -= 1;
+    return r.cache[r.avail];
+  }
+  char text[48];
+  int len = 0;
+  text[len++] = '[';
+  len += u64_digits(text + len, r.type);
+  text[len++] = ',';
+  len += u64_digits(text + len, r.index);
+  text[len++] = ']';
+  uint32_t w[16];
+  for (int i = 0; i < 16; ++i) {
+    uint32_t v = 0;
+    for (int j = 0; j < 4; ++j) {
+      const int b = 4 * i + j;
+      const uint32_t byte = b < len ? (uint32_t)(uint8_t)text[b] : (b == len ? 0x80u : 0u);
+      v = (v << 8) | byte;
+    }
+    w[i] = v;
+  }
+  const uint64_t bits = (r.num_bytes + (uint64_t)len) * 8;
+  w[14] = (uint32_t)(bits >> 32);
+  w[15] = (uint32_t)bits;
+  uint32_t h[8];
+  for (int i = 0; i < 8; ++i) h[i] = r.mid[i];
+  sha256_compress(h, w);
+  r.cache[0] = ((uint64_t)h[0] << 32) | h[1];
+  r.cache[1] = ((uint64_t)h[2] << 32) | h[3];
+  r.cache[2] = ((uint64_t)h[4] << 32) | h[5];
+  r.avail = 3;
+  return ((uint64_t)h[6] << 32) | h[7];
+}
+
+__device__ double site_u_rand(SiteRng& r, double upper, double lower)
+{
+#pragma clang fp contract(off)
+  const double fac = 1.0 / 18446744073709551616.0;  // 2^-64
+  return (double)site_rand(r) * fac * (upper - lower) + lower;
+}
+
+__device__ double site_g_rand(SiteRng& r, double center, double sigma)
+{
+#pragma clang fp contract(off)
+  r.index += 1;
+  if (r.gavail) {
+    r.gavail = false;
+    return r.gauss * sigma + center;
+  }
+  int num_try = 1;
+  double v1, v2, rsq;
+  do {  // at most 9999 tries, as in the reference
+    v1 = site_u_rand(r, 1.0, -1.0);
+    v2 = site_u_rand(r, 1.0, -1.0);
+    rsq = v1 * v1 + v2 * v2;
+    num_try++;
+  } while ((num_try < 10000) && (rsq >= 1.0 || rsq == 0));
+  if (num_try > 9999) return 1e10;
+  const double fac = sqrt(-2.0 * log(rsq) / rsq);
+  r.gauss = v1 * fac;
+  r.gavail = true;
+  return v2 * fac * sigma + center;
+}
+
+__global__ void k_rng_fill_g(RngFillArgs a)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_sites) return;
+  SiteRng r;
+  r.mid = a.mid;
+  r.num_bytes = a.num_bytes;
+  r.type = (uint64_t)g;
+  r.index = a.index0;
+  r.cache[0] = r.cache[1] = r.cache[2] = 0;
+  r.avail = 0;
+  r.gauss = 0.0;
+  r.gavail = false;
+  double* out = a.out + g * a.n_per_site;
+  for (int m = 0; m < a.n_per_site; ++m) out[m] = site_g_rand(r, a.lower, a.scale);
+}
+
+void launch_rng_fill_g(const RngFillArgs& a, hipStream_t st)
+{
+  if (a.n_sites <= 0) return;
+  hipLaunchKernelGGL(k_rng_fill_g, dim3((unsigned)((a.n_sites + kBlasBlock - 1) / kBlasBlock)), dim3(kBlasBlock), 0,
+                     st, a);
+}
+
 }  // namespace qdwf

@@ -132,16 +132,16 @@ class _Field:
         (set_u_rand_device if device else set_u_rand)(self.data, rng, upper, lower)
         return self
 
-    def set_rand_g(self, rng, center=0.0, sigma=1.0):
+    def set_rand_g(self, rng, center=0.0, sigma=1.0, device=False):
         """Field.set_rand_g (field_types.pyx.in:198-199 -> set_g_rand,
-        core.h:1619-1640)."""
-        from .rng import RngState, set_g_rand
+        core.h:1619-1640); device=True draws on the GPU."""
+        from .rng import RngState, set_g_rand, set_g_rand_device
 
         if not isinstance(rng, RngState):
             raise TypeError("set_rand_g: rng must be an RngState")
         if getattr(self, "eo", 0) != 0:
             raise NotImplementedError("set_rand_g: only full-lattice fields")
-        set_g_rand(self.data, rng, center, sigma)
+        (set_g_rand_device if device else set_g_rand)(self.data, rng, center, sigma)
         return self
 
 

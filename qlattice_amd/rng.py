@@ -201,24 +201,33 @@ def midstate(rs):
     return h
 
 
-def set_u_rand_device(data, rs, upper=1.0, lower=-1.0):
-    """set_u_rand on the GPU (qlat_dwf_rng_fill_uniform): the same numbers
-    as set_u_rand, one SHA-256 block per four draws on the device."""
+def _fill_device(name, data, rs, a, b):
     import ctypes
 
     from . import _lib
 
     if rs.type != _ULONG_MAX:
-        raise ValueError("set_u_rand: newtype needs a state without a type")
+        raise ValueError("newtype needs a state without a type")
     reals = _site_reals(data)
     V, n = reals.shape
     out = np.empty((V, n), np.float64)
     mid = (ctypes.c_uint32 * 8)(*midstate(rs))
-    _lib.check(_lib.load().qlat_dwf_rng_fill_uniform(mid, len(rs._msg), rs.index, V, n, float(upper), float(lower),
-                                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
-               "qlat_dwf_rng_fill_uniform")
+    _lib.check(getattr(_lib.load(), name)(mid, len(rs._msg), rs.index, V, n, float(a), float(b),
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), name)
     reals[...] = out
     return data
+
+
+def set_u_rand_device(data, rs, upper=1.0, lower=-1.0):
+    """set_u_rand on the GPU (qlat_dwf_rng_fill_uniform): the same numbers
+    as set_u_rand, one SHA-256 block per four draws on the device."""
+    return _fill_device("qlat_dwf_rng_fill_uniform", data, rs, upper, lower)
+
+
+def set_g_rand_device(data, rs, center=0.0, sigma=1.0):
+    """set_g_rand on the GPU (qlat_dwf_rng_fill_gaussian), one thread per
+    site; equal to set_g_rand up to the last bit of the device logarithm."""
+    return _fill_device("qlat_dwf_rng_fill_gaussian", data, rs, center, sigma)
 
 
 def _site_reals(data):

@@ -36,3 +36,18 @@ def test_device_fill_after_draws_and_golden_source():
     host = q.Prop(geo).set_rand(rs)
     dev = q.Prop(geo).set_rand(rs, device=True)
     assert np.array_equal(host.data, dev.data)  # the free-invert golden's source
+
+
+@pytest.mark.parametrize("shape,center,sigma", [((32, 12, 12), 0.0, 1.0), ((32, 4, 16), 0.0, 0.3),
+                                                 ((32, 3), 1.0, 0.5)], ids=["prop", "qed-5d", "odd"])
+def test_device_gaussian_fill_matches_host(shape, center, sigma):
+    """The same per-site streams, pairs and rejections as set_g_rand; the
+    device logarithm may differ from the host's in the last bit."""
+    from qlattice_amd.rng import set_g_rand, set_g_rand_device
+
+    print(f"\n[gpu] gaussian fill {shape}", flush=True)
+    rs = q.RngState("gauss").split(str(shape))
+    a = set_g_rand(np.zeros(shape, np.complex128), rs, center, sigma)
+    b = set_g_rand_device(np.zeros(shape, np.complex128), rs, center, sigma)
+    assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(a))
+    assert np.mean(a == b) > 0.9
