@@ -15,7 +15,6 @@ Prints one JSON line (value = mixed-precision time-to-solution in seconds).
 """
 
 import argparse
-import ctypes
 import json
 import os
 import platform
@@ -150,14 +149,12 @@ def bench_config_c(args):
     def run(solver):
         for inv in invs:
             inv.set_solver(solver, args.reliable_delta)
-        hv = (ctypes.c_void_p * n)(*[invs[mi].cdata for mi, _ in pairs])
-        sp = (_lib._dp * n)(*[_lib.dev_ptr(srcs[k]) for _, k in pairs])
-        op = (_lib._dp * n)(*[_lib.dev_ptr(x) for x in sols])
-        it = np.zeros(n, dtype=np.int64)
+        hv = [invs[mi].cdata for mi, _ in pairs]
+        sp = [_lib.dev_ptr(srcs[k]) for _, k in pairs]
+        op = [_lib.dev_ptr(x) for x in sols]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _lib.check(L.qlat_dwf_invert_5d_batch_device(hv, n, op, sp, it.ctypes.data_as(_lib._lp)),
-                   "qlat_dwf_invert_5d_batch_device")
+        it = np.array(L.invert_5d_batch(hv, op, sp, device=True), dtype=np.int64)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, it, invs[0].stats()
 
@@ -181,7 +178,7 @@ def bench_config_c(args):
         "iters_per_mass": {str(cfg["masses"][m]): int(np.mean(it16[[j for j, (mi, _) in enumerate(pairs) if mi == m]]))
                            for m in range(len(invs))},
         "fp64_seconds": t64, "speedup_vs_fp64": t64 / t16, "max_rel_diff_vs_fp64": diff,
-        "true_residual": st16["true_residual"],
+        "true_residual": st16["true_residual"], "true_residual_5d": st16["true_residual_5d"],
     }
     print(json.dumps(line), flush=True)
 
@@ -216,12 +213,10 @@ def bench_config_e(args):
     src = torch.from_numpy(src_h).cuda()
     del src_h
     sol = torch.zeros_like(src)
-    it = ctypes.c_long()
     L = _lib.load()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _lib.check(L.qlat_dwf_invert_prop_device(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src), ctypes.byref(it)),
-               "qlat_dwf_invert_prop_device")
+    L.invert_prop(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src), device=True)
     torch.cuda.synchronize()
     t_prop = time.perf_counter() - t0
     st = inv.stats()
@@ -232,20 +227,15 @@ def bench_config_e(args):
     rng = np.random.default_rng(cfg["seed"] + 2)
     phi = torch.from_numpy(rng.standard_normal((V // 2, ls, 12)) + 1j * rng.standard_normal((V // 2, ls, 12))).cuda()
     force = torch.zeros((V, 4, 3, 3), dtype=torch.complex128, device="cuda")
-    S = ctypes.c_double()
-    S_lo = ctypes.c_double()
-    itf = ctypes.c_long()
     inv.set_solver("fp64")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _lib.check(L.qlat_dwf_fermion_force_device(inv.cdata, _lib.dev_ptr(phi), 1e-10, _lib.dev_ptr(force), None,
-                                               ctypes.byref(S), ctypes.byref(S_lo), ctypes.byref(itf)),
-               "qlat_dwf_fermion_force_device")
+    S, S_lo, itf = L.fermion_force(inv.cdata, _lib.dev_ptr(phi), 1e-10, _lib.dev_ptr(force), 0, device=True)
     torch.cuda.synchronize()
     t_force = time.perf_counter() - t0
     herm = (force + force.conj().transpose(-1, -2)).abs().max().item()
     trace = force.diagonal(dim1=-2, dim2=-1).sum(-1).abs().max().item()
-    log(f"force: {t_force:.2f} s, CG iters {itf.value}, S {S.value:.6e}, anti-hermiticity {herm:.1e}, trace {trace:.1e}")
+    log(f"force: {t_force:.2f} s, CG iters {itf}, S {S:.6e}, anti-hermiticity {herm:.1e}, trace {trace:.1e}")
     line = {
         "metric": "32^3x64 Ls16 single-GPU share: propagator time-to-1e-8 and force time",
         "value": t_prop, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp32",
@@ -254,8 +244,8 @@ def bench_config_e(args):
                                "mass=0.004, mixed16 multi-RHS, stop 1e-8; then one fermion force (FP64 CG 1e-10)",
                    "rhs_tile": st["rhs_tile"]},
         "propagator": {"seconds": t_prop, "iters": st["total_iter"], "reliable_updates": st["reliable_updates"],
-                       "true_residual": st["true_residual"]},
-        "force": {"seconds": t_force, "cg_iters": itf.value, "action": S.value},
+                       "true_residual": st["true_residual"], "true_residual_5d": st["true_residual_5d"]},
+        "force": {"seconds": t_force, "cg_iters": itf, "action": S},
     }
     print(json.dumps(line), flush=True)
 
@@ -267,6 +257,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", default="B", choices=sorted(CONFIGS))
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-single-column", action="store_true",
+                    help="skip the FP32 one-column-at-a-time mixed solve (the configuration's literal form)")
     ap.add_argument("--solver", default="mixed16", choices=["mixed16", "mixed"],
                     help="mixed16: FP16 Krylov storage with per-site scale; mixed: FP32 storage")
     ap.add_argument("--reliable-delta", type=float, default=0.01,
@@ -363,13 +355,11 @@ def main():
     src_h[0] = np.eye(12)
     src = torch.from_numpy(src_h).cuda()
     sol = torch.zeros_like(src)
-    it = ctypes.c_long()
 
     def solve(solver):
         inv.set_solver(solver, args.reliable_delta)
         torch.cuda.synchronize()
-        _lib.check(L.qlat_dwf_invert_prop_device(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src),
-                                                 ctypes.byref(it)), "qlat_dwf_invert_prop_device")
+        L.invert_prop(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src), device=True)
         torch.cuda.synchronize()
         return inv.stats()
 
@@ -401,6 +391,32 @@ def main():
     else:
         diff = None
 
+    # The configuration as stated: the 12 columns solved one at a time with
+    # FP32 dslash/CG and FP64 reliable updates (multi-RHS tile 1, FP32 storage).
+    single = None
+    if not args.skip_single_column:
+        log("FP32 mixed solve, one column at a time")
+        os.environ["QLAT_DWF_RHS_TILE"] = "1"
+        try:
+            inv1 = q.InverterDomainWall(gf=gf, fa=fa, solver="mixed", reliable_delta=args.reliable_delta)
+        finally:
+            del os.environ["QLAT_DWF_RHS_TILE"]
+        inv1.set_stop_rsd(1e-8)
+        sol1 = torch.zeros_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.invert_prop(inv1.cdata, _lib.dev_ptr(sol1), _lib.dev_ptr(src), device=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter() - t0
+        st1 = inv1.stats()
+        d1 = None if st64 is None else (torch.linalg.vector_norm(sol1 - sol) / torch.linalg.vector_norm(sol)).item()
+        single = {"seconds": t1, "iters": st1["total_iter"], "reliable_updates": st1["reliable_updates"],
+                  "true_residual": st1["true_residual"], "true_residual_5d": st1["true_residual_5d"],
+                  "spinor_bits": st1["spinor_bits"], "rhs_tile": 1, "solution_rel_diff_vs_fp64": d1}
+        log(f"fp32 one column at a time: {t1:.2f} s, iters {st1['total_iter']}, true_res {st1['true_residual']:.2e}")
+        del inv1, sol1
+        torch.cuda.empty_cache()
+
     cpu = None
     if args.cpu_hermops > 0:
         iters_ref = st64["total_iter"] if st64 else st_mixed["total_iter"]
@@ -424,18 +440,21 @@ def main():
         "dtype": "fp32",
         "data": "synthetic: seeded exp(i*0.3*H) SU(3) links, antiperiodic t, point sources",
         "config": {
-            "workload": f"12-column point-source propagator, {site[0]}^{3}x{site[3]}, Ls={ls}, Moebius b=1.5 c=0.5, "
-                        f"M5=1.8, mass=0.01, stop 1e-8 (FP64 true residual); columns solved as one multi-RHS "
-                        f"block of {tile}, FP32 arithmetic, {prec_tp}-bit Krylov storage"
+            "workload": f"12-column point-source propagator {site[0]}^3x{site[3]} Ls={ls} Moebius b=1.5 c=0.5 "
+                        f"M5=1.8 mass=0.01 to a 1e-8 FP64 true residual, columns solved in multi-RHS "
+                        f"blocks of {tile}, FP32 arithmetic, {prec_tp}-bit Krylov storage"
                         + (" with per-site scale" if prec_tp == 16 else "") + ", FP64 reliable updates",
             "lattice": list(site), "ls": ls, "columns": ncol, "stop_rsd": 1e-8, "rhs_tile": tile,
             "solver": args.solver,
         },
         "mixed": {"seconds": t_mixed, "iters": st_mixed["total_iter"],
                   "reliable_updates": st_mixed["reliable_updates"],
-                  "true_residual": st_mixed["true_residual"], "spinor_bits": st_mixed["spinor_bits"]},
+                  "true_residual": st_mixed["true_residual"], "true_residual_5d": st_mixed["true_residual_5d"],
+                  "spinor_bits": st_mixed["spinor_bits"]},
         "fp64": None if st64 is None else {"seconds": t_fp64, "iters": st64["total_iter"],
-                                           "cycles": st64["cycles"], "true_residual": st64["true_residual"]},
+                                           "cycles": st64["cycles"], "true_residual": st64["true_residual"],
+                                           "true_residual_5d": st64["true_residual_5d"]},
+        "fp32_single_column": single,
         "speedup_vs_fp64": None if t_fp64 is None else t_fp64 / t_mixed,
         "solution_rel_diff_mixed_vs_fp64": diff,
         "dslash": {

@@ -82,6 +82,10 @@ typedef struct {
                              0 for the FP64 solver                              */
   int rhs_tile;           /* columns per multi-RHS block of the last batched solve */
   int sites_per_block;    /* output sites per block of its hop kernels            */
+  double true_residual_5d; /* max over columns of |M x - D_- b| / |D_- b| on the full
+                              5D system, the is_checking_invert check of
+                              invert_with_cg (dslash.h:522-528); 0 when the check
+                              is off (qlat_dwf_set_checking_invert)              */
 } qlat_dwf_solve_stats;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -114,6 +118,13 @@ int qlat_dwf_inverter_get_params(const qlat_dwf_inverter* inv, double* stop_rsd,
 int qlat_dwf_inverter_set_solver(qlat_dwf_inverter* inv, int solver, double reliable_delta);
 
 int qlat_dwf_inverter_get_stats(const qlat_dwf_inverter* inv, qlat_dwf_solve_stats* stats);
+
+/* is_checking_invert() (dslash.h:11-16; on by default): after each solve
+ * compute |M x - D_- b| / |D_- b| on the full 5D system (invert_with_cg,
+ * dslash.h:522-528) into stats.true_residual_5d.  Process-wide, like the
+ * reference's static flag.  qlat_dwf_is_checking_invert returns 0 or 1. */
+int qlat_dwf_set_checking_invert(int on);
+int qlat_dwf_is_checking_invert(void);
 
 /* ------------------------------------------- operators on 5D half fields
  * `out` and `in` are [V/2][Ls][12]; out may equal in ("out can be the same
@@ -178,6 +189,13 @@ int qlat_dwf_cg_with_herm_sym_2(qlat_dwf_inverter* inv, double* sol, const doubl
 /* invert(FermionField5d&, const FermionField5d&, const InverterDomainWall&)
  * (dslash.h:580-581; dslash.cpp:1335-1340 -> invert_with_cg, dslash.h:440-530) */
 int qlat_dwf_invert_5d(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
+
+/* invert_with_cg_with_guess (dslash.h:532-573) with the inverter's solver:
+ * `out` ([V][Ls][12], full 5D) holds the initial guess x0 on entry; the
+ * correction is solved from D_- in - M x0 with stop_rsd scaled by
+ * |D_- in| / |D_- in - M x0| and added to out.  iter: the correction's
+ * iterations (0 when x0 solves the system exactly). */
+int qlat_dwf_invert_5d_with_guess(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
 
 /* invert(FermionField4d&, ...) (dslash.h:583-584; dslash.cpp:1342-1347 ->
  * invert_dwf, qcd-prop.h:146-167) */
@@ -294,10 +312,20 @@ int qlat_dwf_invert_dwf_qed(const int total_site[4], int ls, const double* gf1, 
                             const double* wick, int is_dagger, double stop_rsd, long max_num_iter, double* out4d,
                             const double* in4d, long* iter);
 
-/* find_max_eigen_value_hermop_sym2 (dslash.h:586-588; dslash.cpp:1349-1369)
+/* The power iteration of find_max_eigen_value_hermop_sym2 (dslash.cpp:1349-1369)
  * from a caller-supplied odd half-field start vector. */
 int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,
                                               double* result);
+
+/* find_max_eigen_value_hermop_sym2(inv, rs, max_iter) (dslash.h:586-588;
+ * dslash.cpp:1349-1369): the start vector is set_u_rand(ff, rs) on the odd
+ * half field (uniform in [-1, 1), site x drawing from rs.newtype(global index
+ * of x), core.h:1584-1603), drawn on the GPU; rs is described as for
+ * qlat_dwf_rng_fill_uniform (SHA-256 midstate of its message, message
+ * length num_bytes, index0). */
+int qlat_dwf_find_max_eigen_value_hermop_sym2_rng(qlat_dwf_inverter* inv, const uint32_t midstate[8],
+                                                  uint64_t num_bytes, uint64_t index0, long max_iter,
+                                                  double* result);
 
 /* ------------------------------------------------------------- batches
  * No single counterpart in the reference, which solves one right-hand side
@@ -349,6 +377,7 @@ int qlat_dwf_inverter_set_gauge(qlat_dwf_inverter* inv, const double* gauge, int
  * Same semantics as above; all field pointers are device memory. */
 int qlat_dwf_multiply_hermop_sym2_device(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
 int qlat_dwf_invert_5d_device(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
+int qlat_dwf_invert_5d_with_guess_device(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
 int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter);
 int qlat_dwf_make_source_device(const int total_site[4], const qlat_dwf_source* src, double* prop);
 int qlat_dwf_source_propagator_device(qlat_dwf_inverter* inv, const qlat_dwf_source* src, double* sol,

@@ -800,15 +800,16 @@ void restore_field_from_odd_prec_sym2(Field& out, const Field& out_o_p,
   set_half_fermion(out, out_o, 1, inv);
 }
 
-long invert_with_cg(Field& out, const Field& in, OInv& inv, bool verbose)
+long invert_with_cg(Field& out, const Field& in, OInv& inv, bool verbose, double stop_rsd = -1,
+                    bool dminus_multiplied_already = false)
 // invert_with_cg (dslash.h:440-530) with cg = cg_with_herm_sym_2 and the
 // default cg_diagonal_mee == 2; invert(5d) (dslash.cpp:1335-1340).
 {
-  const double stop_rsd = inv.stop_rsd;
+  if (stop_rsd < 0) stop_rsd = inv.stop_rsd;
   const long max_num_iter = inv.max_num_iter;
   const long max_cycle = inv.max_mixed_precision_cycle;
   Field dm_in;
-  if (inv.is_multiplying_dminus) {
+  if (!dminus_multiplied_already && inv.is_multiplying_dminus) {
     multiply_d_minus(dm_in, in, inv);
   } else {
     dm_in = in;
@@ -845,6 +846,27 @@ long invert_with_cg(Field& out, const Field& in, OInv& inv, bool verbose)
   inv.last_total_iter = total_iter;
   inv.last_cycles = cycle;
   restore_field_from_odd_prec_sym2(out, out_o_p, out_e_p, inv);
+  return total_iter;
+}
+
+long invert_with_cg_with_guess(Field& out, const Field& in, OInv& inv, bool verbose)
+// invert_with_cg_with_guess (dslash.h:532-573) with cg = cg_with_herm_sym_2
+// and the inverter's parameters: `out` holds the guess on entry.
+{
+  Field dm_in;
+  if (inv.is_multiplying_dminus) {
+    multiply_d_minus(dm_in, in, inv);
+  } else {
+    dm_in = in;
+  }
+  const double qnorm_dm_in = qnorm(dm_in);
+  Field tmp;
+  multiply_m(tmp, out, inv, false);
+  sub_from(dm_in, tmp);
+  const double qnorm_dm_in_sub = qnorm(dm_in);
+  const double ratio = std::sqrt(qnorm_dm_in / qnorm_dm_in_sub);
+  const long total_iter = invert_with_cg(tmp, dm_in, inv, verbose, inv.stop_rsd * ratio, true);
+  add_to(out, tmp);
   return total_iter;
 }
 
@@ -1021,6 +1043,17 @@ long oracle_invert_5d(void* p, double* out, const double* in, int verbose)
   const long n = inv->geo.V * inv->ls;
   Field fin = field_from(in, n), fout;
   const long it = invert_with_cg(fout, fin, *inv, verbose);
+  field_to(out, fout);
+  return it;
+}
+
+long oracle_invert_5d_with_guess(void* p, double* out, const double* in, int verbose)
+// invert_with_cg_with_guess (dslash.h:532-573): `out` holds the guess
+{
+  OInv* inv = (OInv*)p;
+  const long n = inv->geo.V * inv->ls;
+  Field fin = field_from(in, n), fout = field_from(out, n);
+  const long it = invert_with_cg_with_guess(fout, fin, *inv, verbose);
   field_to(out, fout);
   return it;
 }

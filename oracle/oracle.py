@@ -89,7 +89,7 @@ def lib():
             ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_long,
             ctypes.c_int,
         ]
-        for name in ("oracle_invert_5d", "oracle_invert_4d", "oracle_invert_prop"):
+        for name in ("oracle_invert_5d", "oracle_invert_5d_with_guess", "oracle_invert_4d", "oracle_invert_prop"):
             f = getattr(L, name)
             f.restype = ctypes.c_long
             f.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
@@ -202,6 +202,14 @@ class OracleInverter:
         src = np.ascontiguousarray(src, dtype=np.complex128)
         out = np.empty_like(src)
         it = lib().oracle_invert_5d(self._h, _ptr(out), _ptr(src), int(verbose))
+        return out, it
+
+    def invert_5d_with_guess(self, src, guess, verbose=False):
+        """invert_with_cg_with_guess (dslash.h:532-573); returns (solution, iterations)."""
+        src = np.ascontiguousarray(src, dtype=np.complex128)
+        out = np.array(guess, dtype=np.complex128, order="C", copy=True)
+        assert out.shape == src.shape
+        it = lib().oracle_invert_5d_with_guess(self._h, _ptr(out), _ptr(src), int(verbose))
         return out, it
 
     def invert_4d(self, src, verbose=False):

@@ -6,7 +6,8 @@ Python surface as ``qlat`` for it: ``FermionAction``, ``InverterDomainWall``,
 ``GaugeField``, ``Prop``, the point / wall / momentum / U(1) sources and
 their ``*_src_propagator`` solves, and the ``multiply_*`` / ``invert`` /
 ``cg_with_herm_sym_2`` functions.  All arithmetic runs in the
-hand-written HIP library ``libqlat_dwf_hip.so`` (C ABI: include/qlat_dwf.h).
+hand-written HIP library ``libqlat_dwf_hip.so`` (C ABI: include/qlat_dwf.h),
+reached through the Cython binding ``_cdwf``.
 """
 
 from .fermion_action import FermionAction
@@ -38,6 +39,7 @@ from .inverter import (
     free_invert,
     invert,
     invert_batch,
+    invert_with_cg_with_guess,
     multiply_d_minus,
     multiply_hermop_sym2,
     multiply_hermop_sym2_fp32,

@@ -1,15 +1,16 @@
-"""ctypes binding of the C ABI in include/qlat_dwf.h.
+"""Access to the HIP library through its Cython binding (_cdwf.pyx).
 
-The shared object is the in-tree build ``qlattice_amd/libqlat_dwf_hip.so``
-(``python -m qlattice_amd.build``).  There is no fallback: if the library is
-missing or cannot be loaded every entry point raises, so nothing in this
-package silently computes on the CPU.
+The shared objects are the in-tree builds ``qlattice_amd/libqlat_dwf_hip.so``
+and ``qlattice_amd/_cdwf.<abi>.so`` (``python -m qlattice_amd.build``).
+There is no fallback: if either is missing or cannot be loaded every entry
+point raises, so nothing in this package silently computes on the CPU.
 """
 
-import ctypes
 import os
 
 import numpy as np
+
+from ._errors import QlatDwfError
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqlat_dwf_hip.so")
@@ -17,8 +18,10 @@ LIB_PATH = os.path.join(HERE, "libqlat_dwf_hip.so")
 QLAT_DWF_SOLVER_FP64 = 0
 QLAT_DWF_SOLVER_MIXED_FP32 = 1
 QLAT_DWF_SOLVER_MIXED_FP16 = 2
+SRC_POINT, SRC_WALL, SRC_MOM, SRC_U1 = 0, 1, 2, 3
 
-# Every symbol declared in include/qlat_dwf.h.
+# Every symbol declared in include/qlat_dwf.h (checked against the exports of
+# the built library by the tests).
 EXPORTED = [
     "qlat_dwf_last_error",
     "qlat_dwf_abi_version",
@@ -28,6 +31,8 @@ EXPORTED = [
     "qlat_dwf_inverter_get_params",
     "qlat_dwf_inverter_set_solver",
     "qlat_dwf_inverter_get_stats",
+    "qlat_dwf_set_checking_invert",
+    "qlat_dwf_is_checking_invert",
     "qlat_dwf_multiply_m_e_e",
     "qlat_dwf_multiply_mdag_e_e",
     "qlat_dwf_multiply_m_e_e_inv",
@@ -47,6 +52,7 @@ EXPORTED = [
     "qlat_dwf_qnorm",
     "qlat_dwf_cg_with_herm_sym_2",
     "qlat_dwf_invert_5d",
+    "qlat_dwf_invert_5d_with_guess",
     "qlat_dwf_invert_4d",
     "qlat_dwf_invert_prop",
     "qlat_dwf_free_invert",
@@ -62,8 +68,10 @@ EXPORTED = [
     "qlat_dwf_source_propagator",
     "qlat_dwf_source_propagator_device",
     "qlat_dwf_find_max_eigen_value_hermop_sym2",
+    "qlat_dwf_find_max_eigen_value_hermop_sym2_rng",
     "qlat_dwf_multiply_hermop_sym2_device",
     "qlat_dwf_invert_5d_device",
+    "qlat_dwf_invert_5d_with_guess_device",
     "qlat_dwf_invert_prop_device",
     "qlat_dwf_bench_kernel",
     "qlat_dwf_fermion_force",
@@ -74,149 +82,49 @@ EXPORTED = [
 ]
 
 
-class FermionActionC(ctypes.Structure):
-    _fields_ = [
-        ("mass", ctypes.c_double),
-        ("ls", ctypes.c_int),
-        ("m5", ctypes.c_double),
-        ("mobius_scale", ctypes.c_double),
-        ("is_multiplying_dminus", ctypes.c_int),
-        ("is_using_zmobius", ctypes.c_int),
-        ("cg_diagonal_mee", ctypes.c_int),
-        ("bs", ctypes.POINTER(ctypes.c_double)),
-        ("cs", ctypes.POINTER(ctypes.c_double)),
-    ]
-
-
-class SolveStatsC(ctypes.Structure):
-    _fields_ = [
-        ("total_iter", ctypes.c_long),
-        ("cycles", ctypes.c_long),
-        ("reliable_updates", ctypes.c_long),
-        ("hermop_fp64", ctypes.c_long),
-        ("hermop_low", ctypes.c_long),
-        ("true_residual", ctypes.c_double),
-        ("seconds", ctypes.c_double),
-        ("spinor_bits", ctypes.c_int),
-        ("rhs_tile", ctypes.c_int),
-        ("sites_per_block", ctypes.c_int),
-    ]
-
-
-class SourceC(ctypes.Structure):
-    """qlat_dwf_source (include/qlat_dwf.h)."""
-    _fields_ = [
-        ("kind", ctypes.c_int),
-        ("xg", ctypes.c_int * 4),
-        ("tslice", ctypes.c_int),
-        ("lmom", ctypes.c_double * 4),
-        ("value", ctypes.c_double * 2),
-        ("fu1", ctypes.c_void_p),
-    ]
-
-
-SRC_POINT, SRC_WALL, SRC_MOM, SRC_U1 = 0, 1, 2, 3
-
-_lib = None
-_dp = ctypes.POINTER(ctypes.c_double)
-_vp = ctypes.c_void_p
-_lp = ctypes.POINTER(ctypes.c_long)
-
-
-class QlatDwfError(RuntimeError):
-    """Raised when a library call returns a non-zero status (the reference
-    raises RuntimeError from its C-API, cqlat/lib.h:28-35)."""
+_mod = None
 
 
 def load():
-    """Load the HIP library, or raise if it is not built."""
-    global _lib
-    if _lib is not None:
-        return _lib
+    """The Cython binding module (_cdwf), or raise if it is not built."""
+    global _mod
+    if _mod is not None:
+        return _mod
     if not os.path.exists(LIB_PATH):
-        raise QlatDwfError(
-            f"{LIB_PATH} is missing: build it with `python -m qlattice_amd.build`"
-        )
-    L = ctypes.CDLL(LIB_PATH)
-    L.qlat_dwf_last_error.restype = ctypes.c_char_p
-    L.qlat_dwf_last_error.argtypes = []
-    L.qlat_dwf_abi_version.restype = ctypes.c_int
-    L.qlat_dwf_inverter_create.argtypes = [
-        ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int), _dp,
-        ctypes.POINTER(FermionActionC),
-    ]
-    L.qlat_dwf_inverter_destroy.argtypes = [_vp]
-    L.qlat_dwf_inverter_set_params.argtypes = [_vp, ctypes.c_double, ctypes.c_long, ctypes.c_long]
-    L.qlat_dwf_inverter_get_params.argtypes = [_vp, _dp, _lp, _lp]
-    L.qlat_dwf_inverter_set_solver.argtypes = [_vp, ctypes.c_int, ctypes.c_double]
-    L.qlat_dwf_inverter_get_stats.argtypes = [_vp, ctypes.POINTER(SolveStatsC)]
-    for name in ("qlat_dwf_multiply_m_e_e", "qlat_dwf_multiply_mdag_e_e",
-                 "qlat_dwf_multiply_m_e_e_inv", "qlat_dwf_multiply_mdag_e_e_inv",
-                 "qlat_dwf_multiply_m", "qlat_dwf_multiply_mdag",
-                 "qlat_dwf_multiply_d_minus"):
-        getattr(L, name).argtypes = [_vp, _dp, _dp]
-    for name in ("qlat_dwf_multiply_wilson_d_e_o", "qlat_dwf_multiply_wilson_ddag_e_o",
-                 "qlat_dwf_multiply_m_e_o", "qlat_dwf_multiply_mdag_e_o",
-                 "qlat_dwf_multiply_mpc_sym2", "qlat_dwf_multiply_mpcdag_sym2",
-                 "qlat_dwf_multiply_hermop_sym2", "qlat_dwf_multiply_hermop_sym2_device",
-                 "qlat_dwf_multiply_hermop_sym2_fp32"):
-        getattr(L, name).argtypes = [_vp, _dp, _dp, ctypes.c_int]
-    L.qlat_dwf_dot_product.argtypes = [_vp, _dp, _dp, ctypes.c_long, _dp, _dp]
-    L.qlat_dwf_qnorm.argtypes = [_vp, _dp, ctypes.c_long, _dp]
-    L.qlat_dwf_cg_with_herm_sym_2.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_long, _lp]
-    for name in ("qlat_dwf_invert_5d", "qlat_dwf_invert_4d", "qlat_dwf_invert_prop",
-                 "qlat_dwf_invert_5d_device", "qlat_dwf_invert_prop_device"):
-        getattr(L, name).argtypes = [_vp, _dp, _dp, _lp]
-    L.qlat_dwf_find_max_eigen_value_hermop_sym2.argtypes = [_vp, _dp, ctypes.c_long, _dp]
-    L.qlat_dwf_bench_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
-    for name in ("qlat_dwf_fermion_force", "qlat_dwf_fermion_force_device"):
-        getattr(L, name).argtypes = [_vp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _lp]
-    L.qlat_dwf_inverter_set_gauge.argtypes = [_vp, _dp, ctypes.c_int]
-    for name in ("qlat_dwf_invert_5d_batch", "qlat_dwf_invert_5d_batch_device"):
-        getattr(L, name).argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_dp),
-                                     ctypes.POINTER(_dp), _lp]
-    _ip4 = ctypes.POINTER(ctypes.c_int)
-    _fp = ctypes.POINTER(ctypes.c_float)
-    L.qlat_dwf_inverter_set_low_modes.argtypes = [_vp, _ip4, ctypes.c_int, ctypes.c_int, _dp, _fp, _fp]
-    L.qlat_dwf_deflate.argtypes = [_vp, _dp, _dp, ctypes.c_int]
-    L.qlat_dwf_free_invert.argtypes = [_ip4, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
-    for name in ("qlat_dwf_rng_fill_uniform", "qlat_dwf_rng_fill_gaussian"):
-        getattr(L, name).argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64, ctypes.c_uint64,
-                                     ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
-    _qed = [_ip4, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int]
-    L.qlat_dwf_multiply_m_dwf_qed.argtypes = _qed + [_dp, _dp]
-    L.qlat_dwf_cg_with_m_dwf_qed.argtypes = _qed + [ctypes.c_double, ctypes.c_long, _dp, _dp, _lp]
-    L.qlat_dwf_invert_dwf_qed.argtypes = _qed + [ctypes.c_double, ctypes.c_long, _dp, _dp, _lp]
-    L.qlat_dwf_make_source.argtypes = [_ip4, ctypes.POINTER(SourceC), _dp]
-    L.qlat_dwf_make_source_device.argtypes = [_ip4, ctypes.POINTER(SourceC), _dp]
-    L.qlat_dwf_source_propagator.argtypes = [_vp, ctypes.POINTER(SourceC), _dp, _lp]
-    L.qlat_dwf_source_propagator_device.argtypes = [_vp, ctypes.POINTER(SourceC), _dp, _lp]
-    for name in EXPORTED:
-        if name not in ("qlat_dwf_last_error", "qlat_dwf_abi_version"):
-            getattr(L, name).restype = ctypes.c_int
-    _lib = L
-    return L
-
-
-def check(rc, what):
-    if rc != 0:
-        msg = load().qlat_dwf_last_error().decode(errors="replace")
-        raise QlatDwfError(f"{what} failed (status {rc}): {msg}")
+        raise QlatDwfError(f"{LIB_PATH} is missing: build it with `python -m qlattice_amd.build`")
+    try:
+        from . import _cdwf
+    except ImportError as e:
+        raise QlatDwfError(f"the Cython binding qlattice_amd/_cdwf is not built or cannot load ({e}): "
+                           "build it with `python -m qlattice_amd.build`") from e
+    _mod = _cdwf
+    return _mod
 
 
 def ptr(a):
-    """Pointer to a C-contiguous complex128 numpy array."""
+    """Address of a C-contiguous complex128 numpy array."""
     if not isinstance(a, np.ndarray) or a.dtype != np.complex128 or not a.flags.c_contiguous:
         raise TypeError("expected a C-contiguous complex128 numpy array")
-    return a.ctypes.data_as(_dp)
+    return a.ctypes.data
+
+
+def addr(a, dtype=None):
+    """Address of a C-contiguous numpy array (of `dtype` when given); 0 for None."""
+    if a is None:
+        return 0
+    if not isinstance(a, np.ndarray) or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous numpy array")
+    if dtype is not None and a.dtype != np.dtype(dtype):
+        raise TypeError(f"expected a {np.dtype(dtype)} array, got {a.dtype}")
+    return a.ctypes.data
 
 
 def dev_ptr(t):
-    """Pointer to a contiguous complex128 torch tensor on the GPU (device
+    """Address of a contiguous complex128 torch tensor on the GPU (device
     memory is plumbing only; the computation is in the HIP library)."""
     import torch
 
     if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex128
             and t.is_contiguous()):
         raise TypeError("expected a contiguous complex128 CUDA/HIP tensor")
-    return ctypes.cast(ctypes.c_void_p(t.data_ptr()), _dp)
+    return t.data_ptr()

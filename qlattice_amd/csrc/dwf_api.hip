@@ -145,7 +145,7 @@ Mat diag_plus(const std::vector<cd>& d, const std::vector<cd>& e, const Mat& S, 
 // ---------------------------------------------------------------- state
 enum Work64 {
   W_IN_E, W_IN_O, W_DM_E, W_DM_O, W_OUT_EP, W_IN_OP, W_OUT_OP, W_TMP, W_ITMP, W_OUT_E, W_OUT_O,
-  W_CG_R, W_CG_P, W_CG_AP, W_CG_X, W_CG_T, W_OP_T1, W_OP_T2, W_OP_T3, W_OP_Y, W_COUNT64
+  W_CG_R, W_CG_P, W_CG_AP, W_CG_X, W_CG_T, W_OP_T1, W_OP_T2, W_OP_T3, W_OP_Y, W_G_E, W_G_O, W_COUNT64
 };
 enum Work32 { V_R, V_P, V_AP, V_X, V_T, V_T1, V_T2, V_T3, V_Y, W_COUNT32 };
 
@@ -1551,13 +1551,42 @@ double true_residual_fp64(Inv& I, const double2* x, const double2* b)
   return b2 > 0 ? std::sqrt(dev_norm2(I, r) / b2) : 0.0;
 }
 
-// invert_with_cg (dslash.h:440-530) on device halves; in_e/in_o: 5D source
-// halves (FP64), results in out_e/out_o.
-long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* out_e, double2* out_o)
+// The reference's is_checking_invert() (dslash.h:11-16, default true):
+// after each solve, |M x - D_- b| / |D_- b| on the full 5D system
+// (dslash.h:522-528) goes into the solve statistics.
+bool g_checking_invert = true;
+
+// |M x - dm| / |dm| over both parities (multiply_m, dslash.cpp:1062-1080);
+// uses W_TMP and W_ITMP.
+double residual_5d(Inv& I, const double2* x_e, const double2* x_o, const double2* dm_e, const double2* dm_o)
 {
+  double2* re = W(I, W_TMP);
+  double2* ro = W(I, W_ITMP);
+  op_m_full(I, re, ro, x_e, x_o);
+  qdwf::launch_axpy<double2>(-1.0, dm_e, re, n64(I), I.stream);
+  qdwf::launch_axpy<double2>(-1.0, dm_o, ro, n64(I), I.stream);
+  check_launch();
+  const double num = dev_norm2(I, re) + dev_norm2(I, ro);
+  const double den = dev_norm2(I, dm_e) + dev_norm2(I, dm_o);
+  return den > 0 ? std::sqrt(num / den) : 0.0;
+}
+
+void note_residual_5d(Inv& I, double r)
+{
+  if (r > I.stats.true_residual_5d) I.stats.true_residual_5d = r;
+}
+
+// invert_with_cg (dslash.h:440-530) on device halves; in_e/in_o: 5D source
+// halves (FP64), results in out_e/out_o.  stop_rsd < 0 selects the
+// inverter's; dminus_already: the source is D_- b already (the
+// dminus_multiplied_already argument, as invert_with_cg_with_guess passes it).
+long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* out_e, double2* out_o,
+                   double stop_rsd = -1.0, bool dminus_already = false)
+{
+  if (stop_rsd < 0) stop_rsd = I.stop_rsd;
   double2* dm_e = W(I, W_DM_E);
   double2* dm_o = W(I, W_DM_O);
-  if (I.dminus) {
+  if (I.dminus && !dminus_already) {
     op_d_minus(I, dm_e, dm_o, in_e, in_o);
   } else {
     copy64(I, dm_e, in_e);
@@ -1597,7 +1626,7 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
       if (guess) deflate_dev(I, x, itmp, 1);
       else zero64(I, x);
       const long iter =
-          cg_fp64(I, x, itmp, I.stop_rsd * std::sqrt(qnorm_in_o_p / qnorm_itmp), I.max_num_iter, !guess);
+          cg_fp64(I, x, itmp, stop_rsd * std::sqrt(qnorm_in_o_p / qnorm_itmp), I.max_num_iter, !guess);
       total_iter += iter;
       qdwf::launch_axpy<double2>(1.0, x, out_o_p, n64(I), I.stream);
       check_launch();
@@ -1614,7 +1643,7 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
     const long cap = I.max_num_iter * I.max_cycle;
     const double saved = I.stats.true_residual;
     I.stats.true_residual = 0.0;
-    total_iter = cg_mixed_multi_s(I, 1, &in_o_p, &out_o_p, I.stop_rsd, cap, storage_of(I));
+    total_iter = cg_mixed_multi_s(I, 1, &in_o_p, &out_o_p, stop_rsd, cap, storage_of(I));
     true_rel = I.stats.true_residual;
     I.stats.true_residual = saved;
     I.stats.cycles += 1;
@@ -1623,7 +1652,7 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
     // deflated initial guess when low modes are attached (dslash.h:496-500);
     // cg_mixed starts from the x it is given
     if (I.lm_nvec > 0) deflate_dev(I, out_o_p, in_o_p, 1);
-    total_iter = cg_mixed(I, out_o_p, in_o_p, I.stop_rsd, cap, &true_rel);
+    total_iter = cg_mixed(I, out_o_p, in_o_p, stop_rsd, cap, &true_rel);
     I.stats.cycles += 1;
     I.stats.spinor_bits = 32;
   }
@@ -1636,7 +1665,50 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
   copy64(I, out_e, out_e_p);
   qdwf::launch_axpy<double2>(-1.0, W(I, W_ITMP), out_e, n64(I), I.stream);
   check_launch();
+  if (g_checking_invert) note_residual_5d(I, residual_5d(I, out_e, out_o, dm_e, dm_o));
   return total_iter;
+}
+
+// invert_with_cg_with_guess (dslash.h:532-573) on device halves: out_e/out_o
+// hold the guess on entry and the solution on exit.  The correction is
+// solved from the residual D_- b - M x with stop_rsd scaled by
+// |D_- b| / |D_- b - M x|.  out_e/out_o must not be W_IN_E, W_IN_O,
+// W_OUT_E or W_OUT_O (the correction solve's fields); in_e/in_o may be
+// W_IN_E/W_IN_O (they are read before those are reused).
+long invert_halves_with_guess(Inv& I, const double2* in_e, const double2* in_o, double2* out_e, double2* out_o)
+{
+  double2* dm_e = W(I, W_DM_E);
+  double2* dm_o = W(I, W_DM_O);
+  if (I.dminus) {
+    op_d_minus(I, dm_e, dm_o, in_e, in_o);
+  } else {
+    copy64(I, dm_e, in_e);
+    copy64(I, dm_o, in_o);
+  }
+  const double q = dev_norm2(I, dm_e) + dev_norm2(I, dm_o);
+  double2* me = W(I, W_TMP);
+  double2* mo = W(I, W_ITMP);
+  op_m_full(I, me, mo, out_e, out_o);
+  // the residual is the source of the correction solve
+  double2* re = W(I, W_OUT_E);
+  double2* ro = W(I, W_OUT_O);
+  QDWF_REQUIRE(re != out_e && ro != out_o && re != in_e && ro != in_o, "internal field aliasing");
+  copy64(I, re, dm_e);
+  copy64(I, ro, dm_o);
+  qdwf::launch_axpy<double2>(-1.0, me, re, n64(I), I.stream);
+  qdwf::launch_axpy<double2>(-1.0, mo, ro, n64(I), I.stream);
+  check_launch();
+  const double q_sub = dev_norm2(I, re) + dev_norm2(I, ro);
+  if (q_sub == 0.0) return 0;  // the guess solves the system exactly
+  const double ratio = std::sqrt(q / q_sub);
+  double2* ce = W(I, W_IN_E);
+  double2* co = W(I, W_IN_O);
+  QDWF_REQUIRE(ce != out_e && co != out_o, "internal field aliasing");
+  const long it = invert_halves(I, re, ro, ce, co, I.stop_rsd * ratio, true);
+  qdwf::launch_axpy<double2>(1.0, ce, out_e, n64(I), I.stream);
+  qdwf::launch_axpy<double2>(1.0, co, out_o, n64(I), I.stream);
+  check_launch();
+  return it;
 }
 
 void reset_stats(Inv& I) { std::memset(&I.stats, 0, sizeof(I.stats)); }
@@ -1887,6 +1959,11 @@ long solve_columns(Inv& I, int n, Inv* const* colinv, Load&& load, Store&& store
       load(c0 + j, J, ie, io);
       out_e_p_of(J, ie, io, oep);
       restore_column(J, xs[j], oep, oe, oo);
+      if (g_checking_invert) {
+        // out_e_p_of left (D_- b)_e in W_DM_E, and (D_- b)_o in W_DM_O when D_- is applied
+        const double r = residual_5d(J, oe, oo, W(J, W_DM_E), J.dminus ? W(J, W_DM_O) : io);
+        note_residual_5d(I, r);
+      }
       store(c0 + j, J, oe, oo);
     }
   }
@@ -1997,6 +2074,7 @@ long invert_5d_batch_dev(qlat_dwf_inverter* const* invs, int n, double2* const* 
       I.stats.total_iter += j ? it : 0;
       I.stats.hermop_fp64 += j ? J.stats.hermop_fp64 : 0;
       I.stats.true_residual = std::max(I.stats.true_residual, J.stats.true_residual);
+      if (j) note_residual_5d(I, J.stats.true_residual_5d);
     }
     return total;
   }
@@ -2186,7 +2264,16 @@ extern "C" {
 
 const char* qlat_dwf_last_error(void) { return g_last_error.c_str(); }
 
-int qlat_dwf_abi_version(void) { return 2; }
+int qlat_dwf_abi_version(void) { return 3; }
+
+int qlat_dwf_set_checking_invert(int on)
+{
+  g_checking_invert = on != 0;
+  g_last_error.clear();
+  return QLAT_DWF_OK;
+}
+
+int qlat_dwf_is_checking_invert(void) { return g_checking_invert ? 1 : 0; }
 
 int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], const double* gauge,
                              const qlat_dwf_fermion_action* fa)
@@ -2476,6 +2563,60 @@ int qlat_dwf_invert_5d(qlat_dwf_inverter* inv, double* out, const double* in, lo
     double2* d_out = stage(*inv, 1, bytes);
     *iter = invert_5d_dev(*inv, d_out, d_in);
     download(*inv, out, d_out, bytes);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+namespace {
+
+// invert_with_cg_with_guess on full 5D AoS device fields: out holds the guess
+long invert_5d_with_guess_dev(Inv& I, double2* out_aos, const double2* in_aos)
+{
+  double2* ie = W(I, W_IN_E);
+  double2* io = W(I, W_IN_O);
+  double2* ge = W(I, W_G_E);
+  double2* go = W(I, W_G_O);
+  qdwf::launch_pack5(in_aos, ie, I.geo, I.ls, 2, 1, I.stream);
+  qdwf::launch_pack5(in_aos, io, I.geo, I.ls, 1, 1, I.stream);
+  qdwf::launch_pack5(out_aos, ge, I.geo, I.ls, 2, 1, I.stream);
+  qdwf::launch_pack5(out_aos, go, I.geo, I.ls, 1, 1, I.stream);
+  check_launch();
+  const long it = invert_halves_with_guess(I, ie, io, ge, go);
+  qdwf::launch_unpack5(ge, out_aos, I.geo, I.ls, 2, 1, I.stream);
+  qdwf::launch_unpack5(go, out_aos, I.geo, I.ls, 1, 1, I.stream);
+  check_launch();
+  return it;
+}
+
+}  // namespace
+
+int qlat_dwf_invert_5d_with_guess(qlat_dwf_inverter* inv, double* out, const double* in, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(out && in && iter, "invalid argument");
+    QDWF_REQUIRE((const double*)out != in, "out and in must be different fields (dslash.h:545)");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = full_aos_bytes(*inv);
+    const double2* d_in = upload(*inv, 0, in, bytes);
+    double2* d_out = (double2*)upload(*inv, 1, out, bytes);
+    *iter = invert_5d_with_guess_dev(*inv, d_out, d_in);
+    download(*inv, out, d_out, bytes);
+    inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int qlat_dwf_invert_5d_with_guess_device(qlat_dwf_inverter* inv, double* out, const double* in, long* iter)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(out && in && iter, "invalid argument");
+    QDWF_REQUIRE((const double*)out != in, "out and in must be different fields (dslash.h:545)");
+    reset_stats(*inv);
+    const auto t0 = std::chrono::steady_clock::now();
+    *iter = invert_5d_with_guess_dev(*inv, (double2*)out, (const double2*)in);
+    sync(*inv);
     inv->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   });
 }
@@ -2953,6 +3094,33 @@ int qlat_dwf_deflate(qlat_dwf_inverter* inv, double* out, const double* in, int 
   });
 }
 
+namespace {
+
+// The power iteration of find_max_eigen_value_hermop_sym2 (dslash.cpp:1349-1369)
+// on the odd half field ff (device layout): normalise, then repeatedly apply
+// Mpc^+ Mpc and renormalise; returns the last norm ratio.
+double power_iteration(Inv& I, double2* ff, long max_iter)
+{
+  const int64_t n = n64(I);
+  const double q = dev_norm2(I, ff);
+  QDWF_REQUIRE(q > 0, "find_max_eigen_value_hermop_sym2: the start vector is zero");
+  double ratio = 1.0;
+  double2* tmp = W(I, W_TMP);  // zero vector: ff = 0 + scale * ff
+  zero64(I, tmp);
+  qdwf::launch_xpay<double2>(tmp, 1.0 / std::sqrt(q), ff, n, I.stream);
+  check_launch();
+  for (long i = 0; i < max_iter; ++i) {
+    op_hermop<double>(I, ff, ff, 1);
+    ratio = std::sqrt(dev_norm2(I, ff));
+    qdwf::launch_xpay<double2>(tmp, 1.0 / ratio, ff, n, I.stream);
+    check_launch();
+  }
+  sync(I);
+  return ratio;
+}
+
+}  // namespace
+
 int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const double* start, long max_iter,
                                               double* result)
 {
@@ -2964,22 +3132,39 @@ int qlat_dwf_find_max_eigen_value_hermop_sym2(qlat_dwf_inverter* inv, const doub
     double2* ff = W(*inv, W_IN_OP);
     qdwf::launch_pack5(d_in, ff, inv->geo, inv->ls, 1, 0, inv->stream);
     check_launch();
-    const int64_t n = n64(*inv);
-    // dslash.cpp:1349-1369: normalise, then repeatedly apply and renormalise
-    const double s = 1.0 / std::sqrt(dev_norm2(*inv, ff));
-    double ratio = 1.0;
-    double2* tmp = W(*inv, W_TMP);  // zero vector: ff = 0 + scale * ff
-    zero64(*inv, tmp);
-    qdwf::launch_xpay<double2>(tmp, s, ff, n, inv->stream);
+    *result = power_iteration(*inv, ff, max_iter);
+  });
+}
+
+int qlat_dwf_find_max_eigen_value_hermop_sym2_rng(qlat_dwf_inverter* inv, const uint32_t midstate[8],
+                                                  uint64_t num_bytes, uint64_t index0, long max_iter,
+                                                  double* result)
+{
+  return guarded([&] {
+    check_inv(inv);
+    QDWF_REQUIRE(midstate && result && max_iter >= 0, "invalid argument");
+    QDWF_REQUIRE(num_bytes % 64 == 0, "the RngState message is a whole number of 64-byte blocks");
+    Inv& I = *inv;
+    // set_u_rand(ff, rs) on the odd half field (dslash.cpp:1355-1357): site x
+    // draws Ls*24 reals in [-1, 1) from rs.newtype(global index of x); the
+    // draws of a full 5D field in reference layout are made and its odd
+    // half packed (each site's stream is independent of the others)
+    qdwf::RngFillArgs a{};
+    for (int i = 0; i < 8; ++i) a.mid[i] = midstate[i];
+    a.num_bytes = num_bytes;
+    a.index0 = index0;
+    a.n_sites = 2 * I.geo.vh;
+    a.n_per_site = I.ls * 24;
+    a.scale = 2.0;
+    a.lower = -1.0;
+    double2* full = stage(I, 0, full_aos_bytes(I));
+    a.out = (double*)full;
+    qdwf::launch_rng_fill_u(a, I.stream);
     check_launch();
-    for (long i = 0; i < max_iter; ++i) {
-      op_hermop<double>(*inv, ff, ff, 1);
-      ratio = std::sqrt(dev_norm2(*inv, ff));
-      qdwf::launch_xpay<double2>(tmp, 1.0 / ratio, ff, n, inv->stream);
-      check_launch();
-    }
-    sync(*inv);
-    *result = ratio;
+    double2* ff = W(I, W_IN_OP);
+    qdwf::launch_pack5(full, ff, I.geo, I.ls, 1, 1, I.stream);
+    check_launch();
+    *result = power_iteration(I, ff, max_iter);
   });
 }
 

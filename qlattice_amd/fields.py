@@ -127,10 +127,14 @@ class _Field:
 
         if not isinstance(rng, RngState):
             raise TypeError("set_rand: rng must be an RngState")
-        if getattr(self, "eo", 0) != 0:
-            raise NotImplementedError("set_rand: only full-lattice fields")
-        (set_u_rand_device if device else set_u_rand)(self.data, rng, upper, lower)
+        (set_u_rand_device if device else set_u_rand)(self.data, rng, upper, lower, self._rng_sites())
         return self
+
+    def _rng_sites(self):
+        """Global site index of each row: None for a full field, the sites of
+        the parity for a half field (eo geometry, core.h:248-262)."""
+        eo = getattr(self, "eo", 0)
+        return None if eo == 0 else self.geo.parity_sites(eo)
 
     def set_rand_g(self, rng, center=0.0, sigma=1.0, device=False):
         """Field.set_rand_g (field_types.pyx.in:198-199 -> set_g_rand,
@@ -139,9 +143,7 @@ class _Field:
 
         if not isinstance(rng, RngState):
             raise TypeError("set_rand_g: rng must be an RngState")
-        if getattr(self, "eo", 0) != 0:
-            raise NotImplementedError("set_rand_g: only full-lattice fields")
-        (set_g_rand_device if device else set_g_rand)(self.data, rng, center, sigma)
+        (set_g_rand_device if device else set_g_rand)(self.data, rng, center, sigma, self._rng_sites())
         return self
 
 

@@ -17,8 +17,6 @@ phi and Y = Mpc X are recomputed on every call, nothing is stored along a
 trajectory.  PyTorch only wraps it for autograd.
 """
 
-import ctypes
-
 import numpy as np
 
 from . import _lib
@@ -30,8 +28,7 @@ def set_gauge(inv, gf):
     dslash.h:154-162)."""
     g = np.ascontiguousarray(gf.data if isinstance(gf, GaugeField) else gf, dtype=np.complex128)
     assert g.shape == (inv.geo().total_volume(), 4, 3, 3)
-    _lib.check(_lib.load().qlat_dwf_inverter_set_gauge(inv.cdata, _lib.ptr(g), 0),
-               "qlat_dwf_inverter_set_gauge")
+    _lib.load().inverter_set_gauge(inv.cdata, _lib.ptr(g), False)
 
 
 def fermion_force(inv, phi, stop_rsd=1e-12, with_grad=False, action_pair=False):
@@ -50,14 +47,9 @@ def fermion_force(inv, phi, stop_rsd=1e-12, with_grad=False, action_pair=False):
     assert p.shape == (V // 2, inv.fa.ls(), 12)
     force = np.empty((V, 4, 3, 3), dtype=np.complex128)
     grad = np.empty_like(force) if with_grad else None
-    S = ctypes.c_double()
-    S_lo = ctypes.c_double()
-    it = ctypes.c_long()
-    _lib.check(_lib.load().qlat_dwf_fermion_force(
-        inv.cdata, _lib.ptr(p), float(stop_rsd), _lib.ptr(force),
-        _lib.ptr(grad) if with_grad else None, ctypes.byref(S), ctypes.byref(S_lo), ctypes.byref(it)),
-        "qlat_dwf_fermion_force")
-    return (S.value, S_lo.value) if action_pair else S.value, force, grad, it.value
+    S, S_lo, it = _lib.load().fermion_force(inv.cdata, _lib.ptr(p), float(stop_rsd), _lib.ptr(force),
+                                            _lib.ptr(grad) if with_grad else 0)
+    return (S, S_lo) if action_pair else S, force, grad, it
 
 
 def _torch_function():

@@ -9,10 +9,9 @@ module-level ``multiply_*``, ``cg_with_herm_sym_2``, ``invert`` and
 overloads in qlat/qlat/include/qlat/dslash.h:218-588.
 
 Everything is computed by the HIP library through the C ABI of
-include/qlat_dwf.h; this module only marshals numpy arrays.
+include/qlat_dwf.h, reached through the Cython binding (_cdwf.pyx); this
+module only marshals numpy arrays.
 """
-
-import ctypes
 
 import numpy as np
 
@@ -34,10 +33,9 @@ def free_invert(prop_src, mass, m5=1.0, momtwist=None):
     geo = prop_src.geo
     src = np.ascontiguousarray(prop_src.data, dtype=np.complex128)
     sol = Prop(geo)
-    ts = (ctypes.c_int * 4)(*geo.total_site())
-    tw = None if momtwist is None else (ctypes.c_double * 4)(*[float(v) for v in momtwist])
-    _lib.check(_lib.load().qlat_dwf_free_invert(ts, float(mass), float(m5), tw, _lib.ptr(sol.data), _lib.ptr(src)),
-               "qlat_dwf_free_invert")
+    _lib.load().free_invert(geo.total_site(), float(mass), float(m5),
+                            None if momtwist is None else [float(v) for v in momtwist],
+                            _lib.ptr(sol.data), _lib.ptr(src))
     return sol
 
 
@@ -83,41 +81,28 @@ class InverterDomainWall(Inverter):
         self.timer = qtimer
         self._bs = np.ascontiguousarray(fa.bs, dtype=np.complex128)
         self._cs = np.ascontiguousarray(fa.cs, dtype=np.complex128)
-        cfa = _lib.FermionActionC(
-            mass=fa.mass(), ls=fa.ls(), m5=fa.m5(), mobius_scale=fa.mobius_scale(),
-            is_multiplying_dminus=int(fa.is_multiplying_dminus),
-            is_using_zmobius=int(fa.is_using_zmobius), cg_diagonal_mee=2,
-            bs=_lib.ptr(self._bs), cs=_lib.ptr(self._cs),
-        )
         gauge = np.ascontiguousarray(gf.data, dtype=np.complex128)
-        site = (ctypes.c_int * 4)(*gf.geo.total_site())
-        h = ctypes.c_void_p()
-        _lib.check(L.qlat_dwf_inverter_create(ctypes.byref(h), site, _lib.ptr(gauge),
-                                              ctypes.byref(cfa)),
-                   "qlat_dwf_inverter_create")
-        self.cdata = h
+        # the handle is the C pointer as an integer (inverter.py:88-94)
+        self.cdata = L.inverter_create(gf.geo.total_site(), _lib.ptr(gauge), fa.mass(), fa.ls(), fa.m5(),
+                                       fa.mobius_scale(), int(fa.is_multiplying_dminus),
+                                       int(fa.is_using_zmobius), 2, _lib.ptr(self._bs), _lib.ptr(self._cs))
         self.set_solver(solver, reliable_delta)
 
     def __del__(self):
         h = getattr(self, "cdata", None)
         if h:
-            _lib.load().qlat_dwf_inverter_destroy(h)
             self.cdata = None
+            _lib.load().inverter_destroy(h)
 
     # ----------------------------------------------------------- params
     def geo(self):
         return self.geo_
 
     def _params(self):
-        s, n, c = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
-        _lib.check(_lib.load().qlat_dwf_inverter_get_params(self.cdata, ctypes.byref(s),
-                                                            ctypes.byref(n), ctypes.byref(c)),
-                   "qlat_dwf_inverter_get_params")
-        return s.value, n.value, c.value
+        return _lib.load().inverter_get_params(self.cdata)
 
     def _set_params(self, s, n, c):
-        _lib.check(_lib.load().qlat_dwf_inverter_set_params(self.cdata, s, n, c),
-                   "qlat_dwf_inverter_set_params")
+        _lib.load().inverter_set_params(self.cdata, s, n, c)
 
     def stop_rsd(self):
         return self._params()[0]
@@ -143,9 +128,7 @@ class InverterDomainWall(Inverter):
     def set_solver(self, solver, reliable_delta=0.01):
         if solver not in SOLVERS:
             raise ValueError(f"solver must be one of {sorted(SOLVERS)}")
-        _lib.check(_lib.load().qlat_dwf_inverter_set_solver(self.cdata, SOLVERS[solver],
-                                                            float(reliable_delta)),
-                   "qlat_dwf_inverter_set_solver")
+        _lib.load().inverter_set_solver(self.cdata, SOLVERS[solver], float(reliable_delta))
         self.solver = solver
 
     def set_low_modes(self, lm):
@@ -154,19 +137,15 @@ class InverterDomainWall(Inverter):
         restart cycle from deflate(residual) (dslash.h:496-500)."""
         L = _lib.load()
         if lm is None:
-            _lib.check(L.qlat_dwf_inverter_set_low_modes(self.cdata, None, 0, 0, None, None, None),
-                       "qlat_dwf_inverter_set_low_modes")
+            L.inverter_set_low_modes(self.cdata, (0, 0, 0, 0), 0, 0, 0, 0, 0)
             self.lm = None
             return
-        bs = (ctypes.c_int * 4)(*lm.block_site)
         ev = np.ascontiguousarray(lm.eigen_values, dtype=np.float64)
         bases = np.ascontiguousarray(lm.bases, dtype=np.complex64)
         coefs = np.ascontiguousarray(lm.coefs, dtype=np.complex64)
-        fp = ctypes.POINTER(ctypes.c_float)
-        _lib.check(L.qlat_dwf_inverter_set_low_modes(self.cdata, bs, int(lm.n_basis), int(lm.n_vec),
-                                                     ev.ctypes.data_as(_lib._dp), bases.ctypes.data_as(fp),
-                                                     coefs.ctypes.data_as(fp)),
-                   "qlat_dwf_inverter_set_low_modes")
+        L.inverter_set_low_modes(self.cdata, lm.block_site, int(lm.n_basis), int(lm.n_vec),
+                                 _lib.addr(ev, np.float64), _lib.addr(bases, np.complex64),
+                                 _lib.addr(coefs, np.complex64))
         self.lm = lm
 
     def set_gauge(self, gf):
@@ -176,10 +155,7 @@ class InverterDomainWall(Inverter):
         set_gauge(self, gf)
 
     def stats(self):
-        st = _lib.SolveStatsC()
-        _lib.check(_lib.load().qlat_dwf_inverter_get_stats(self.cdata, ctypes.byref(st)),
-                   "qlat_dwf_inverter_get_stats")
-        return {k: getattr(st, k) for k, _ in _lib.SolveStatsC._fields_}
+        return _lib.load().inverter_get_stats(self.cdata)
 
     # ------------------------------------------------------------ apply
     def __mul__(self, src):
@@ -228,7 +204,7 @@ def _apply_half(name, out, in_, inv, eo_arg, eo_out):
     args = (inv.cdata, _lib.ptr(res.data), _lib.ptr(src))
     if eo_arg is not None:
         args = args + (int(eo_arg),)
-    _lib.check(fn(*args), name)
+    fn(*args)
     if out is not None:
         out.geo, out.ls, out.eo, out.data = res.geo, res.ls, res.eo, res.data
         return out
@@ -237,57 +213,57 @@ def _apply_half(name, out, in_, inv, eo_arg, eo_out):
 
 def multiply_m_e_e(out, in_, inv):
     """dslash.cpp:591-639 (works for _o_o as well)."""
-    return _apply_half("qlat_dwf_multiply_m_e_e", out, in_, inv, None, in_.eo)
+    return _apply_half("multiply_m_e_e", out, in_, inv, None, in_.eo)
 
 
 def multiply_mdag_e_e(out, in_, inv):
     """dslash.cpp:641-692."""
-    return _apply_half("qlat_dwf_multiply_mdag_e_e", out, in_, inv, None, in_.eo)
+    return _apply_half("multiply_mdag_e_e", out, in_, inv, None, in_.eo)
 
 
 def multiply_m_e_e_inv(out, in_, inv):
     """dslash.cpp:694-769."""
-    return _apply_half("qlat_dwf_multiply_m_e_e_inv", out, in_, inv, None, in_.eo)
+    return _apply_half("multiply_m_e_e_inv", out, in_, inv, None, in_.eo)
 
 
 def multiply_mdag_e_e_inv(out, in_, inv):
     """dslash.cpp:771-857."""
-    return _apply_half("qlat_dwf_multiply_mdag_e_e_inv", out, in_, inv, None, in_.eo)
+    return _apply_half("multiply_mdag_e_e_inv", out, in_, inv, None, in_.eo)
 
 
 def multiply_wilson_d_e_o_no_comm(out, in_, inv):
     """dslash.cpp:859-908: output parity 3 - in.eo."""
-    return _apply_half("qlat_dwf_multiply_wilson_d_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+    return _apply_half("multiply_wilson_d_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
 
 
 def multiply_wilson_ddag_e_o_no_comm(out, in_, inv):
     """dslash.cpp:910-959."""
-    return _apply_half("qlat_dwf_multiply_wilson_ddag_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+    return _apply_half("multiply_wilson_ddag_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
 
 
 def multiply_m_e_o(out, in_, inv):
     """dslash.cpp:961-1003 (works for _o_e as well)."""
-    return _apply_half("qlat_dwf_multiply_m_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+    return _apply_half("multiply_m_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
 
 
 def multiply_mdag_e_o(out, in_, inv):
     """dslash.cpp:1005-1060."""
-    return _apply_half("qlat_dwf_multiply_mdag_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+    return _apply_half("multiply_mdag_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
 
 
 def multiply_mpc_sym2(out, in_, inv):
     """dslash.cpp:1102-1119 (odd <- odd, works for even <- even)."""
-    return _apply_half("qlat_dwf_multiply_mpc_sym2", out, in_, inv, in_.eo, in_.eo)
+    return _apply_half("multiply_mpc_sym2", out, in_, inv, in_.eo, in_.eo)
 
 
 def multiply_mpcdag_sym2(out, in_, inv):
     """dslash.cpp:1121-1138."""
-    return _apply_half("qlat_dwf_multiply_mpcdag_sym2", out, in_, inv, in_.eo, in_.eo)
+    return _apply_half("multiply_mpcdag_sym2", out, in_, inv, in_.eo, in_.eo)
 
 
 def multiply_hermop_sym2(out, in_, inv):
     """dslash.cpp:1140-1149."""
-    return _apply_half("qlat_dwf_multiply_hermop_sym2", out, in_, inv, in_.eo, in_.eo)
+    return _apply_half("multiply_hermop_sym2", out, in_, inv, in_.eo, in_.eo)
 
 
 class LowModes:
@@ -321,12 +297,12 @@ def deflate(out, in_, inv):
     (dslash.cpp:251-300) with the inverter's low modes: out = in + sum_i v_i
     <v_i, in> / lambda_i with the input rounded to FP32 (the reference's
     produce step accumulates into the blocked input), or 0 without low modes."""
-    return _apply_half("qlat_dwf_deflate", out, in_, inv, in_.eo, in_.eo)
+    return _apply_half("deflate", out, in_, inv, in_.eo, in_.eo)
 
 
 def multiply_hermop_sym2_fp32(out, in_, inv):
     """Mpc^+ Mpc evaluated in FP32 (the mixed solver's inner operator)."""
-    return _apply_half("qlat_dwf_multiply_hermop_sym2_fp32", out, in_, inv, in_.eo, in_.eo)
+    return _apply_half("multiply_hermop_sym2_fp32", out, in_, inv, in_.eo, in_.eo)
 
 
 def _apply_full(name, out, in_, inv):
@@ -335,7 +311,7 @@ def _apply_full(name, out, in_, inv):
     res = FermionField5d.__new__(FermionField5d)
     res.geo, res.ls, res.eo = in_.geo, in_.ls, 0
     res.data = np.empty_like(src)
-    _lib.check(getattr(_lib.load(), name)(inv.cdata, _lib.ptr(res.data), _lib.ptr(src)), name)
+    getattr(_lib.load(), name)(inv.cdata, _lib.ptr(res.data), _lib.ptr(src))
     if out is not None:
         out.geo, out.ls, out.eo, out.data = res.geo, res.ls, 0, res.data
         return out
@@ -344,17 +320,17 @@ def _apply_full(name, out, in_, inv):
 
 def multiply_m(out, in_, inv):
     """dslash.cpp:1062-1080."""
-    return _apply_full("qlat_dwf_multiply_m", out, in_, inv)
+    return _apply_full("multiply_m", out, in_, inv)
 
 
 def multiply_mdag(out, in_, inv):
     """dslash.cpp:1082-1100."""
-    return _apply_full("qlat_dwf_multiply_mdag", out, in_, inv)
+    return _apply_full("multiply_mdag", out, in_, inv)
 
 
 def multiply_d_minus(out, in_, inv):
     """dslash.cpp:448-488."""
-    return _apply_full("qlat_dwf_multiply_d_minus", out, in_, inv)
+    return _apply_full("multiply_d_minus", out, in_, inv)
 
 
 def dot_product(ff1, ff2, inv):
@@ -362,20 +338,13 @@ def dot_product(ff1, ff2, inv):
     a = np.ascontiguousarray(ff1.data, dtype=np.complex128)
     b = np.ascontiguousarray(ff2.data, dtype=np.complex128)
     assert a.shape == b.shape
-    re, im = ctypes.c_double(), ctypes.c_double()
-    _lib.check(_lib.load().qlat_dwf_dot_product(inv.cdata, _lib.ptr(a), _lib.ptr(b), a.size // 12,
-                                                ctypes.byref(re), ctypes.byref(im)),
-               "qlat_dwf_dot_product")
-    return complex(re.value, im.value)
+    return _lib.load().dot_product(inv.cdata, _lib.ptr(a), _lib.ptr(b), a.size // 12)
 
 
 def qnorm(ff, inv):
     """field.h:170-195, reduced on the GPU of `inv`."""
     a = np.ascontiguousarray(ff.data, dtype=np.complex128)
-    r = ctypes.c_double()
-    _lib.check(_lib.load().qlat_dwf_qnorm(inv.cdata, _lib.ptr(a), a.size // 12, ctypes.byref(r)),
-               "qlat_dwf_qnorm")
-    return r.value
+    return _lib.load().qnorm(inv.cdata, _lib.ptr(a), a.size // 12)
 
 
 def cg_with_herm_sym_2(sol, src, inv, stop_rsd=1e-8, max_num_iter=50000):
@@ -388,13 +357,9 @@ def cg_with_herm_sym_2(sol, src, inv, stop_rsd=1e-8, max_num_iter=50000):
         x = np.zeros_like(b)
     else:
         x = np.ascontiguousarray(sol.data, dtype=np.complex128).copy()
-    it = ctypes.c_long()
-    _lib.check(_lib.load().qlat_dwf_cg_with_herm_sym_2(inv.cdata, _lib.ptr(x), _lib.ptr(b),
-                                                       float(stop_rsd), int(max_num_iter),
-                                                       ctypes.byref(it)),
-               "qlat_dwf_cg_with_herm_sym_2")
+    it = _lib.load().cg_with_herm_sym_2(inv.cdata, _lib.ptr(x), _lib.ptr(b), float(stop_rsd), int(max_num_iter))
     sol.geo, sol.ls, sol.eo, sol.data = src.geo, src.ls, src.eo, x
-    return it.value
+    return it
 
 
 def invert(out, in_, inv):
@@ -402,25 +367,37 @@ def invert(out, in_, inv):
     or invert(Prop) (qcd-prop.h:169-186), by the type of `in_`.  Returns the
     total number of CG iterations."""
     L = _lib.load()
-    it = ctypes.c_long()
     src = np.ascontiguousarray(in_.data, dtype=np.complex128)
     res = np.empty_like(src)
     if isinstance(in_, Prop):
-        _lib.check(L.qlat_dwf_invert_prop(inv.cdata, _lib.ptr(res), _lib.ptr(src), ctypes.byref(it)),
-                   "qlat_dwf_invert_prop")
+        it = L.invert_prop(inv.cdata, _lib.ptr(res), _lib.ptr(src))
     elif isinstance(in_, FermionField4d):
-        _lib.check(L.qlat_dwf_invert_4d(inv.cdata, _lib.ptr(res), _lib.ptr(src), ctypes.byref(it)),
-                   "qlat_dwf_invert_4d")
+        it = L.invert_4d(inv.cdata, _lib.ptr(res), _lib.ptr(src))
     elif isinstance(in_, FermionField5d):
         assert in_.eo == 0
-        _lib.check(L.qlat_dwf_invert_5d(inv.cdata, _lib.ptr(res), _lib.ptr(src), ctypes.byref(it)),
-                   "qlat_dwf_invert_5d")
+        it = L.invert_5d(inv.cdata, _lib.ptr(res), _lib.ptr(src))
         out.ls, out.eo = in_.ls, 0
     else:
         raise TypeError("invert: unsupported field type")
     out.geo = in_.geo
     out.data = res
-    return it.value
+    return it
+
+
+def invert_with_cg_with_guess(out, in_, inv):
+    """invert_with_cg_with_guess (dslash.h:532-573) with the inverter's
+    solver on full FermionField5d: `out` holds the initial guess x0 and
+    receives x0 plus the correction solved from D_- in - M x0, with stop_rsd
+    scaled by |D_- in| / |D_- in - M x0|.  Returns the correction's CG
+    iterations."""
+    assert isinstance(in_, FermionField5d) and in_.eo == 0
+    assert isinstance(out, FermionField5d) and out.eo == 0 and out.data is not None
+    src = np.ascontiguousarray(in_.data, dtype=np.complex128)
+    res = np.array(out.data, dtype=np.complex128, order="C", copy=True)
+    assert res.shape == src.shape
+    it = _lib.load().invert_5d_with_guess(inv.cdata, _lib.ptr(res), _lib.ptr(src))
+    out.data = res
+    return it
 
 
 def invert_batch(invs, srcs):
@@ -436,12 +413,8 @@ def invert_batch(invs, srcs):
     for f in srcs:
         assert isinstance(f, FermionField5d) and f.eo == 0
     outs = [np.empty_like(d) for d in data]
-    hv = (ctypes.c_void_p * n)(*[inv.cdata for inv in invs])
-    sp = (_lib._dp * n)(*[_lib.ptr(d) for d in data])
-    op = (_lib._dp * n)(*[_lib.ptr(o) for o in outs])
-    it = np.zeros(n, dtype=np.int64)
-    _lib.check(_lib.load().qlat_dwf_invert_5d_batch(hv, n, op, sp, it.ctypes.data_as(_lib._lp)),
-               "qlat_dwf_invert_5d_batch")
+    it = np.array(_lib.load().invert_5d_batch([inv.cdata for inv in invs], [_lib.ptr(o) for o in outs],
+                                              [_lib.ptr(d) for d in data]), dtype=np.int64)
     sols = []
     for f, o in zip(srcs, outs):
         r = FermionField5d(f.geo, f.ls, 0)
@@ -450,16 +423,21 @@ def invert_batch(invs, srcs):
     return sols, it
 
 
-def find_max_eigen_value_hermop_sym2(inv, start, max_iter=100):
-    """Power iteration for the largest eigenvalue of Mpc^+ Mpc
-    (dslash.cpp:1349-1369) from the odd half-field `start`."""
-    _check_half(start, inv)
-    a = np.ascontiguousarray(start.data, dtype=np.complex128)
-    r = ctypes.c_double()
-    _lib.check(_lib.load().qlat_dwf_find_max_eigen_value_hermop_sym2(inv.cdata, _lib.ptr(a),
-                                                                     int(max_iter), ctypes.byref(r)),
-               "qlat_dwf_find_max_eigen_value_hermop_sym2")
-    return r.value
+def find_max_eigen_value_hermop_sym2(inv, rs, max_iter=100):
+    """find_max_eigen_value_hermop_sym2(inv, rs, max_iter) (dslash.h:586-588,
+    dslash.cpp:1349-1369): power iteration for the largest eigenvalue of
+    Mpc^+ Mpc.  With an RngState `rs` the start vector is the reference's
+    set_u_rand(ff, rs) on the odd half field, drawn on the GPU; an odd
+    FermionField5d `rs` is used as the start vector itself."""
+    from .rng import RngState, midstate
+
+    L = _lib.load()
+    if isinstance(rs, RngState):
+        return L.find_max_eigen_value_hermop_sym2_rng(inv.cdata, midstate(rs), len(rs._msg), rs.index,
+                                                      int(max_iter))
+    _check_half(rs, inv)
+    a = np.ascontiguousarray(rs.data, dtype=np.complex128)
+    return L.find_max_eigen_value_hermop_sym2(inv.cdata, _lib.ptr(a), int(max_iter))
 
 
 def bench_kernel(inv, which, precision, iters):
@@ -467,8 +445,4 @@ def bench_kernel(inv, which, precision, iters):
     stream.  which: 0 hop, 1 hop^+, 2 s-matrix, 3 Mpc^+ Mpc, 4 copy
     (precision 64/32); 5 hop + s-matrix, 6 CG step, 7 Mpc^+ Mpc with dot on
     the three-phase solver's tile (precision 32/16)."""
-    ms = ctypes.c_double()
-    _lib.check(_lib.load().qlat_dwf_bench_kernel(inv.cdata, int(which), int(precision), int(iters),
-                                                 ctypes.byref(ms)),
-               "qlat_dwf_bench_kernel")
-    return ms.value
+    return _lib.load().bench_kernel(inv.cdata, int(which), int(precision), int(iters))

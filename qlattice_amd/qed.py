@@ -10,8 +10,6 @@ All arithmetic runs in the HIP library (``qlat_dwf_multiply_m_dwf_qed``,
 ``qlat_dwf_cg_with_m_dwf_qed``, ``qlat_dwf_invert_dwf_qed``).
 """
 
-import ctypes
-
 import numpy as np
 
 from . import _lib
@@ -33,8 +31,7 @@ def _common(total_site, ls, gf1, wick):
 
 
 def _args(ts, ls, g, mass, m5, w, is_dagger):
-    return ((ctypes.c_int * 4)(*ts), int(ls), _lib.ptr(g), float(mass), float(m5),
-            None if w is None else _lib.ptr(w), 1 if is_dagger else 0)
+    return (ts, int(ls), _lib.ptr(g), float(mass), float(m5), 0 if w is None else _lib.ptr(w), bool(is_dagger))
 
 
 def multiply_m_dwf_qed(f_in5d, gf1, mass, m5, ls, total_site, wick=None, is_dagger=False):
@@ -42,8 +39,7 @@ def multiply_m_dwf_qed(f_in5d, gf1, mass, m5, ls, total_site, wick=None, is_dagg
     ts, V, g, w = _common(total_site, ls, gf1, wick)
     f = _c(f_in5d, (V, ls, 4))
     out = np.empty_like(f)
-    _lib.check(_lib.load().qlat_dwf_multiply_m_dwf_qed(*_args(ts, ls, g, mass, m5, w, is_dagger),
-                                                       _lib.ptr(out), _lib.ptr(f)), "qlat_dwf_multiply_m_dwf_qed")
+    _lib.load().multiply_m_dwf_qed(*_args(ts, ls, g, mass, m5, w, is_dagger), _lib.ptr(out), _lib.ptr(f))
     return out
 
 
@@ -54,11 +50,9 @@ def cg_with_m_dwf_qed(f_in5d, gf1, mass, m5, ls, total_site, wick=None, is_dagge
     ts, V, g, w = _common(total_site, ls, gf1, wick)
     f = _c(f_in5d, (V, ls, 4))
     x = np.zeros_like(f) if guess is None else _c(guess, (V, ls, 4)).copy()
-    it = ctypes.c_long(0)
-    _lib.check(_lib.load().qlat_dwf_cg_with_m_dwf_qed(*_args(ts, ls, g, mass, m5, w, is_dagger),
-                                                      float(stop_rsd), int(max_num_iter), _lib.ptr(x),
-                                                      _lib.ptr(f), ctypes.byref(it)), "qlat_dwf_cg_with_m_dwf_qed")
-    return x, it.value
+    it = _lib.load().cg_with_m_dwf_qed(*_args(ts, ls, g, mass, m5, w, is_dagger), float(stop_rsd),
+                                       int(max_num_iter), _lib.ptr(x), _lib.ptr(f))
+    return x, it
 
 
 def invert_dwf_qed(f_in4d, gf1, mass, m5, ls, total_site, wick=None, is_dagger=False, stop_rsd=1e-8,
@@ -68,11 +62,9 @@ def invert_dwf_qed(f_in4d, gf1, mass, m5, ls, total_site, wick=None, is_dagger=F
     ts, V, g, w = _common(total_site, ls, gf1, wick)
     f = _c(f_in4d, (V, 4))
     out = np.empty_like(f)
-    it = ctypes.c_long(0)
-    _lib.check(_lib.load().qlat_dwf_invert_dwf_qed(*_args(ts, ls, g, mass, m5, w, is_dagger),
-                                                   float(stop_rsd), int(max_num_iter), _lib.ptr(out),
-                                                   _lib.ptr(f), ctypes.byref(it)), "qlat_dwf_invert_dwf_qed")
-    return out, it.value
+    it = _lib.load().invert_dwf_qed(*_args(ts, ls, g, mass, m5, w, is_dagger), float(stop_rsd),
+                                    int(max_num_iter), _lib.ptr(out), _lib.ptr(f))
+    return out, it
 
 
 def invert_qed(sp_src, gf1, mass, m5, ls, total_site, wick=None, is_dagger=False, stop_rsd=1e-8,

@@ -201,62 +201,71 @@ def midstate(rs):
     return h
 
 
-def _fill_device(name, data, rs, a, b):
-    import ctypes
-
+def _fill_device(gaussian, data, rs, a, b, sites=None):
     from . import _lib
 
     if rs.type != _ULONG_MAX:
         raise ValueError("newtype needs a state without a type")
     reals = _site_reals(data)
     V, n = reals.shape
-    out = np.empty((V, n), np.float64)
-    mid = (ctypes.c_uint32 * 8)(*midstate(rs))
-    _lib.check(getattr(_lib.load(), name)(mid, len(rs._msg), rs.index, V, n, float(a), float(b),
-                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), name)
-    reals[...] = out
+    # the device fill draws for sites 0 .. nfill-1; rows of a half field take
+    # their sites' draws from it
+    nfill = V if sites is None else int(np.max(sites)) + 1
+    out = np.empty((nfill, n), np.float64)
+    _lib.load().rng_fill(gaussian, midstate(rs), len(rs._msg), rs.index, nfill, n, float(a), float(b),
+                         _lib.addr(out, np.float64))
+    reals[...] = out if sites is None else out[np.asarray(sites)]
     return data
 
 
-def set_u_rand_device(data, rs, upper=1.0, lower=-1.0):
+def set_u_rand_device(data, rs, upper=1.0, lower=-1.0, sites=None):
     """set_u_rand on the GPU (qlat_dwf_rng_fill_uniform): the same numbers
     as set_u_rand, one SHA-256 block per four draws on the device."""
-    return _fill_device("qlat_dwf_rng_fill_uniform", data, rs, upper, lower)
+    return _fill_device(False, data, rs, upper, lower, sites)
 
 
-def set_g_rand_device(data, rs, center=0.0, sigma=1.0):
+def set_g_rand_device(data, rs, center=0.0, sigma=1.0, sites=None):
     """set_g_rand on the GPU (qlat_dwf_rng_fill_gaussian), one thread per
     site; equal to set_g_rand up to the last bit of the device logarithm."""
-    return _fill_device("qlat_dwf_rng_fill_gaussian", data, rs, center, sigma)
+    return _fill_device(True, data, rs, center, sigma, sites)
 
 
 def _site_reals(data):
-    a = np.asarray(data)
+    """The site-major real view of `data` that the fills write through; a
+    view, never a copy, so `data` must be a C-contiguous array."""
+    a = data
+    if not isinstance(a, np.ndarray) or not a.flags.c_contiguous:
+        raise ValueError("the RNG fills write in place: pass a C-contiguous numpy array")
     V = a.shape[0]
     if np.iscomplexobj(a):
         return a.view(np.float64).reshape(V, -1)
     return a.reshape(V, -1)
 
 
-def set_u_rand(data, rs, upper=1.0, lower=-1.0):
+def set_u_rand(data, rs, upper=1.0, lower=-1.0, sites=None):
     """set_u_rand (core.h:1584-1603) on a (V, ...) float64 / complex128 array
-    in reference site order: site g draws its reals in memory order from
+    in reference site order: row i is the site with global index
+    g = sites[i] (g = i when sites is None; a half field passes the global
+    indices of its parity, as geo.coordinate_from_index of an eo geometry
+    gives, core.h:248-262), whose reals are drawn in memory order from
     rs.newtype(g); returns data."""
     reals = _site_reals(data)
     n = reals.shape[1]
     scale = upper - lower
-    for g in range(reals.shape[0]):
+    for i in range(reals.shape[0]):
+        g = i if sites is None else int(sites[i])
         u = rs.newtype(g).u_rand_u64(n).astype(np.float64)
-        reals[g] = u * _FAC * scale + lower
+        reals[i] = u * _FAC * scale + lower
     return data
 
 
-def set_g_rand(data, rs, center=0.0, sigma=1.0):
-    """set_g_rand (core.h:1619-1640): Gaussian reals, site g from rs.newtype(g)."""
+def set_g_rand(data, rs, center=0.0, sigma=1.0, sites=None):
+    """set_g_rand (core.h:1619-1640): Gaussian reals, row i from
+    rs.newtype(sites[i]) (rs.newtype(i) when sites is None)."""
     reals = _site_reals(data)
-    for g in range(reals.shape[0]):
-        rsi = rs.newtype(g)
-        reals[g] = [rsi.g_rand_gen(center, sigma) for _ in range(reals.shape[1])]
+    for i in range(reals.shape[0]):
+        rsi = rs.newtype(i if sites is None else int(sites[i]))
+        reals[i] = [rsi.g_rand_gen(center, sigma) for _ in range(reals.shape[1])]
     return data
 
 

@@ -14,8 +14,6 @@ with mom = lmom * 2 pi / total_site (lattice_mom_mult, field.h:14-22).  The
 columns as one multi-RHS problem, so the source never crosses PCIe.
 """
 
-import ctypes
-
 import numpy as np
 
 from . import _lib
@@ -23,18 +21,12 @@ from .fields import Coordinate, CoordinateD, FieldComplexD, Prop
 
 
 def _source(kind, xg=(0, 0, 0, 0), tslice=0, lmom=None, value=1.0, fu1=None):
-    s = _lib.SourceC()
-    s.kind = kind
-    for mu in range(4):
-        s.xg[mu] = int(xg[mu])
-        s.lmom[mu] = float(0.0 if lmom is None else lmom[mu])
-    s.tslice = int(tslice)
-    v = complex(value)
-    s.value[0], s.value[1] = v.real, v.imag
-    keep = None
-    if fu1 is not None:
-        keep = np.ascontiguousarray(fu1, dtype=np.complex128)
-        s.fu1 = keep.ctypes.data
+    """The fields of a qlat_dwf_source as keyword arguments of the binding's
+    make_source / source_propagator; keep is the host array fu1 points to."""
+    keep = None if fu1 is None else np.ascontiguousarray(fu1, dtype=np.complex128)
+    s = dict(kind=int(kind), xg=[int(v) for v in xg], tslice=int(tslice),
+             lmom=[float(0.0 if lmom is None else lmom[mu]) for mu in range(4)], value=complex(value),
+             fu1=0 if keep is None else keep.ctypes.data)
     return s, keep
 
 
@@ -42,9 +34,8 @@ def _fill(prop, geo, s):
     assert prop.data.shape == (geo.total_volume(), 12, 12)
     if prop.data.dtype != np.complex128 or not prop.data.flags.c_contiguous:
         prop.data = np.zeros((geo.total_volume(), 12, 12), dtype=np.complex128)
-    ts = (ctypes.c_int * 4)(*geo.total_site())
-    _lib.check(_lib.load().qlat_dwf_make_source(ts, ctypes.byref(s), _lib.ptr(prop.data)),
-               "qlat_dwf_make_source")
+    _lib.load().make_source(geo.total_site(), s["kind"], s["xg"], s["tslice"], s["lmom"], s["value"], s["fu1"],
+                            _lib.ptr(prop.data))
     return prop
 
 
@@ -145,9 +136,8 @@ def _solve(prop, inv, s):
     _prop_for(prop, geo)
     if prop.data.dtype != np.complex128 or not prop.data.flags.c_contiguous:
         prop.data = np.zeros((geo.total_volume(), 12, 12), dtype=np.complex128)
-    it = ctypes.c_long(0)
-    _lib.check(_lib.load().qlat_dwf_source_propagator(inv.cdata, ctypes.byref(s), _lib.ptr(prop.data),
-                                                      ctypes.byref(it)), "qlat_dwf_source_propagator")
+    _lib.load().source_propagator(inv.cdata, s["kind"], s["xg"], s["tslice"], s["lmom"], s["value"], s["fu1"],
+                                  _lib.ptr(prop.data))
     return prop
 
 

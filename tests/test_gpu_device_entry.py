@@ -2,8 +2,6 @@
 against their host-pointer counterparts on the same inputs: the same
 computation, so the results are identical."""
 
-import ctypes
-
 import numpy as np
 import pytest
 import torch
@@ -32,36 +30,28 @@ def test_hermop_and_invert_prop_device(inv):
     L = _lib.load()
     x = rand_field((geo.total_volume() // 2, 8, 12), 92)
     host = np.empty_like(x)
-    _lib.check(L.qlat_dwf_multiply_hermop_sym2(inv.cdata, _lib.ptr(host), _lib.ptr(x), 1), "hermop")
+    L.multiply_hermop_sym2(inv.cdata, _lib.ptr(host), _lib.ptr(x), 1)
     xd = torch.from_numpy(x).cuda()
     od = torch.empty_like(xd)
-    _lib.check(L.qlat_dwf_multiply_hermop_sym2_device(inv.cdata, _lib.dev_ptr(od), _lib.dev_ptr(xd), 1), "hermop_d")
+    L.multiply_hermop_sym2_device(inv.cdata, _lib.dev_ptr(od), _lib.dev_ptr(xd), 1)
     assert np.array_equal(od.cpu().numpy(), host)
     src = q.mk_point_src(geo, q.Coordinate([1, 0, 2, 3]))
     sol_h = inv * src
     sd = torch.from_numpy(src.data).cuda()
     so = torch.zeros_like(sd)
-    it = ctypes.c_long()
-    _lib.check(L.qlat_dwf_invert_prop_device(inv.cdata, _lib.dev_ptr(so), _lib.dev_ptr(sd), ctypes.byref(it)),
-               "invert_prop_device")
+    L.invert_prop(inv.cdata, _lib.dev_ptr(so), _lib.dev_ptr(sd), device=True)
     assert np.array_equal(so.cpu().numpy(), sol_h.data)
 
 
 def test_source_device_entry_points(inv):
     geo, inv = inv
     L = _lib.load()
-    s = _lib.SourceC()
-    s.kind = _lib.SRC_WALL
-    s.tslice = 3
-    s.lmom[0], s.lmom[1] = 1.0, 2.0
-    ts = (ctypes.c_int * 4)(*SITE)
+    src = dict(kind=_lib.SRC_WALL, xg=(0, 0, 0, 0), tslice=3, lmom=(1.0, 2.0, 0.0, 0.0), value=1.0, fu1=0)
     host = q.mk_wall_src(geo, 3, q.CoordinateD((1, 2, 0, 0)))
     dev = torch.zeros((geo.total_volume(), 12, 12), dtype=torch.complex128, device="cuda")
-    _lib.check(L.qlat_dwf_make_source_device(ts, ctypes.byref(s), _lib.dev_ptr(dev)), "make_source_device")
+    L.make_source(SITE, **src, prop=_lib.dev_ptr(dev), device=True)
     assert np.array_equal(dev.cpu().numpy(), host.data)
     sol_h = q.set_wall_src_propagator(q.Prop(), inv, 3, q.CoordinateD((1, 2, 0, 0)))
     so = torch.zeros_like(dev)
-    it = ctypes.c_long()
-    _lib.check(L.qlat_dwf_source_propagator_device(inv.cdata, ctypes.byref(s), _lib.dev_ptr(so), ctypes.byref(it)),
-               "source_propagator_device")
+    L.source_propagator(inv.cdata, **src, sol=_lib.dev_ptr(so), device=True)
     assert np.array_equal(so.cpu().numpy(), sol_h.data)

@@ -1,0 +1,131 @@
+"""invert_with_cg_with_guess, find_max_eigen_value_hermop_sym2 from an
+RngState, and the is_checking_invert residual of the solve statistics,
+against the oracle (4^3x8, Moebius Ls 8)."""
+
+import numpy as np
+import pytest
+
+import qlattice_amd as q
+from oracle import oracle as O
+from qlattice_amd.rng import RngState
+from tests.helpers import rand_field, rand_gauge, rel
+
+pytestmark = pytest.mark.gpu
+
+SITE = (4, 4, 4, 8)
+LS = 8
+
+
+@pytest.fixture(scope="module")
+def pair():
+    print("\n[gpu] entry points 4^3x8 Ls8: with_guess, RngState power iteration, 5D residual", flush=True)
+    U = rand_gauge(SITE, 0.4, 801)
+    kw = dict(mass=0.05, ls=LS, m5=1.8, mobius_scale=2.0)
+    geo = q.Geometry(SITE)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    inv = q.InverterDomainWall(gf=gf, fa=q.FermionAction(**kw))
+    orc = O.OracleInverter(SITE, U, **kw)
+    return geo, inv, orc
+
+
+def full(geo, data):
+    f = q.FermionField5d(geo, LS, 0)
+    f.data = np.ascontiguousarray(data)
+    return f
+
+
+def test_with_guess_zero_guess_is_invert(pair):
+    geo, inv, orc = pair
+    b = rand_field((geo.total_volume(), LS, 12), 802)
+    x = q.FermionField5d(geo, LS, 0)
+    it = q.invert(x, full(geo, b), inv)
+    g = full(geo, np.zeros_like(b))
+    itg = q.invert_with_cg_with_guess(g, full(geo, b), inv)
+    assert itg == it and np.array_equal(g.data, x.data)
+
+
+def test_with_guess_matches_oracle(pair):
+    geo, inv, orc = pair
+    inv.set_stop_rsd(1e-10)
+    orc.set_params(1e-10, 200, 300)
+    b = rand_field((geo.total_volume(), LS, 12), 803)
+    x0, _ = orc.invert_5d(b)
+    guess = x0 + 1e-3 * rand_field(b.shape, 804)
+    ref, it_ref = orc.invert_5d_with_guess(b, guess)
+    g = full(geo, guess)
+    it = q.invert_with_cg_with_guess(g, full(geo, b), inv)
+    inv.set_stop_rsd(1e-8)
+    orc.set_params(1e-8, 200, 300)
+    assert it == it_ref
+    assert rel(g.data, ref) < 1e-10
+
+
+@pytest.mark.parametrize("solver", ["mixed", "mixed16"])
+def test_with_guess_mixed_solvers(pair, solver):
+    geo, inv, orc = pair
+    b = rand_field((geo.total_volume(), LS, 12), 805)
+    orc.set_params(1e-12, 200, 300)
+    exact, _ = orc.invert_5d(b)
+    guess = exact + 1e-4 * rand_field(b.shape, 806)
+    inv.set_solver(solver)
+    inv.set_stop_rsd(1e-9)
+    g = full(geo, guess)
+    it = q.invert_with_cg_with_guess(g, full(geo, b), inv)
+    inv.set_solver("fp64")
+    inv.set_stop_rsd(1e-8)
+    assert it > 0
+    assert rel(g.data, exact) < 1e-6
+    assert rel(g.data, exact) < 0.1 * rel(guess, exact)
+
+
+def test_power_iteration_from_rngstate_matches_oracle(pair):
+    """find_max_eigen_value_hermop_sym2(inv, rs, max_iter) (dslash.cpp:1349-1369):
+    the start vector is set_u_rand on the odd half field (drawn on the GPU);
+    the oracle iterates from the host restatement's draw of the same field."""
+    geo, inv, orc = pair
+    rs = RngState("seed").split("find_max_eigen_value_hermop_sym2")
+    lam = q.find_max_eigen_value_hermop_sym2(inv, rs, 20)
+    start = q.FermionField5d(geo, LS, 1)
+    start.set_rand(rs, 1.0, -1.0)  # host restatement of set_u_rand on the odd sites
+    lam_ref = orc.find_max_eigen_value_hermop_sym2(start.data, 20)
+    assert abs(lam - lam_ref) < 1e-10 * lam_ref
+    # and the device draw is that start vector: the same value from it
+    assert abs(q.find_max_eigen_value_hermop_sym2(inv, start, 20) - lam) < 1e-12 * lam
+
+
+@pytest.mark.parametrize("solver", ["fp64", "mixed16"])
+def test_five_d_residual_in_stats_matches_oracle(pair, solver):
+    """stats.true_residual_5d is |M x - D_- b| / |D_- b| (dslash.h:522-528):
+    the oracle's operators give the same number for the returned solution."""
+    geo, inv, orc = pair
+    b = rand_field((geo.total_volume(), LS, 12), 807)
+    inv.set_solver(solver)
+    x = q.FermionField5d(geo, LS, 0)
+    q.invert(x, full(geo, b), inv)
+    st = inv.stats()
+    inv.set_solver("fp64")
+    r = rel(orc.apply_full("m", x.data), orc.apply_full("d_minus", b))
+    assert st["true_residual_5d"] > 0
+    assert abs(st["true_residual_5d"] - r) < 1e-3 * r, (st["true_residual_5d"], r)
+    q._lib.load().set_checking_invert(False)
+    try:
+        q.invert(x, full(geo, b), inv)
+        assert inv.stats()["true_residual_5d"] == 0.0
+    finally:
+        q._lib.load().set_checking_invert(True)
+
+
+def test_five_d_residual_of_multi_rhs_columns(pair):
+    """The multi-RHS path (three systems in one tile) reports the largest
+    column residual; the oracle's operators give it for the returned
+    solutions."""
+    geo, inv, orc = pair
+    inv.set_solver("mixed16")
+    bs = [rand_field((geo.total_volume(), LS, 12), 810 + k) for k in range(3)]
+    sols, its = q.invert_batch([inv] * 3, [full(geo, b) for b in bs])
+    st = inv.stats()
+    inv.set_solver("fp64")
+    r = [rel(orc.apply_full("m", x.data), orc.apply_full("d_minus", b)) for x, b in zip(sols, bs)]
+    assert st["true_residual"] <= 1e-8
+    assert abs(st["true_residual_5d"] - max(r)) < 1e-3 * max(r), (st["true_residual_5d"], r)

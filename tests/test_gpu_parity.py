@@ -172,8 +172,8 @@ def test_power_iteration_matches_oracle(mobius):
 def test_invalid_arguments_raise(mobius):
     geo, inv, _ = mobius
     x = rand_field((geo.total_volume() // 2, LS, 12), 308)
-    rc = q._lib.load().qlat_dwf_multiply_hermop_sym2(inv.cdata, q._lib.ptr(x), q._lib.ptr(x), 3)
-    assert rc == -1 and b"parity" in q._lib.load().qlat_dwf_last_error()
+    with pytest.raises(q.QlatDwfError, match="parity"):
+        q._lib.load().multiply_hermop_sym2(inv.cdata, q._lib.ptr(x), q._lib.ptr(x), 3)
     with pytest.raises(q.QlatDwfError):
         geo_bad = q.Geometry((4, 4, 4, 4))
         gf = q.GaugeField(geo_bad).set_unit()

@@ -20,20 +20,43 @@ def header_symbols():
 
 
 def test_library_exports_every_header_symbol():
-    L = _lib.load()
+    # the shared object itself, opened only to list its exported symbols
+    so = ctypes.CDLL(_lib.LIB_PATH)
     syms = header_symbols()
     assert len(syms) >= 30
     for s in syms:
-        assert hasattr(L, s), s
+        assert hasattr(so, s), s
     assert sorted(_lib.EXPORTED) == syms
-    assert L.qlat_dwf_abi_version() == 2
+    assert _lib.load().abi_version() == 3
+
+
+def test_binding_declares_every_header_symbol():
+    """The Cython module declares (and so is compiled against) every entry
+    point of the header."""
+    text = open(os.path.join(ROOT, "qlattice_amd", "_cdwf.pyx")).read()
+    declared = set(re.findall(r"\b(qlat_dwf_\w+)\s*\(", text))
+    assert set(header_symbols()) <= declared
 
 
 def test_null_handle_is_an_error_not_a_crash():
     L = _lib.load()
-    rc = L.qlat_dwf_inverter_set_params(None, 1e-8, 10, 10)
-    assert rc != 0
-    assert b"null" in L.qlat_dwf_last_error()
+    with pytest.raises(q.QlatDwfError, match="null"):
+        L.inverter_set_params(0, 1e-8, 10, 10)
+    # the C entry point itself returns a status and sets the message
+    so = ctypes.CDLL(_lib.LIB_PATH)
+    so.qlat_dwf_inverter_set_params.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_long, ctypes.c_long]
+    so.qlat_dwf_last_error.restype = ctypes.c_char_p
+    assert so.qlat_dwf_inverter_set_params(None, 1e-8, 10, 10) == -1
+    assert b"null" in so.qlat_dwf_last_error()
+
+
+def test_checking_invert_flag_round_trip():
+    L = _lib.load()
+    assert L.is_checking_invert()  # on by default, like is_checking_invert() (dslash.h:11-16)
+    L.set_checking_invert(False)
+    assert not L.is_checking_invert()
+    L.set_checking_invert(True)
+    assert L.is_checking_invert()
 
 
 def test_fermion_action_mobius_and_zmobius():

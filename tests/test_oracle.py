@@ -260,3 +260,40 @@ def test_power_iteration_bounds_hermop(small):
     ax = inv.apply_half("hermop_sym2", x, 1)
     rayleigh = np.vdot(x, ax).real / np.vdot(x, x).real
     assert 0 < rayleigh <= lam * (1 + 1e-6)
+
+
+def test_invert_with_guess_pins(small):
+    """invert_with_cg_with_guess (dslash.h:532-573): a zero guess is
+    invert(5d) itself (ratio 1, the same CG sequence); from a rough guess the
+    corrected solution still satisfies M x = D_- b (dslash-tests:233-242);
+    from the exact solution of a tighter solve the correction reaches the
+    tolerance in fewer iterations than from zero."""
+    V, inv = small["V"], small["inv"]
+    inv.set_params(1e-10, 200, 300)
+    b = rand_field((V, LS, 12), 16)
+    x0, it0 = inv.invert_5d(b)
+    xg, itg = inv.invert_5d_with_guess(b, np.zeros_like(b))
+    assert itg == it0 and np.array_equal(xg, x0)
+    rough = x0 + 1e-3 * rand_field(b.shape, 17)
+    xr, itr = inv.invert_5d_with_guess(b, rough)
+    db = inv.apply_full("d_minus", b)
+    assert rel(inv.apply_full("m", xr), db) < 1e-8
+    assert rel(inv.apply_full("m", rough), db) > 1e-5  # the guess alone is far from it
+    assert 0 < itr < it0
+    inv.set_params(1e-8, 200, 300)
+
+
+def test_five_d_residual_of_invert(small):
+    """The is_checking_invert quantity (dslash.h:522-528) of a solve to
+    stop_rsd lies within a condition factor of stop_rsd: the CG stops on the
+    preconditioned odd system, so |M x - D_- b| / |D_- b| is of the order of
+    stop_rsd, not exactly it."""
+    V, inv = small["V"], small["inv"]
+    b = rand_field((V, LS, 12), 18)
+    for stop in (1e-6, 1e-10):
+        inv.set_params(stop, 200, 300)
+        x, _ = inv.invert_5d(b)
+        db = inv.apply_full("d_minus", b)
+        r = rel(inv.apply_full("m", x), db)
+        assert 1e-3 * stop < r < 1e3 * stop, (stop, r)
+    inv.set_params(1e-8, 200, 300)

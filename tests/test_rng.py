@@ -148,12 +148,16 @@ def test_device_fill_rejects_bad_arguments():
     from qlattice_amd import _lib
 
     L = _lib.load()
-    mid = (ctypes.c_uint32 * 8)()
+    mid = [0] * 8
     out = np.zeros(8)
-    dp = out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-    assert L.qlat_dwf_rng_fill_uniform(mid, 64, 0, 8, 0, 1.0, 0.0, dp) != 0
-    assert b"n_per_site" in L.qlat_dwf_last_error()
-    assert L.qlat_dwf_rng_fill_uniform(mid, 63, 0, 8, 1, 1.0, 0.0, dp) != 0
-    assert b"64-byte" in L.qlat_dwf_last_error()
-    assert L.qlat_dwf_rng_fill_uniform(None, 64, 0, 8, 1, 1.0, 0.0, dp) != 0
-    assert L.qlat_dwf_rng_fill_uniform(mid, 64, 0, 0, 1, 1.0, 0.0, dp) == 0  # nothing to fill
+    dp = _lib.addr(out, np.float64)
+    with pytest.raises(q.QlatDwfError, match="n_per_site"):
+        L.rng_fill(False, mid, 64, 0, 8, 0, 1.0, 0.0, dp)
+    with pytest.raises(q.QlatDwfError, match="64-byte"):
+        L.rng_fill(False, mid, 63, 0, 8, 1, 1.0, 0.0, dp)
+    L.rng_fill(False, mid, 64, 0, 0, 1, 1.0, 0.0, dp)  # nothing to fill
+    # a null midstate pointer is an error of the C entry point
+    so = ctypes.CDLL(_lib.LIB_PATH)
+    so.qlat_dwf_rng_fill_uniform.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
+                                             ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    assert so.qlat_dwf_rng_fill_uniform(None, 64, 0, 8, 1, 1.0, 0.0, dp) != 0
