@@ -185,6 +185,8 @@ struct qlat_dwf_inverter {
 
   double2* l64 = nullptr;
   float2* l32 = nullptr;
+  unsigned* l16 = nullptr;  // FP16 links for the FP16-arithmetic hop (h2_hop)
+  bool h2_hop = true;       // FP16-storage solver: hop phase in FP16 arithmetic (QLAT_DWF_H2)
   cplx<double>* mat64 = nullptr;  // [MAT_COUNT][2][ls][ls]
   cplx<float>* mat32 = nullptr;
   cplx<double>* beta64 = nullptr;  // [2][ls]: ones, c~
@@ -1144,6 +1146,7 @@ struct ThreePhase {
     a.out = out;
     a.x = x;
     a.links = I.l32;
+    a.links16 = (half() && I.h2_hop) ? I.l16 : nullptr;
     a.sb = sb;
     a.site_major = I.site_major;
     a.cg = cg;
@@ -2101,10 +2104,12 @@ void pack_gauge_dev(Inv& I, const double2* g)
   if (!I.l64) {
     I.l64 = (double2*)dev_alloc(lcount * sizeof(double2));
     I.l32 = (float2*)dev_alloc(lcount * sizeof(float2));
+    I.l16 = (unsigned*)dev_alloc(lcount * sizeof(unsigned));
     QDWF_HIP(hipMemsetAsync(I.l64, 0, lcount * sizeof(double2), I.stream));
     QDWF_HIP(hipMemsetAsync(I.l32, 0, lcount * sizeof(float2), I.stream));
+    QDWF_HIP(hipMemsetAsync(I.l16, 0, lcount * sizeof(unsigned), I.stream));
   }
-  qdwf::launch_pack_gauge(g, I.l64, I.l32, I.geo, I.stream);
+  qdwf::launch_pack_gauge(g, I.l64, I.l32, I.l16, I.geo, I.stream);
   check_launch();
 }
 
@@ -2232,6 +2237,7 @@ void free_all(Inv* I)
   };
   f(I->l64);
   f(I->l32);
+  f(I->l16);
   f(I->mat64);
   f(I->mat32);
   f(I->beta64);
@@ -2362,6 +2368,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_XDEFER")) I->x_defer = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_STEP_SB")) I->step_sb = std::atoi(e) == 16 ? 16 : 32;
     if (const char* e = std::getenv("QLAT_DWF_GRAPH")) I->graph_iters = std::max(0, std::min(64, std::atoi(e)));
+    if (const char* e = std::getenv("QLAT_DWF_H2")) I->h2_hop = std::atoi(e) != 0;
     {
       // partial sums per column: BLAS grids, one per block of the fused
       // kernels (32 sites) and of the three-phase kernels (16 sites, up to

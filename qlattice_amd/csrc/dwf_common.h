@@ -602,4 +602,88 @@ __device__ __forceinline__ void hop_term(const cplx<T> W[9], const cplx<T> psi[1
   }
 }
 
+// ------------------------------------------- FP16-arithmetic hop term
+// The same hop term with the neighbour spinor as stored (FP16 values of a
+// site, times the site's scale sc) and FP16 links: the projection is one
+// packed FP16 add per colour and spin (the swizzle and signs of B fold into
+// the add's operand modifiers), each SU(3) row is three 2-term FP16 dot
+// products accumulated in FP32 (v_dot2_f32_f16: the products are exact in
+// FP32), and the scale is applied when the FP32 result is accumulated.  A
+// link element enters as la = (re, -im) and lb = (im, re), so that
+// Re(U w) = la . w and Im(U w) = lb . w.
+
+// w = u + sg*B l for one colour, packed (x = re, y = im)
+template <int MU, int SG>
+__device__ __forceinline__ void proj_h2(h16x2 u0, h16x2 u1, h16x2 l0, h16x2 l1, h16x2& w0, h16x2& w1)
+{
+  const _Float16 s = SG > 0 ? (_Float16)1 : (_Float16)-1;
+  if (MU == 0) {  // B l = (i l1, i l0)
+    w0 = u0 + s * (h16x2){-l1.y, l1.x};
+    w1 = u1 + s * (h16x2){-l0.y, l0.x};
+  } else if (MU == 1) {  // B l = (-l1, l0)
+    w0 = u0 - s * l1;
+    w1 = u1 + s * l0;
+  } else if (MU == 2) {  // B l = (i l0, -i l1)
+    w0 = u0 + s * (h16x2){-l0.y, l0.x};
+    w1 = u1 - s * (h16x2){-l1.y, l1.x};
+  } else {  // B l = l
+    w0 = u0 + s * l0;
+    w1 = u1 + s * l1;
+  }
+}
+
+// (t2, t3) = B^+ (y0, y1)
+template <int MU>
+__device__ __forceinline__ void bdag(cplx<float> y0, cplx<float> y1, cplx<float>& t2, cplx<float>& t3)
+{
+  if (MU == 0) {  // (-i y1, -i y0)
+    t2 = cmk<float>(y1.im, -y1.re);
+    t3 = cmk<float>(y0.im, -y0.re);
+  } else if (MU == 1) {  // (y1, -y0)
+    t2 = y1;
+    t3 = cneg(y0);
+  } else if (MU == 2) {  // (-i y0, i y1)
+    t2 = cmk<float>(y0.im, -y0.re);
+    t3 = cmk<float>(-y1.im, y1.re);
+  } else {  // (y0, y1)
+    t2 = y0;
+    t3 = y1;
+  }
+}
+
+__device__ __forceinline__ void cfma_s(cplx<float>& acc, float s, cplx<float> t)
+{
+  acc.re = fmaf(s, t.re, acc.re);
+  acc.im = fmaf(s, t.im, acc.im);
+}
+
+// acc += sc * (1 + sg gamma_mu) U psi with psi, U in FP16 (la, lb as above)
+template <int MU, int SG>
+__device__ __forceinline__ void hop_term_h2(const h16x2 la[9], const h16x2 lb[9], const h16x2 psi[12], float sc,
+                                            cplx<float> acc[12])
+{
+  h16x2 w0[3], w1[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) proj_h2<MU, SG>(psi[c], psi[3 + c], psi[6 + c], psi[9 + c], w0[c], w1[c]);
+  const float ssg = SG > 0 ? sc : -sc;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float y0r = 0.f, y0i = 0.f, y1r = 0.f, y1i = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      y0r = __builtin_amdgcn_fdot2(la[r * 3 + c], w0[c], y0r, false);
+      y0i = __builtin_amdgcn_fdot2(lb[r * 3 + c], w0[c], y0i, false);
+      y1r = __builtin_amdgcn_fdot2(la[r * 3 + c], w1[c], y1r, false);
+      y1i = __builtin_amdgcn_fdot2(lb[r * 3 + c], w1[c], y1i, false);
+    }
+    const cplx<float> y0 = cmk<float>(y0r, y0i), y1 = cmk<float>(y1r, y1i);
+    cfma_s(acc[r], sc, y0);
+    cfma_s(acc[3 + r], sc, y1);
+    cplx<float> t2, t3;
+    bdag<MU>(y0, y1, t2, t3);
+    cfma_s(acc[6 + r], ssg, t2);
+    cfma_s(acc[9 + r], ssg, t3);
+  }
+}
+
 }  // namespace qdwf

@@ -120,6 +120,10 @@ struct HopM5Args {
   void* out;
   const void* x;  // optional
   const void* links;
+  // FP16 links (h16x2 (re, im) in the 8-link layout): with FP16 spinor
+  // storage, the hop phase runs in FP16 arithmetic (hop_term_h2); null keeps
+  // FP32 arithmetic
+  const void* links16;
   const cplx<float>* mat;  // [2][ls][ls]
   bool real_mat;
   const void* dotp;  // optional (dag xpay form): part[i*nblk + block] = Re <dotp, out>
@@ -313,6 +317,8 @@ void launch_prop_col_to_5d(const double2* src, double2* soa, const LatGeo& g, in
                            hipStream_t st);
 void launch_5d_to_prop_col(const double2* soa, double2* dst, const LatGeo& g, int ls, int eo, int col,
                            hipStream_t st);
-void launch_pack_gauge(const double2* gauge, double2* l64, float2* l32, const LatGeo& g, hipStream_t st);
+// l16 (optional): FP16 copy of the links, h16x2 (re, im) per element
+void launch_pack_gauge(const double2* gauge, double2* l64, float2* l32, unsigned* l16, const LatGeo& g,
+                       hipStream_t st);
 
 }  // namespace qdwf

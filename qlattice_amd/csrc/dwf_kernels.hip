@@ -481,12 +481,16 @@ __device__ __forceinline__ void store_qs(typename SpinorIO<ST>::slot_t* __restri
   else SpinorIO<ST>::store_stream(f, ls, s, h, v);
 }
 
-template <class ST, int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT, bool SMAT>
+template <class ST, int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT, bool SMAT, bool H2>
 __global__ void __launch_bounds__(SB * LSMAX) __attribute__((amdgpu_waves_per_eu(3)))
 k_hop_m5s(HopM5Args a)
 {
   typedef typename SpinorIO<ST>::slot_t slot_t;
-  __shared__ float2 sl[72][SB];
+  static_assert(!H2 || std::is_same<ST, Half16>::value, "FP16 arithmetic needs FP16 storage");
+  // links of the block's sites: FP32 complex, or (H2) the FP16 forms
+  // la = (re, -im), lb = (im, re) of hop_term_h2
+  __shared__ float2 sl[H2 ? 1 : 72][SB];
+  __shared__ h16x2 sla[H2 ? 72 : 1][SB], slb[H2 ? 72 : 1][SB];
   // block numbers are below 2^32: 32-bit divisions
   const unsigned lbc = (unsigned)xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
   const int ncol = a.cols.n;
@@ -498,8 +502,17 @@ k_hop_m5s(HopM5Args a)
   if (a.cg && a.cg->mode[ci] != 0) return;  // frozen column (the whole block)
   const int64_t h0 = (int64_t)lb * SB;
   const int pslot = (a.eo_out == 2) ? 0 : 1;
-  const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
-  for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) sl[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
+  if constexpr (H2) {
+    const unsigned* __restrict__ l16 = (const unsigned*)a.links16 + lidx(a.geo.stride, pslot, 0, h0);
+    for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) {
+      const h16x2 u = __builtin_bit_cast(h16x2, l16[(i / SB) * 64 + (i % SB)]);
+      sla[i / SB][i % SB] = (h16x2){u.x, -u.y};
+      slb[i / SB][i % SB] = (h16x2){u.y, u.x};
+    }
+  } else {
+    const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
+    for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) sl[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
+  }
   // neighbour indices once per site (row 0), shared by the Ls rows
   __shared__ int nbs[8][SB];
   if (threadIdx.x < SB && h0 + threadIdx.x < a.geo.vh) {
@@ -532,7 +545,22 @@ k_hop_m5s(HopM5Args a)
     typename SpinorIO<ST>::Raw raw[NB];
 #define QDWF_LD(J, D) SpinorIO<ST>::load_raw(in, ls, row, nb[D], raw[J]);
 #define QDWF_CP(J, D, MU, SG)                                       \
-  {                                                                 \
+  if constexpr (H2) {                                               \
+    h16x2 la[9], lb[9], hp[12];                                     \
+    _Pragma("unroll") for (int e = 0; e < 9; ++e)                   \
+    {                                                               \
+      la[e] = sla[(D) * 9 + e][site];                               \
+      lb[e] = slb[(D) * 9 + e][site];                               \
+    }                                                               \
+    _Pragma("unroll") for (int j = 0; j < 3; ++j)                   \
+    {                                                               \
+      hp[4 * j + 0] = __builtin_bit_cast(h16x2, raw[J].q[j].x);     \
+      hp[4 * j + 1] = __builtin_bit_cast(h16x2, raw[J].q[j].y);     \
+      hp[4 * j + 2] = __builtin_bit_cast(h16x2, raw[J].q[j].z);     \
+      hp[4 * j + 3] = __builtin_bit_cast(h16x2, raw[J].q[j].w);     \
+    }                                                               \
+    hop_term_h2<MU, SG>(la, lb, hp, raw[J].sc, acc);                \
+  } else {                                                          \
     cplx<float> W[9];                                               \
     _Pragma("unroll") for (int e = 0; e < 9; ++e)                   \
     {                                                               \
@@ -626,18 +654,18 @@ k_hop_m5s(HopM5Args a)
   }
 }
 
-template <class ST, int SB, int LSMAX>
+template <class ST, int SB, int LSMAX, bool H2>
 static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
 {
   const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB * a.cols.n));
   const dim3 blk(SB * a.ls);
   const bool xpay = a.x != nullptr;
-#define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, D, X, R, DT, true>), grid, blk, 0, st, a)
+#define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, D, X, R, DT, true, H2>), grid, blk, 0, st, a)
   if (!a.mats[0]) {  // plain hop: out = alpha H in + beta x
-    if (!dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, false, true, false, false>), grid, blk, 0, st, a);
-    if (!dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, true, true, false, false>), grid, blk, 0, st, a);
-    if (dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, false, true, false, false>), grid, blk, 0, st, a);
-    if (dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, true, true, false, false>), grid, blk, 0, st, a);
+    if (!dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, false, true, false, false, H2>), grid, blk, 0, st, a);
+    if (!dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, true, true, false, false, H2>), grid, blk, 0, st, a);
+    if (dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, false, true, false, false, H2>), grid, blk, 0, st, a);
+    if (dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, true, true, false, false, H2>), grid, blk, 0, st, a);
     return;
   }
   if (a.dotp) {
@@ -664,23 +692,26 @@ static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
 bool three_phase_ok(int ls, int sb) { return ls >= 6 && ls <= 16 && (sb == 16 || sb == 32) && (sb * ls) % 64 == 0; }
 int64_t hop_m5s_blocks(const LatGeo& g, int sb) { return (g.vh + sb - 1) / sb; }
 
-template <class ST, int SB>
+template <class ST, int SB, bool H2>
 static void launch_hop_m5s_sb(const HopM5Args& a, bool dag, hipStream_t st)
 {
-  if (a.ls <= 8) launch_hop_m5s_ls<ST, SB, 8>(a, dag, st);
-  else if (a.ls <= 12) launch_hop_m5s_ls<ST, SB, 12>(a, dag, st);
-  else launch_hop_m5s_ls<ST, SB, 16>(a, dag, st);
+  if (a.ls <= 8) launch_hop_m5s_ls<ST, SB, 8, H2>(a, dag, st);
+  else if (a.ls <= 12) launch_hop_m5s_ls<ST, SB, 12, H2>(a, dag, st);
+  else launch_hop_m5s_ls<ST, SB, 16, H2>(a, dag, st);
 }
 
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
 {
   if (!three_phase_ok(a.ls, a.sb)) return false;
-  if (half) {
-    if (a.sb == 16) launch_hop_m5s_sb<Half16, 16>(a, dag, st);
-    else launch_hop_m5s_sb<Half16, 32>(a, dag, st);
+  if (half && a.links16) {
+    if (a.sb == 16) launch_hop_m5s_sb<Half16, 16, true>(a, dag, st);
+    else launch_hop_m5s_sb<Half16, 32, true>(a, dag, st);
+  } else if (half) {
+    if (a.sb == 16) launch_hop_m5s_sb<Half16, 16, false>(a, dag, st);
+    else launch_hop_m5s_sb<Half16, 32, false>(a, dag, st);
   } else {
-    if (a.sb == 16) launch_hop_m5s_sb<float, 16>(a, dag, st);
-    else launch_hop_m5s_sb<float, 32>(a, dag, st);
+    if (a.sb == 16) launch_hop_m5s_sb<float, 16, false>(a, dag, st);
+    else launch_hop_m5s_sb<float, 32, false>(a, dag, st);
   }
   return true;
 }
@@ -2339,7 +2370,8 @@ void launch_5d_to_prop_col(const double2* soa, double2* dst, const LatGeo& g, in
 // (dwf_common.h), FP64 and FP32 copies.  Backward links are stored already
 // adjointed: U_mu(x - mu)^+.
 __global__ void __launch_bounds__(kBlasBlock) k_pack_gauge(const double2* __restrict__ gauge, double2* __restrict__ l64,
-                                                           float2* __restrict__ l32, LatGeo g)
+                                                           float2* __restrict__ l32, unsigned* __restrict__ l16,
+                                                           LatGeo g)
 {
   const int64_t n = g.vh * 2 * 8;
   for (int64_t i = (int64_t)blockIdx.x * kBlasBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlasBlock) {
@@ -2358,14 +2390,16 @@ __global__ void __launch_bounds__(kBlasBlock) k_pack_gauge(const double2* __rest
         const int64_t off = lidx(g.stride, p, d * 9 + r * 3 + c, h);
         l64[off] = v;
         l32[off] = make_float2((float)v.x, (float)v.y);
+        if (l16) l16[off] = pack_h2((float)v.x, (float)v.y);
       }
   }
 }
 
-void launch_pack_gauge(const double2* gauge, double2* l64, float2* l32, const LatGeo& g, hipStream_t st)
+void launch_pack_gauge(const double2* gauge, double2* l64, float2* l32, unsigned* l16, const LatGeo& g,
+                       hipStream_t st)
 {
   const int64_t n = g.vh * 16;
-  hipLaunchKernelGGL(k_pack_gauge, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, gauge, l64, l32, g);
+  hipLaunchKernelGGL(k_pack_gauge, dim3(blas_grid(n)), dim3(kBlasBlock), 0, st, gauge, l64, l32, l16, g);
 }
 
 

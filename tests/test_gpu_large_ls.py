@@ -17,9 +17,13 @@ SITE = (4, 4, 4, 8)
 
 def zmobius_omega(ls):
     """A zMoebius-like omega: real values falling with s, the last four as
-    two complex-conjugate pairs (the shape of free-invert.py:49-67)."""
-    w = [1.1 * 0.8 ** s for s in range(ls - 4)]
-    w += [complex(0.06, -0.008), complex(0.06, 0.008), complex(0.037, -0.033), complex(0.037, 0.033)]
+    two complex-conjugate pairs (the shape of free-invert.py:49-67), kept
+    above 0.1 so that M_ee stays well conditioned at these Ls (a smaller
+    omega makes b_s = (1/omega_s + 1)/2 large, and the FP64 solves of the
+    dense-matrix and the LDU forms of M_ee^-1 then part by more than their
+    1e-10 bar through rounding alone)."""
+    w = [1.1 * 0.92 ** s for s in range(ls - 4)]
+    w += [complex(0.2, -0.02), complex(0.2, 0.02), complex(0.12, -0.03), complex(0.12, 0.03)]
     return w
 
 
