@@ -49,8 +49,9 @@ extern "C" {
 #define QLAT_DWF_SOLVER_FP64 0       /* the reference algorithm, FP64 throughout */
 #define QLAT_DWF_SOLVER_MIXED_FP32 1 /* FP32 CG with FP64 reliable updates      */
 #define QLAT_DWF_SOLVER_MIXED_FP16 2 /* as MIXED_FP32, Krylov vectors stored in FP16
-                                        with a per-5D-site scale (6 <= ls <= 16, ls
-                                        even; FP32 storage otherwise)              */
+                                        with a per-5D-site scale (ls even in
+                                        [6, 16], or 20 or 24; FP32 storage
+                                        otherwise)                                 */
 
 typedef struct qlat_dwf_inverter qlat_dwf_inverter;
 

@@ -173,6 +173,7 @@ struct qlat_dwf_inverter {
   // hop launch shapes (launch_hop): LDS-staged links; FP64 with streaming stores
   int hop_spt64 = -102, hop_spt32 = -4;
   bool fused32 = true;  // FP32 hop + s-matrix fusion (QLAT_DWF_FUSED=0 disables)
+  bool fused64 = true;  // FP64 hop + s-matrix fusion, k_hop_m5d (QLAT_DWF_FUSED64=0 disables)
   bool three_phase32 = true;   // FP32 solver on the three-phase kernels (QLAT_DWF_THREE_PHASE=0: fused
                                // FP32 kernels, single s-matrix pass per hop)
   int tp_sb = 32;              // sites per block of the three-phase kernels (QLAT_DWF_TP_SB)
@@ -186,7 +187,11 @@ struct qlat_dwf_inverter {
   double2* l64 = nullptr;
   float2* l32 = nullptr;
   unsigned* l16 = nullptr;  // FP16 links for the FP16-arithmetic hop (h2_hop)
-  bool h2_hop = true;       // FP16-storage solver: hop phase in FP16 arithmetic (QLAT_DWF_H2)
+  // FP16-storage solver: hop phase in FP16 arithmetic (QLAT_DWF_H2=1).  Off by
+  // default: 40% fewer VALU instructions, but 9% slower at 24^3x64 Ls12
+  // (1.281 against 1.175 ms for b = K H t over 4 columns), the kernel being
+  // bound by its memory requests, not its arithmetic
+  bool h2_hop = false;
   cplx<double>* mat64 = nullptr;  // [MAT_COUNT][2][ls][ls]
   cplx<float>* mat32 = nullptr;
   cplx<double>* beta64 = nullptr;  // [2][ls]: ones, c~
@@ -472,6 +477,30 @@ bool hop_m5_32(Inv& I, float4* out, const float4* in, int eo_out, bool dag, int 
   return true;
 }
 
+// FP64 fused out = alpha * Mat (H in) + beta * x (false if not available)
+bool hop_m5_64(Inv& I, double2* out, const double2* in, int eo_out, bool dag, int mat, double alpha = 1.0,
+               const double2* x = nullptr, double beta = 0.0, const double2* dotp = nullptr)
+{
+  if (!I.fused64) return false;
+  qdwf::HopM5dArgs a{};
+  a.geo = I.geo;
+  a.ls = I.ls;
+  a.eo_out = eo_out;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.in = in;
+  a.out = out;
+  a.x = x;
+  a.links = I.l64;
+  a.mat = Prec<double>::mat(I, mat);
+  a.real_mat = I.real_mat[mat];
+  a.dotp = dotp;
+  a.part = I.part;
+  if (!qdwf::launch_hop_m5d(a, dag, I.stream)) return false;
+  check_launch();
+  return true;
+}
+
 // Mpc = 1 - H K H K
 template <class T>
 void op_mpc(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* in, int eo)
@@ -481,6 +510,11 @@ void op_mpc(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot* i
   m5<T>(I, t1, in, MAT_K);
   if constexpr (std::is_same<T, float>::value) {
     if (hop_m5_32(I, t2, t1, 3 - eo, false, MAT_K)) {
+      hop<T>(I, out, t2, eo, false, (T)-1, in, 0);
+      return;
+    }
+  } else {
+    if (hop_m5_64(I, t2, t1, 3 - eo, false, MAT_K)) {
       hop<T>(I, out, t2, eo, false, (T)-1, in, 0);
       return;
     }
@@ -499,6 +533,11 @@ void op_mpcdag(Inv& I, typename Prec<T>::slot* out, const typename Prec<T>::slot
   if constexpr (std::is_same<T, float>::value) {
     if (hop_m5_32(I, t2, in, 3 - eo, true, MAT_K_DAG)) {
       hop_m5_32(I, out, t2, eo, true, MAT_K_DAG, -1.f, in, 1.f);
+      return;
+    }
+  } else {
+    if (hop_m5_64(I, t2, in, 3 - eo, true, MAT_K_DAG)) {
+      hop_m5_64(I, out, t2, eo, true, MAT_K_DAG, -1.0, in, 1.0);
       return;
     }
   }
@@ -533,6 +572,15 @@ void hermop_t1_dot(Inv& I, typename Prec<T>::slot* ap, const typename Prec<T>::s
       hop_m5_32(I, B, Y, 3 - eo, true, MAT_K_DAG, 1.f, nullptr, 0.f, nullptr, &cols);  // u = K^+ H^+ y
       hop_m5_32(I, ap, B, eo, true, MAT_K_DAG, -1.f, Y, 1.f, p, &cols);      // ap = y - K^+ H^+ u
       nparts = qdwf::hop_m5_blocks(I.geo);
+      done = true;
+    }
+  } else {
+    if (cols.n == 1 && cols.rhs[0] == 0 && I.fused64 && qdwf::hop_m5d_ok(I.ls)) {
+      hop_m5_64(I, B, t1, 3 - eo, false, MAT_K);                      // b = K H K p
+      hop<T>(I, Y, B, eo, false, (T)-1, p, 0, &cols);                  // y = p - H b
+      hop_m5_64(I, B, Y, 3 - eo, true, MAT_K_DAG);                     // u = K^+ H^+ y
+      hop_m5_64(I, ap, B, eo, true, MAT_K_DAG, -1.0, Y, 1.0, p);       // ap = y - K^+ H^+ u
+      nparts = qdwf::hop_m5d_blocks(I.geo);
       done = true;
     }
   }
@@ -992,8 +1040,9 @@ struct ThreePhase {
   ThreePhase(Inv& inv, int ncol, int storage_, int sb_ = 0)
       : I(inv), storage(storage_), sb(sb_ ? sb_ : inv.tp_sb)
   {
-    if (!qdwf::three_phase_ok(I.ls, sb)) sb = 32;
-    if (!qdwf::three_phase_ok(I.ls, sb)) throw ArgError("the three-phase kernels need 6 <= ls <= 16, ls even");
+    if (!qdwf::three_phase_ok(I.ls, sb)) sb = (sb == 32) ? 16 : 32;
+    if (!qdwf::three_phase_ok(I.ls, sb))
+      throw ArgError("the three-phase kernels need an even ls in [6, 16], or ls 20 or 24");
     if (storage && fieldh_units(I) * 16 >= (1ll << 32))
       throw ArgError("FP16 spinor storage needs V/2 * Ls * 52 bytes below 4 GB per field");
     for (auto*& p : colinv) p = &I;
@@ -1538,7 +1587,7 @@ TileChoice tile_for(Inv& I, int storage, int ncol)
 
 bool use_three_phase(const Inv& I)
 {
-  return (I.solver == QLAT_DWF_SOLVER_MIXED_FP16 || I.three_phase32) && qdwf::three_phase_ok(I.ls);
+  return (I.solver == QLAT_DWF_SOLVER_MIXED_FP16 || I.three_phase32) && qdwf::three_phase_ls_ok(I.ls);
 }
 
 double true_residual_fp64(Inv& I, const double2* x, const double2* b)
@@ -1917,7 +1966,7 @@ long solve_columns(Inv& I, int n, Inv* const* colinv, Load&& load, Store&& store
 {
   bool several = false;
   for (int j = 0; j < n; ++j) several = several || colinv[j] != &I;
-  const bool three = use_three_phase(I) || (several && qdwf::three_phase_ok(I.ls));
+  const bool three = use_three_phase(I) || (several && qdwf::three_phase_ls_ok(I.ls));
   const int storage = storage_of(I);
   TileChoice tc{I.rhs_tile > 0 ? I.rhs_tile : 12, I.tp_sb};
   if (three) tc = tile_for(I, storage, n);
@@ -2360,6 +2409,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT32")) I->hop_spt32 = std::atoi(e);
     if (const char* e = std::getenv("QLAT_DWF_HOP_SPT64")) I->hop_spt64 = std::atoi(e);
     if (const char* e = std::getenv("QLAT_DWF_FUSED")) I->fused32 = std::atoi(e) != 0;
+    if (const char* e = std::getenv("QLAT_DWF_FUSED64")) I->fused64 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_THREE_PHASE")) I->three_phase32 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_RHS_TILE")) I->rhs_tile = std::max(0, std::min(qdwf::kMaxRhs, std::atoi(e)));
     if (const char* e = std::getenv("QLAT_DWF_TP_SB")) I->tp_sb = std::atoi(e) == 16 ? 16 : 32;

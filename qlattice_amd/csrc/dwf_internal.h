@@ -141,6 +141,28 @@ struct HopM5Args {
   const CgState* cg;  // optional: blocks of columns with cg->mode != 0 do nothing
 };
 
+// FP64 fused hop + s-matrix (k_hop_m5d): out = alpha * Mat (H in) + beta * x
+// on one FP64 field; with dotp (dagger xpay form) part[block] = Re <dotp, out>.
+struct HopM5dArgs {
+  LatGeo geo;
+  int ls;
+  int eo_out;
+  double alpha, beta;
+  const double2* in;
+  double2* out;
+  const double2* x;     // optional
+  const double2* links; // FP64 8-link layout
+  const cplx<double>* mat;  // [2][ls][ls]
+  bool real_mat;
+  const double2* dotp;  // optional
+  double* part;
+};
+// false (nothing launched) when ls does not fit (4 <= ls <= 16, 16 ls a
+// multiple of 64) or the form is not available
+bool launch_hop_m5d(const HopM5dArgs& a, bool dag, hipStream_t st);
+bool hop_m5d_ok(int ls);
+int64_t hop_m5d_blocks(const LatGeo& g);
+
 // r_i -= alpha_i ap_i with per-column partial |r_i|^2 (part[i*nblk + block])
 template <class S>
 struct AxpyNormArgs {
@@ -187,6 +209,8 @@ int64_t hop_m5_blocks(const LatGeo& g);
 // Three-phase kernels for FP32 or FP16 (per-site scale) spinor storage
 // (half = true); false when ls does not fit them (6 <= ls <= 16, ls even).
 bool three_phase_ok(int ls, int sb = 32);
+// some sites-per-block choice fits ls
+inline bool three_phase_ls_ok(int ls) { return three_phase_ok(ls, 32) || three_phase_ok(ls, 16); }
 int64_t hop_m5s_blocks(const LatGeo& g, int sb);
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st);
 // storage: 0 all FP32; 1 p, t (ap) FP16 and x, r FP32; 2 all FP16

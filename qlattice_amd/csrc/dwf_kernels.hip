@@ -688,8 +688,12 @@ static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
 }
 
 // phase 2 needs >= 6 rows; whole waves (sb * ls multiple of 64) for the
-// block reductions
-bool three_phase_ok(int ls, int sb) { return ls >= 6 && ls <= 16 && (sb == 16 || sb == 32) && (sb * ls) % 64 == 0; }
+// block reductions; above Ls 16 (the 24-row instances) only 16 sites per
+// block, which keeps a block's LDS (rows x 12 complex x sites) below 40 KB
+bool three_phase_ok(int ls, int sb)
+{
+  return ls >= 6 && ls <= 24 && (sb == 16 || (sb == 32 && ls <= 16)) && (sb * ls) % 64 == 0;
+}
 int64_t hop_m5s_blocks(const LatGeo& g, int sb) { return (g.vh + sb - 1) / sb; }
 
 template <class ST, int SB, bool H2>
@@ -697,7 +701,8 @@ static void launch_hop_m5s_sb(const HopM5Args& a, bool dag, hipStream_t st)
 {
   if (a.ls <= 8) launch_hop_m5s_ls<ST, SB, 8, H2>(a, dag, st);
   else if (a.ls <= 12) launch_hop_m5s_ls<ST, SB, 12, H2>(a, dag, st);
-  else launch_hop_m5s_ls<ST, SB, 16, H2>(a, dag, st);
+  else if (a.ls <= 16) launch_hop_m5s_ls<ST, SB, 16, H2>(a, dag, st);
+  else if constexpr (SB == 16) launch_hop_m5s_ls<ST, SB, 24, H2>(a, dag, st);
 }
 
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
@@ -713,6 +718,171 @@ bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
     if (a.sb == 16) launch_hop_m5s_sb<float, 16, false>(a, dag, st);
     else launch_hop_m5s_sb<float, 32, false>(a, dag, st);
   }
+  return true;
+}
+
+// ================================================ FP64 hop + s-matrix
+// out = alpha * Mat (H in) + beta * x on FP64 fields (one column), in the
+// three phases of k_hop_m5s: a block is SB output half-sites x Ls rows;
+// phase 1, thread (site, s) forms the hop of its 5D site with the block's
+// links staged in LDS; phase 2, the Ls hop results of each site go through
+// LDS and thread (site, s) forms row s of the dense s-matrix product;
+// phase 3 adds beta * x, stores, and reduces Re <dotp, out> per block.
+// The links' LDS space is reused for the hop results once every thread
+// has finished phase 1.
+template <int SB, int LSMAX>
+__device__ __forceinline__ void stage_km_d(cplx<double> (*km)[LSMAX][LSMAX], const cplx<double>* __restrict__ M,
+                                           int ls)
+{
+  for (int i = threadIdx.x; i < 2 * LSMAX * LSMAX; i += blockDim.x) {
+    const int chi = i / (LSMAX * LSMAX), r = (i / LSMAX) % LSMAX, t = i % LSMAX;
+    if (r < ls && t < ls) km[chi][r][t] = M[(chi * ls + r) * ls + t];
+  }
+}
+
+template <int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT>
+__global__ void __launch_bounds__(SB * LSMAX) k_hop_m5d(HopM5dArgs a)
+{
+  constexpr int NSLOT = (72 > LSMAX * 12) ? 72 : LSMAX * 12;
+  __shared__ double2 buf[NSLOT][SB];  // links [72][SB], then hop results [LSMAX * 12][SB]
+  __shared__ int nbs[8][SB];
+  __shared__ cplx<double> km[2][LSMAX][LSMAX];
+  const unsigned lb = (unsigned)xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t h0 = (int64_t)lb * SB;
+  const int pslot = (a.eo_out == 2) ? 0 : 1;
+  const double2* __restrict__ links = a.links + lidx(a.geo.stride, pslot, 0, h0);
+  for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) buf[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
+  if (threadIdx.x < SB && h0 + threadIdx.x < a.geo.vh) {
+    int x[4];
+    half_coord(a.geo, h0 + threadIdx.x, a.eo_out, x);
+    int64_t nb[8];
+    neighbours(a.geo, x, nb);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) nbs[d][threadIdx.x] = (int)nb[d];
+  }
+  stage_km_d<SB, LSMAX>(km, a.mat, a.ls);
+  __syncthreads();
+  const int ls = a.ls;
+  const int site = threadIdx.x % SB, row = threadIdx.x / SB;
+  const int64_t h = h0 + site;
+  const bool active = h < a.geo.vh && row < ls;
+  cplx<double> acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = cmk<double>(0, 0);
+  if (active) {
+    constexpr int SF = DAG ? 1 : -1;
+    constexpr int SG = -SF;
+#define QDWF_D(D, MU, SGN)                                         \
+  {                                                                \
+    cplx<double> W[9];                                             \
+    _Pragma("unroll") for (int e = 0; e < 9; ++e)                  \
+    {                                                              \
+      const double2 q = buf[(D) * 9 + e][site];                    \
+      W[e] = cmk<double>(q.x, q.y);                                \
+    }                                                              \
+    cplx<double> psi[12];                                          \
+    SpinorIO<double>::load(a.in, ls, row, nbs[D][site], psi);      \
+    hop_term<MU, SGN>(W, psi, acc);                                \
+  }
+    QDWF_D(0, 0, SF) QDWF_D(1, 1, SF) QDWF_D(2, 2, SF) QDWF_D(3, 3, SF)
+    QDWF_D(4, 0, SG) QDWF_D(5, 1, SG) QDWF_D(6, 2, SG) QDWF_D(7, 3, SG)
+#undef QDWF_D
+  }
+  const double sc = -0.5 * a.alpha;
+  __syncthreads();  // every thread is done with the links: buf becomes hr
+  double2 (*hr)[SB] = buf;  // hr[row * 12 + k][site]
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) hr[row * 12 + k][site] = make_double2(sc * acc[k].re, sc * acc[k].im);
+  }
+  __syncthreads();
+  double dot = 0.0;
+  if (active) {
+    cplx<double> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cmk<double>(0, 0);
+#pragma unroll
+    for (int t = 0; t < LSMAX; ++t) {
+      if (t < ls) {
+        const cplx<double> k0 = km[0][row][t], k1 = km[1][row][t];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          const cplx<double> m = k < 6 ? k0 : k1;  // spins 0, 1: chirality 0
+          const double2 q = hr[t * 12 + k][site];
+          if (REAL) {
+            v[k].re += m.re * q.x;
+            v[k].im += m.re * q.y;
+          } else {
+            v[k] = cfma(v[k], m, cmk<double>(q.x, q.y));
+          }
+        }
+      }
+    }
+    if (XPAY) {
+      cplx<double> xv[12];
+      SpinorIO<double>::load(a.x, ls, row, h, xv);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v[k] = cmk<double>(v[k].re + a.beta * xv[k].re, v[k].im + a.beta * xv[k].im);
+    }
+    SpinorIO<double>::store_stream(a.out, ls, row, h, v);
+    if (DOT) {
+      cplx<double> pv[12];
+      SpinorIO<double>::load(a.dotp, ls, row, h, pv);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) dot += pv[k].re * v[k].re + pv[k].im * v[k].im;
+    }
+  }
+  if (DOT) {
+    __shared__ double red[(SB * LSMAX + 63) / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dot += __shfl_down(dot, o, 64);
+    if (lane == 0) red[wv] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) t += red[i];
+      a.part[lb] = t;
+    }
+  }
+}
+
+template <int SB, int LSMAX>
+static void launch_hop_m5d_ls(const HopM5dArgs& a, bool dag, hipStream_t st)
+{
+  const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB));
+  const dim3 blk(SB * a.ls);
+  const bool xpay = a.x != nullptr, dot = a.dotp != nullptr;
+#define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5d<SB, LSMAX, D, X, R, DT>), grid, blk, 0, st, a)
+  if (dot) {
+    if (dag && xpay && a.real_mat) QDWF_L(true, true, true, true);
+    else if (dag && xpay) QDWF_L(true, true, false, true);
+    return;
+  }
+  if (a.real_mat) {
+    if (!dag && !xpay) QDWF_L(false, false, true, false);
+    if (!dag && xpay) QDWF_L(false, true, true, false);
+    if (dag && !xpay) QDWF_L(true, false, true, false);
+    if (dag && xpay) QDWF_L(true, true, true, false);
+  } else {
+    if (!dag && !xpay) QDWF_L(false, false, false, false);
+    if (!dag && xpay) QDWF_L(false, true, false, false);
+    if (dag && !xpay) QDWF_L(true, false, false, false);
+    if (dag && xpay) QDWF_L(true, true, false, false);
+  }
+#undef QDWF_L
+}
+
+bool hop_m5d_ok(int ls) { return ls >= 4 && ls <= 16 && (16 * ls) % 64 == 0; }
+int64_t hop_m5d_blocks(const LatGeo& g) { return (g.vh + 15) / 16; }
+
+bool launch_hop_m5d(const HopM5dArgs& a, bool dag, hipStream_t st)
+{
+  if (!hop_m5d_ok(a.ls) || !a.mat) return false;
+  if (a.dotp && !(dag && a.x)) return false;
+  if (a.ls <= 8) launch_hop_m5d_ls<16, 8>(a, dag, st);
+  else if (a.ls <= 12) launch_hop_m5d_ls<16, 12>(a, dag, st);
+  else launch_hop_m5d_ls<16, 16>(a, dag, st);
   return true;
 }
 
@@ -827,21 +997,23 @@ static void launch_cg_step_s_sb(const CgStepArgs<float>& a, hipStream_t st)
 {
   if (a.ls <= 8) launch_cg_step_s_ls<ST, SR, SX, SB, 8>(a, st);
   else if (a.ls <= 12) launch_cg_step_s_ls<ST, SR, SX, SB, 12>(a, st);
-  else launch_cg_step_s_ls<ST, SR, SX, SB, 16>(a, st);
+  else if (a.ls <= 16) launch_cg_step_s_ls<ST, SR, SX, SB, 16>(a, st);
+  else if constexpr (SB == 16) launch_cg_step_s_ls<ST, SR, SX, SB, 24>(a, st);
 }
 
-// 16 sites per block when asked for and Ls allows whole waves, else 32
+// 16 sites per block when asked for (or needed: Ls above 16) and Ls allows
+// whole waves, else 32
 template <class ST, class SR, class SX>
 static void launch_cg_step_s_st(const CgStepArgs<float>& a, hipStream_t st)
 {
-  if (a.sb == 16 && three_phase_ok(a.ls, 16)) launch_cg_step_s_sb<ST, SR, SX, 16>(a, st);
+  if ((a.sb == 16 || !three_phase_ok(a.ls, 32)) && three_phase_ok(a.ls, 16)) launch_cg_step_s_sb<ST, SR, SX, 16>(a, st);
   else launch_cg_step_s_sb<ST, SR, SX, 32>(a, st);
 }
 
 // storage: 0 all FP32; 1 p, t FP16, r and x FP32; 2 all FP16
 bool launch_cg_step_s(const CgStepArgs<float>& a, int storage, hipStream_t st)
 {
-  if (!three_phase_ok(a.ls, 32)) return false;
+  if (!three_phase_ok(a.ls, 32) && !three_phase_ok(a.ls, 16)) return false;
   if (storage == 2) launch_cg_step_s_st<Half16, Half16, Half16>(a, st);
   else if (storage == 1) launch_cg_step_s_st<Half16, float, float>(a, st);
   else launch_cg_step_s_st<float, float, float>(a, st);

@@ -78,11 +78,29 @@ def test_fp64_solve_matches_oracle(pair):
     inv.set_stop_rsd(1e-8)
 
 
+def test_low_precision_operators_match_oracle(pair):
+    """The operators of the mixed solvers at Ls 20 and 24: FP32 (2e-6) and
+    the 16-bit storage path (three-phase kernels with 24 rows, FP16
+    arithmetic in the hop phase) within its 3e-3 bound, and above 1e-5 so
+    that it is the 16-bit path that ran."""
+    geo, ls, inv, orc = pair
+    x = rand_field((geo.total_volume() // 2, ls, 12), 740)
+    f = q.FermionField5d(geo, ls, 1)
+    f.data = x
+    ref = orc.apply_half("hermop_sym2", x, 1)
+    inv.set_solver("mixed")
+    e32 = rel(q.multiply_hermop_sym2_fp32(None, f, inv).data, ref)
+    inv.set_solver("mixed16")
+    e16 = rel(q.multiply_hermop_sym2_fp32(None, f, inv).data, ref)
+    inv.set_solver("fp64")
+    assert e32 < 2e-6
+    assert 1e-5 < e16 < 3e-3
+
+
 @pytest.mark.parametrize("solver", ["mixed", "mixed16"])
 def test_mixed_solvers_reach_tolerance(pair, solver):
-    """The mixed solvers for Ls > 16 (FP32 arithmetic and storage: the
-    16-bit storage path needs Ls <= 16, so mixed16 reports 32-bit storage
-    here)."""
+    """The mixed solvers for Ls 20 and 24 on the three-phase kernels (the
+    24-row instances): FP32 storage, and FP16 storage with per-site scale."""
     geo, ls, inv, orc = pair
     orc.set_params(1e-12, 200, 300)
     src = q.FermionField4d(geo)
@@ -95,5 +113,5 @@ def test_mixed_solvers_reach_tolerance(pair, solver):
     inv.set_solver("fp64")
     inv.set_stop_rsd(1e-8)
     assert st["true_residual"] <= 1e-9
-    assert st["spinor_bits"] in (16, 32)
+    assert st["spinor_bits"] == (16 if solver == "mixed16" else 32)
     assert rel(sol.data, exact) < 1e-6
