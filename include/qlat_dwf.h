@@ -377,6 +377,18 @@ int qlat_dwf_inverter_set_gauge(qlat_dwf_inverter* inv, const double* gauge, int
 /* ------------------------------------------ device-resident variants
  * Same semantics as above; all field pointers are device memory. */
 int qlat_dwf_multiply_hermop_sym2_device(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
+
+/* qlat_dwf_multiply_hermop_sym2_device that allocates nothing: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its five FP64 work fields live in a static device
+ * array of 64 MiB per GPU (held until the process ends), so the lattice must
+ * satisfy 5 * 192 * Ls * (V/2 rounded up to 256) <= 64 MiB (e.g. 8^3 x 16 at
+ * Ls 16); larger ones are rejected with QLAT_DWF_EINVAL.  The inverter
+ * itself must have been created beforehand (creation allocates).  Calls on
+ * one GPU share the array: calls that overlap on one GPU (from several
+ * threads) race. */
+int qlat_dwf_multiply_hermop_sym2_device_noalloc(qlat_dwf_inverter* inv, double* out, const double* in, int eo);
 int qlat_dwf_invert_5d_device(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
 int qlat_dwf_invert_5d_with_guess_device(qlat_dwf_inverter* inv, double* out, const double* in, long* iter);
 int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter);

@@ -125,6 +125,8 @@ cdef extern from "qlat_dwf.h" nogil:
                                       double* grad, double* action, double* action_lo, long* iter)
     int qlat_dwf_inverter_set_gauge(qlat_dwf_inverter* inv, const double* gauge, int on_device)
     int qlat_dwf_multiply_hermop_sym2_device(qlat_dwf_inverter* inv, double* out, const double* inp, int eo)
+    int qlat_dwf_multiply_hermop_sym2_device_noalloc(qlat_dwf_inverter* inv, double* out, const double* inp,
+                                                     int eo)
     int qlat_dwf_invert_5d_device(qlat_dwf_inverter* inv, double* out, const double* inp, long* iter)
     int qlat_dwf_invert_5d_with_guess_device(qlat_dwf_inverter* inv, double* out, const double* inp, long* iter)
     int qlat_dwf_invert_prop_device(qlat_dwf_inverter* inv, double* sol, const double* src, long* iter)
@@ -329,6 +331,11 @@ def multiply_hermop_sym2_fp32(uintptr_t h, uintptr_t out, uintptr_t inp, int eo)
 
 def multiply_hermop_sym2_device(uintptr_t h, uintptr_t out, uintptr_t inp, int eo):
     _call3(qlat_dwf_multiply_hermop_sym2_device, "qlat_dwf_multiply_hermop_sym2_device", h, out, inp, eo)
+
+
+def multiply_hermop_sym2_device_noalloc(uintptr_t h, uintptr_t out, uintptr_t inp, int eo):
+    _call3(qlat_dwf_multiply_hermop_sym2_device_noalloc, "qlat_dwf_multiply_hermop_sym2_device_noalloc", h, out,
+           inp, eo)
 
 
 def deflate(uintptr_t h, uintptr_t out, uintptr_t inp, int eo):

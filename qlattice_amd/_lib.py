@@ -70,6 +70,7 @@ EXPORTED = [
     "qlat_dwf_find_max_eigen_value_hermop_sym2",
     "qlat_dwf_find_max_eigen_value_hermop_sym2_rng",
     "qlat_dwf_multiply_hermop_sym2_device",
+    "qlat_dwf_multiply_hermop_sym2_device_noalloc",
     "qlat_dwf_invert_5d_device",
     "qlat_dwf_invert_5d_with_guess_device",
     "qlat_dwf_invert_prop_device",

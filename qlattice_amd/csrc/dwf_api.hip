@@ -19,6 +19,7 @@
 #include <cmath>
 #include <complex>
 #include <memory>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -34,9 +35,21 @@ using qdwf::cplx;
 using qdwf::LatGeo;
 typedef std::complex<double> cd;
 
+// Scratch of the allocation-free entry point
+// (qlat_dwf_multiply_hermop_sym2_device_noalloc): the five FP64 half fields
+// of one Mpc^+ Mpc application (input and output in the device layout and
+// the operator's three temporaries).  Each GPU holds its own copy for the
+// life of the process; host code uses only the per-device address obtained
+// with hipGetSymbolAddress (g_scratch_ptr), never the array's name.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
 
 thread_local std::string g_last_error;
+// > 0 inside an allocation-free entry point: dev_alloc throws instead of
+// allocating (a guard against any path that would reach hipMalloc)
+thread_local int g_noalloc_depth = 0;
 
 struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -230,6 +243,12 @@ struct qlat_dwf_inverter {
   std::vector<double2*> col64;  // 3 per column: b, x, out_e_p
 
   qlat_dwf_solve_stats stats{};
+
+  // allocation-free calls: while noalloc is set, W() hands out only the
+  // slices of the static scratch array listed in w64_noalloc and every other
+  // buffer request throws instead of allocating
+  bool noalloc = false;
+  double2* w64_noalloc[W_COUNT64] = {};
 };
 
 namespace {
@@ -241,8 +260,36 @@ size_t field32_bytes(const Inv& I) { return (size_t)I.ls * 6 * I.geo.stride * si
 int64_t n64(const Inv& I) { return (int64_t)I.ls * 12 * I.geo.stride; }
 int64_t n32(const Inv& I) { return (int64_t)I.ls * 6 * I.geo.stride; }
 
+// Device address of `scratch` on each device, looked up once per device
+// (with that device current) when the first inverter is created on it.
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;
+
+void init_scratch(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  }
+  g_scratch_ptr[device] = (unsigned char*)p;
+}
+
+unsigned char* scratch_ptr(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= (int)g_scratch_ptr.size() || !g_scratch_ptr[device])
+    throw std::runtime_error("the allocation-free scratch of this device is not set up");
+  return g_scratch_ptr[device];
+}
+
 void* dev_alloc(size_t bytes)
 {
+  if (g_noalloc_depth > 0) throw std::logic_error("internal: device allocation in an allocation-free call");
   void* p = nullptr;
   const hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
@@ -254,6 +301,10 @@ void* dev_alloc(size_t bytes)
 
 double2* W(Inv& I, int id)
 {
+  if (I.noalloc) {
+    if (!I.w64_noalloc[id]) throw std::logic_error("internal: a work field outside the scratch in an allocation-free call");
+    return I.w64_noalloc[id];
+  }
   if (!I.w64[id]) {
     I.w64[id] = (double2*)dev_alloc(field64_bytes(I));
     QDWF_HIP(hipMemsetAsync(I.w64[id], 0, field64_bytes(I), I.stream));
@@ -262,6 +313,7 @@ double2* W(Inv& I, int id)
 }
 float4* V(Inv& I, int id)
 {
+  if (I.noalloc) throw std::logic_error("internal: an FP32 work field in an allocation-free call");
   if (!I.w32[id]) {
     I.w32[id] = (float4*)dev_alloc(field32_bytes(I));
     QDWF_HIP(hipMemsetAsync(I.w32[id], 0, field32_bytes(I), I.stream));
@@ -273,6 +325,7 @@ enum Multi32 { M_R, M_P, M_AP, M_X, M_T, M_A, M_B, M_Y, M_P2 };
 // FP32 stack buffer `id` with room for `ncol` fields.
 float4* MV(Inv& I, int id, int ncol)
 {
+  if (I.noalloc) throw std::logic_error("internal: a multi-RHS stack in an allocation-free call");
   if (ncol > I.mcap) {
     QDWF_HIP(hipStreamSynchronize(I.stream));
     for (auto*& p : I.m32)
@@ -327,6 +380,7 @@ size_t stage_size(const Inv& I);
 // enough for a 5D full field or a propagator, allocated on first use.
 double2* stage(Inv& I, int k, size_t bytes)
 {
+  if (I.noalloc) throw std::logic_error("internal: a staging buffer in an allocation-free call");
   if (bytes > stage_size(I)) throw ArgError("field larger than the staging buffer");
   if (!I.stage[k]) {
     I.stage_bytes = stage_size(I);
@@ -2353,6 +2407,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
                  "only cg_diagonal_mee == 2 is supported (dslash.h:483, :520)");
     I = new Inv;
     QDWF_HIP(hipGetDevice(&I->device));
+    init_scratch(I->device);
     QDWF_HIP(hipStreamCreate(&I->stream));
     for (int i = 0; i < 4; ++i) I->geo.n[i] = total_site[i];
     I->geo.xh = total_site[0] / 2;
@@ -3277,6 +3332,63 @@ int qlat_dwf_multiply_hermop_sym2_device(qlat_dwf_inverter* inv, double* out, co
     qdwf::launch_unpack5(b, (double2*)out, inv->geo, inv->ls, eo, 0, inv->stream);
     check_launch();
     sync(*inv);
+  });
+}
+
+namespace {
+
+// While alive: the inverter's work fields `ids` are consecutive slices of
+// this device's scratch array, and any other buffer request or device
+// allocation throws.
+struct NoAllocScope {
+  Inv& I;
+  NoAllocScope(Inv& inv, const int* ids, int n, size_t field_bytes) : I(inv)
+  {
+    unsigned char* base = scratch_ptr(I.device);
+    for (auto*& p : I.w64_noalloc) p = nullptr;
+    for (int k = 0; k < n; ++k) I.w64_noalloc[ids[k]] = (double2*)(base + (size_t)k * field_bytes);
+    I.noalloc = true;
+    ++g_noalloc_depth;
+  }
+  ~NoAllocScope()
+  {
+    I.noalloc = false;
+    for (auto*& p : I.w64_noalloc) p = nullptr;
+    --g_noalloc_depth;
+  }
+};
+
+}  // namespace
+
+int qlat_dwf_multiply_hermop_sym2_device_noalloc(qlat_dwf_inverter* inv, double* out, const double* in, int eo)
+{
+  return guarded([&] {
+    check_inv(inv);
+    check_eo(eo);
+    QDWF_REQUIRE(out && in, "null field pointer");
+    Inv& I = *inv;
+    // the device-layout input and output and the FP64 operator's temporaries
+    // (op_hermop -> op_mpc / op_mpcdag use W_OP_T1, W_OP_T2 and W_OP_Y)
+    const int ids[5] = {W_IN_E, W_OUT_E, W_OP_T1, W_OP_T2, W_OP_Y};
+    const size_t fb = (field64_bytes(I) + 255) / 256 * 256;
+    if (5 * fb > kNoAllocScratchBytes)
+      throw ArgError("qlat_dwf_multiply_hermop_sym2_device_noalloc: this lattice needs " + std::to_string(5 * fb) +
+                     " bytes of scratch (five FP64 half fields); the maximum is " +
+                     std::to_string(kNoAllocScratchBytes) +
+                     " bytes -- use qlat_dwf_multiply_hermop_sym2_device");
+    NoAllocScope scope(I, ids, 5, fb);
+    // the scratch keeps the previous call's contents: clear it, so that no
+    // field (pad entries of the device layout included) carries values left
+    // by an earlier call
+    QDWF_HIP(hipMemsetAsync(scratch_ptr(I.device), 0, 5 * fb, I.stream));
+    double2* a = W(I, W_IN_E);
+    double2* b = W(I, W_OUT_E);
+    qdwf::launch_pack5((const double2*)in, a, I.geo, I.ls, eo, 0, I.stream);
+    check_launch();
+    op_hermop<double>(I, b, a, eo);
+    qdwf::launch_unpack5(b, (double2*)out, I.geo, I.ls, eo, 0, I.stream);
+    check_launch();
+    sync(I);
   });
 }
 
