@@ -200,11 +200,11 @@ struct qlat_dwf_inverter {
   double2* l64 = nullptr;
   float2* l32 = nullptr;
   unsigned* l16 = nullptr;  // FP16 links for the FP16-arithmetic hop (h2_hop)
-  // FP16-storage solver: hop phase in FP16 arithmetic (QLAT_DWF_H2=1).  Off by
-  // default: 40% fewer VALU instructions, but 9% slower at 24^3x64 Ls12
-  // (1.281 against 1.175 ms for b = K H t over 4 columns), the kernel being
-  // bound by its memory requests, not its arithmetic
-  bool h2_hop = false;
+  // FP16-storage solver: hop phase in FP16 arithmetic (FP16 links, packed
+  // FP16 projection, v_dot2_f32_f16 SU(3) rows; QLAT_DWF_H2=0 selects FP32
+  // arithmetic).  Measured at 24^3x64 Ls12, b = K H t over 4 columns:
+  // 1.063 ms against 1.177 ms
+  int h2_hop = 1;
   cplx<double>* mat64 = nullptr;  // [MAT_COUNT][2][ls][ls]
   cplx<float>* mat32 = nullptr;
   cplx<double>* beta64 = nullptr;  // [2][ls]: ones, c~
@@ -1249,7 +1249,8 @@ struct ThreePhase {
     a.out = out;
     a.x = x;
     a.links = I.l32;
-    a.links16 = (half() && I.h2_hop) ? I.l16 : nullptr;
+    a.links16 = half() ? I.l16 : nullptr;
+    a.hop_mode = I.h2_hop;
     a.sb = sb;
     a.site_major = I.site_major;
     a.cg = cg;
@@ -1263,13 +1264,14 @@ struct ThreePhase {
     qdwf::launch_hop_m5s(a, dag, half(), I.stream);
     check_launch();
   }
+  int64_t hop_blocks() const { return qdwf::hop_m5s_blocks(I.geo, sb); }
   // ap = Mpc^+ Mpc p on parity eo given t = K p, as y = p - H K H t,
   // ap = y - K^+ H^+ K^+ H^+ y.  I.red[i] = Re <p, ap> per column and, in
   // the merged form, I.red[n + i] = Re <r, ap>, I.red[2n + i] = |ap|^2.
   void hermop_dot(const qdwf::RhsSet& cs, int eo)
   {
     const int nv = merged() ? 3 : 1;
-    const int64_t nblk = qdwf::hop_m5s_blocks(I.geo, sb);
+    const int64_t nblk = hop_blocks();
     if (nblk * cs.n * nv > I.part_cap) throw std::runtime_error("partial-sum buffer too small");
     hopm5(B, T, 3 - eo, false, MAT_K, 1.f, nullptr, 0.f, nullptr, cs);
     hopm5(Y, B, eo, false, -1, -1.f, P, 1.f, nullptr, cs);
@@ -2473,7 +2475,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_XDEFER")) I->x_defer = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_STEP_SB")) I->step_sb = std::atoi(e) == 16 ? 16 : 32;
     if (const char* e = std::getenv("QLAT_DWF_GRAPH")) I->graph_iters = std::max(0, std::min(64, std::atoi(e)));
-    if (const char* e = std::getenv("QLAT_DWF_H2")) I->h2_hop = std::atoi(e) != 0;
+    if (const char* e = std::getenv("QLAT_DWF_H2")) I->h2_hop = std::atoi(e) != 0 ? 1 : 0;
     {
       // partial sums per column: BLAS grids, one per block of the fused
       // kernels (32 sites) and of the three-phase kernels (16 sites, up to

@@ -124,6 +124,7 @@ struct HopM5Args {
   // storage, the hop phase runs in FP16 arithmetic (hop_term_h2); null keeps
   // FP32 arithmetic
   const void* links16;
+  int hop_mode;  // with links16: non-zero selects FP16 arithmetic in the hop phase
   const cplx<float>* mat;  // [2][ls][ls]
   bool real_mat;
   const void* dotp;  // optional (dag xpay form): part[i*nblk + block] = Re <dotp, out>

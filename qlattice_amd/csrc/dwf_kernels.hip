@@ -422,6 +422,28 @@ bool launch_hop_m5(const HopM5Args& a, bool dag, hipStream_t st)
 // CG dot products from the stored values.  SMAT = false skips the LDS round
 // trip (plain hop, out = alpha H in + beta x; selected by a null mats[0]).
 
+// A block's share of a staging copy global -> LDS: element f = tid + i *
+// blockDim for i < N (f < total).  The N loads are all issued before the
+// first LDS store (a loop with a run-time trip count would wait for each
+// load before issuing the next).  load(f) returns the value, store(f, v)
+// writes it; the caller synchronises.
+template <int N, class T, class Load, class Store>
+__device__ __forceinline__ void stage_copy(int total, Load&& load, Store&& store)
+{
+  T v[N];
+  const int bd = blockDim.x, t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int f = t + i * bd;
+    if (f < total) v[i] = load(f);
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int f = t + i * bd;
+    if (f < total) store(f, v[i]);
+  }
+}
+
 // The block's s-matrices (both chiralities, [2][ls][ls] on the device) into
 // LDS; every thread of the block takes part, the caller synchronises.
 template <int LSMAX>
@@ -481,11 +503,25 @@ __device__ __forceinline__ void store_qs(typename SpinorIO<ST>::slot_t* __restri
   else SpinorIO<ST>::store_stream(f, ls, s, h, v);
 }
 
-template <class ST, int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT, bool SMAT, bool H2>
-__global__ void __launch_bounds__(SB * LSMAX) __attribute__((amdgpu_waves_per_eu(3)))
+// HM: 0 FP32 arithmetic in the hop phase (8 neighbour spinors in flight per
+// thread, a register budget of 3 waves per SIMD); 1 FP16 arithmetic
+// (hop_term_h2) with 4 neighbour spinors in flight and 4 waves per SIMD.
+// Measured at 24^3x64 Ls12 over 4 columns (b = K H t): 1.177 ms and
+// 1.063 ms; FP16 arithmetic with 8 in flight needs 129 registers (3
+// waves per SIMD, 1.28 ms), or spills when held to 128.
+template <int HM>
+struct HopMode {
+  static constexpr bool h2 = HM != 0;
+  static constexpr int nb = HM ? 4 : 8;
+  static constexpr int wpe = HM ? 4 : 3;
+};
+
+template <class ST, int SB, int LSMAX, bool DAG, bool XPAY, bool REAL, bool DOT, bool SMAT, int HM>
+__global__ void __launch_bounds__(SB * LSMAX) __attribute__((amdgpu_waves_per_eu(HopMode<HM>::wpe)))
 k_hop_m5s(HopM5Args a)
 {
   typedef typename SpinorIO<ST>::slot_t slot_t;
+  constexpr bool H2 = HopMode<HM>::h2;
   static_assert(!H2 || std::is_same<ST, Half16>::value, "FP16 arithmetic needs FP16 storage");
   // links of the block's sites: FP32 complex, or (H2) the FP16 forms
   // la = (re, -im), lb = (im, re) of hop_term_h2
@@ -502,16 +538,21 @@ k_hop_m5s(HopM5Args a)
   if (a.cg && a.cg->mode[ci] != 0) return;  // frozen column (the whole block)
   const int64_t h0 = (int64_t)lb * SB;
   const int pslot = (a.eo_out == 2) ? 0 : 1;
+  // 72 SB link elements over SB ls >= 6 SB threads: at most 12 each
   if constexpr (H2) {
     const unsigned* __restrict__ l16 = (const unsigned*)a.links16 + lidx(a.geo.stride, pslot, 0, h0);
-    for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) {
-      const h16x2 u = __builtin_bit_cast(h16x2, l16[(i / SB) * 64 + (i % SB)]);
-      sla[i / SB][i % SB] = (h16x2){u.x, -u.y};
-      slb[i / SB][i % SB] = (h16x2){u.y, u.x};
-    }
+    stage_copy<12, unsigned>(
+        72 * SB, [&](int i) { return l16[(i / SB) * 64 + (i % SB)]; },
+        [&](int i, unsigned w) {
+          const h16x2 u = __builtin_bit_cast(h16x2, w);
+          sla[i / SB][i % SB] = (h16x2){u.x, -u.y};
+          slb[i / SB][i % SB] = (h16x2){u.y, u.x};
+        });
   } else {
     const float2* __restrict__ links = (const float2*)a.links + lidx(a.geo.stride, pslot, 0, h0);
-    for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) sl[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
+    stage_copy<12, float2>(
+        72 * SB, [&](int i) { return links[(i / SB) * 64 + (i % SB)]; },
+        [&](int i, float2 w) { sl[i / SB][i % SB] = w; });
   }
   // neighbour indices once per site (row 0), shared by the Ls rows
   __shared__ int nbs[8][SB];
@@ -541,7 +582,7 @@ k_hop_m5s(HopM5Args a)
     constexpr int SB_ = -SF;
     // neighbour spinors are loaded in batches of NB directions before any
     // of them is used, so that each wave keeps several gathers in flight
-    constexpr int NB = std::is_same<ST, Half16>::value ? 8 : 4;
+    constexpr int NB = std::is_same<ST, Half16>::value ? HopMode<HM>::nb : 4;
     typename SpinorIO<ST>::Raw raw[NB];
 #define QDWF_LD(J, D) SpinorIO<ST>::load_raw(in, ls, row, nb[D], raw[J]);
 #define QDWF_CP(J, D, MU, SG)                                       \
@@ -654,7 +695,7 @@ k_hop_m5s(HopM5Args a)
   }
 }
 
-template <class ST, int SB, int LSMAX, bool H2>
+template <class ST, int SB, int LSMAX, int H2>
 static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
 {
   const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB * a.cols.n));
@@ -696,7 +737,7 @@ bool three_phase_ok(int ls, int sb)
 }
 int64_t hop_m5s_blocks(const LatGeo& g, int sb) { return (g.vh + sb - 1) / sb; }
 
-template <class ST, int SB, bool H2>
+template <class ST, int SB, int H2>
 static void launch_hop_m5s_sb(const HopM5Args& a, bool dag, hipStream_t st)
 {
   if (a.ls <= 8) launch_hop_m5s_ls<ST, SB, 8, H2>(a, dag, st);
@@ -708,15 +749,16 @@ static void launch_hop_m5s_sb(const HopM5Args& a, bool dag, hipStream_t st)
 bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
 {
   if (!three_phase_ok(a.ls, a.sb)) return false;
-  if (half && a.links16) {
-    if (a.sb == 16) launch_hop_m5s_sb<Half16, 16, true>(a, dag, st);
-    else launch_hop_m5s_sb<Half16, 32, true>(a, dag, st);
+  const bool h2 = half && a.links16 && a.hop_mode != 0;
+  if (h2) {
+    if (a.sb == 16) launch_hop_m5s_sb<Half16, 16, 1>(a, dag, st);
+    else launch_hop_m5s_sb<Half16, 32, 1>(a, dag, st);
   } else if (half) {
-    if (a.sb == 16) launch_hop_m5s_sb<Half16, 16, false>(a, dag, st);
-    else launch_hop_m5s_sb<Half16, 32, false>(a, dag, st);
+    if (a.sb == 16) launch_hop_m5s_sb<Half16, 16, 0>(a, dag, st);
+    else launch_hop_m5s_sb<Half16, 32, 0>(a, dag, st);
   } else {
-    if (a.sb == 16) launch_hop_m5s_sb<float, 16, false>(a, dag, st);
-    else launch_hop_m5s_sb<float, 32, false>(a, dag, st);
+    if (a.sb == 16) launch_hop_m5s_sb<float, 16, 0>(a, dag, st);
+    else launch_hop_m5s_sb<float, 32, 0>(a, dag, st);
   }
   return true;
 }

@@ -44,13 +44,17 @@ def test_three_phase_fp32_hermop_matches_oracle(monkeypatch, ls, omega):
         assert rel(got.data, orc.apply_half("hermop_sym2", x, eo)) < 2e-6
 
 
+@pytest.mark.parametrize("h2", ["1", "0"], ids=["fp16_arith", "fp32_arith"])
 @pytest.mark.parametrize("ls,omega", [(8, None), (12, "z"), (16, None)])
-def test_fp16_storage_hermop_matches_oracle(ls, omega):
+def test_fp16_storage_hermop_matches_oracle(monkeypatch, ls, omega, h2):
     """FP16 storage with a per-site scale: the input and every intermediate
     field are rounded to 11 significant bits relative to the site maximum,
     so the operator agrees with FP64 to a few 1e-4 (and is visibly not
-    FP32: the error is far above FP32 rounding)."""
-    print(f"\n[gpu] FP16-storage hermop Ls{ls}", flush=True)
+    FP32: the error is far above FP32 rounding).  Both hop arithmetics:
+    FP16 links with FP16 products accumulated in FP32 (the default), and
+    FP32 links and arithmetic (QLAT_DWF_H2=0)."""
+    monkeypatch.setenv("QLAT_DWF_H2", h2)
+    print(f"\n[gpu] FP16-storage hermop Ls{ls} H2={h2}", flush=True)
     site = (4, 4, 4, 8)
     U = rand_gauge(site, 0.4, 41)
     geo, inv, orc = make(site, U, mass=0.05, ls=ls, omega=ZMOBIUS_OMEGA if omega else None)
