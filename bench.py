@@ -5,9 +5,10 @@ U = exp(i 0.3 H) (seeded, antiperiodic in time), Moebius Ls = 12, M5 = 1.8,
 b = 1.5, c = 0.5, mass = 0.01; a 12-column point-source propagator solved
 to a 1e-8 FP64 true residual of the preconditioned odd system.  One step is
 one full propagator solve with the mixed-precision multi-RHS solver (the 12
-columns as one block; FP32 arithmetic, Krylov vectors stored in FP16 with a
-per-site scale by default, FP64 reliable updates); the FP64 solver is timed
-once on the same inputs.
+columns in autotuned multi-RHS blocks; Krylov vectors stored in FP16 with a
+per-site scale, FP16 links and FP16 products with FP32 sums in the hop
+phase, FP32 s-matrix and CG arithmetic, FP64 reliable updates by default);
+the FP64 solver is timed once on the same inputs.
 Inputs (gauge field, source propagator) are resident in device memory
 before the timed region; solutions stay in device memory.
 
@@ -29,6 +30,9 @@ PEAK_HBM_GBS = 8000.0
 
 CONFIGS = {
     "B": dict(site=(24, 24, 24, 64), ls=12, sigma=0.3, mass=0.01, m5=1.8, scale=2.0, seed=2024),
+    # the production action's fifth dimension (Ls 24, mobius_scale 4, as in the RBC/UKQCD 24D
+    # ensembles) on the same lattice: kernel timings and profiles (not a bench line)
+    "B24": dict(site=(24, 24, 24, 64), ls=24, sigma=0.3, mass=0.01, m5=1.8, scale=4.0, seed=2024),
     # small configuration for quick local runs (not a bench line)
     "S": dict(site=(8, 8, 8, 16), ls=12, sigma=0.3, mass=0.01, m5=1.8, scale=2.0, seed=2024),
     # batched multi-RHS: 16 Gaussian sources x 4 masses (its own JSON line)
@@ -61,16 +65,16 @@ def hop_flops(site, ls):
     return 1320 * (V // 2) * ls
 
 
-def hop_m5s_bytes(site, ls, ncol, site_bytes):
+def hop_m5s_bytes(site, ls, ncol, site_bytes, link_real_bytes):
     """Algorithmic bytes of one multi-RHS hop + s-matrix launch over ncol
     columns: read and write every column's 5D half field once (site_bytes
     per 5D site and field: 52 = 48 FP16 data + 4 scale, or 96 FP32), read
-    the 8 FP32 links of every output site once."""
+    the 8 links (18 reals each, FP16 or FP32) of every output site once."""
     V = 1
     for n in site:
         V *= n
     N = V // 2 * ls
-    return ncol * N * site_bytes * 2 + V * 4 * 18 * 4
+    return ncol * N * site_bytes * 2 + V * 4 * 18 * link_real_bytes
 
 
 def read_traffic(kernel_tag):
@@ -164,20 +168,24 @@ def bench_config_c(args):
     log(f"mixed16 batch: {t16:.2f} s, tile {st16['rhs_tile']}, iterations per mass "
         f"{[int(np.mean(it16[[j for j, (mi, _) in enumerate(pairs) if mi == m]])) for m in range(len(invs))]}, "
         f"true residual {st16['true_residual']:.2e}")
-    x16 = [x.clone() for x in sols]
-    t64, it64, st64 = run("fp64")
-    diff = max((torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item() for a, b in zip(x16, sols))
-    log(f"fp64 one by one: {t64:.2f} s, max rel diff mixed16 vs fp64 {diff:.2e}")
+    t64, diff = None, None
+    if not args.skip_fp64:
+        x16 = [x.clone() for x in sols]
+        t64, it64, st64 = run("fp64")
+        diff = max((torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item() for a, b in zip(x16, sols))
+        log(f"fp64 one by one: {t64:.2f} s, max rel diff mixed16 vs fp64 {diff:.2e}")
     line = {
         "metric": "batched multi-RHS time-to-1e-8, 16^3x32 Ls8, 16 sources x 4 masses",
-        "value": t16, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp32",
+        "value": t16, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "fp16 hop products with fp32 sums, fp32 CG, fp64 reliable updates" if st16["hop_arith_bits"] == 16
+        else "fp32",
         "data": "synthetic: seeded exp(i*0.3*H) links, antiperiodic t, Gaussian 5D sources",
         "config": {"workload": "64 systems (16 sources x masses 0.005, 0.01, 0.03, 0.1) in one batched call, "
                                "Moebius b=1.5 c=0.5, M5=1.8, Ls=8, stop 1e-8 (FP64 true residual)",
                    "rhs_tile": st16["rhs_tile"], "sites_per_block": st16["sites_per_block"]},
         "iters_per_mass": {str(cfg["masses"][m]): int(np.mean(it16[[j for j, (mi, _) in enumerate(pairs) if mi == m]]))
                            for m in range(len(invs))},
-        "fp64_seconds": t64, "speedup_vs_fp64": t64 / t16, "max_rel_diff_vs_fp64": diff,
+        "fp64_seconds": t64, "speedup_vs_fp64": None if t64 is None else t64 / t16, "max_rel_diff_vs_fp64": diff,
         "true_residual": st16["true_residual"], "true_residual_5d": st16["true_residual_5d"],
     }
     print(json.dumps(line), flush=True)
@@ -208,37 +216,45 @@ def bench_config_e(args):
     inv.set_stop_rsd(1e-8)
     inv.set_max_num_iter(1000)
     del U, gf
-    src_h = np.zeros((V, 12, 12), dtype=np.complex128)
-    src_h[0] = np.eye(12)
-    src = torch.from_numpy(src_h).cuda()
-    del src_h
-    sol = torch.zeros_like(src)
     L = _lib.load()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    L.invert_prop(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src), device=True)
-    torch.cuda.synchronize()
-    t_prop = time.perf_counter() - t0
-    st = inv.stats()
-    log(f"mixed16 propagator: {t_prop:.2f} s, iters {st['total_iter']}, reliable {st['reliable_updates']}, "
-        f"true residual {st['true_residual']:.2e}, tile {st['rhs_tile']}")
-    del src, sol
-    torch.cuda.empty_cache()
+    t_prop, st = None, None
+    if not args.force_only:
+        src_h = np.zeros((V, 12, 12), dtype=np.complex128)
+        src_h[0] = np.eye(12)
+        src = torch.from_numpy(src_h).cuda()
+        del src_h
+        sol = torch.zeros_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.invert_prop(inv.cdata, _lib.dev_ptr(sol), _lib.dev_ptr(src), device=True)
+        torch.cuda.synchronize()
+        t_prop = time.perf_counter() - t0
+        st = inv.stats()
+        log(f"mixed16 propagator: {t_prop:.2f} s, iters {st['total_iter']}, reliable {st['reliable_updates']}, "
+            f"true residual {st['true_residual']:.2e}, tile {st['rhs_tile']}")
+        del src, sol
+        torch.cuda.empty_cache()
     rng = np.random.default_rng(cfg["seed"] + 2)
     phi = torch.from_numpy(rng.standard_normal((V // 2, ls, 12)) + 1j * rng.standard_normal((V // 2, ls, 12))).cuda()
     force = torch.zeros((V, 4, 3, 3), dtype=torch.complex128, device="cuda")
     inv.set_solver("fp64")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    S, S_lo, itf = L.fermion_force(inv.cdata, _lib.dev_ptr(phi), 1e-10, _lib.dev_ptr(force), 0, device=True)
+    S, S_lo, itf = L.fermion_force(inv.cdata, _lib.dev_ptr(phi), args.force_stop_rsd, _lib.dev_ptr(force), 0,
+                                   device=True)
     torch.cuda.synchronize()
     t_force = time.perf_counter() - t0
     herm = (force + force.conj().transpose(-1, -2)).abs().max().item()
     trace = force.diagonal(dim1=-2, dim2=-1).sum(-1).abs().max().item()
     log(f"force: {t_force:.2f} s, CG iters {itf}, S {S:.6e}, anti-hermiticity {herm:.1e}, trace {trace:.1e}")
+    if args.force_only:
+        print(json.dumps({"force": {"seconds": t_force, "cg_iters": itf, "action": S}}), flush=True)
+        return
     line = {
         "metric": "32^3x64 Ls16 single-GPU share: propagator time-to-1e-8 and force time",
-        "value": t_prop, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "fp32",
+        "value": t_prop, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "fp16 hop products with fp32 sums, fp32 CG, fp64 reliable updates" if st["hop_arith_bits"] == 16
+        else "fp32",
         "data": "synthetic: seeded exp(i*0.25*H) links, antiperiodic t, point sources, Gaussian pseudofermion",
         "config": {"workload": "12-column point-source propagator, 32^3x64, Ls=16, Moebius b=1.5 c=0.5, M5=1.8, "
                                "mass=0.004, mixed16 multi-RHS, stop 1e-8; then one fermion force (FP64 CG 1e-10)",
@@ -265,6 +281,9 @@ def main():
                     help="mixed solvers: residual drop between FP64 reliable updates")
     ap.add_argument("--cpu-hermops", type=int, default=3, help="oracle hermop applications (0 = skip)")
     ap.add_argument("--kernel-iters", type=int, default=50)
+    ap.add_argument("--force-only", action="store_true",
+                    help="config E: one fermion-force evaluation without the propagator (for profiler runs)")
+    ap.add_argument("--force-stop-rsd", type=float, default=1e-10, help="config E: stop_rsd of the force's CG")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the kernels and stop (for profiler runs)")
     ap.add_argument("--profile-column", action="store_true",
@@ -312,12 +331,13 @@ def main():
     hm5s_ms = q.bench_kernel(inv, 5, prec_tp, max(1, args.kernel_iters // 5))
     tile = inv.stats()["rhs_tile"]  # the solver's tile (autotuned unless QLAT_DWF_RHS_TILE is set)
     sb = inv.stats()["sites_per_block"]
+    arith = inv.stats()["hop_arith_bits"]  # 16: FP16 links and FP16 dot products in the hop phase
     step_ms = q.bench_kernel(inv, 6, prec_tp, max(1, args.kernel_iters // 5))
     hermtp_ms = q.bench_kernel(inv, 7, prec_tp, max(1, args.kernel_iters // 10))
     site_bytes = 52 if prec_tp == 16 else 96
-    bm5s = hop_m5s_bytes(site, ls, tile, site_bytes)
+    bm5s = hop_m5s_bytes(site, ls, tile, site_bytes, 2 if arith == 16 else 4)
     gbs_m5s = bm5s / (hm5s_ms * 1e-3) / 1e9
-    log(f"three-phase ({prec_tp}-bit storage, {tile} columns, {sb} sites/block): hop+s-matrix {hm5s_ms:.3f} ms "
+    log(f"three-phase ({prec_tp}-bit storage, FP{arith} hop, {tile} columns, {sb} sites/block): hop+s-matrix {hm5s_ms:.3f} ms "
         f"{gbs_m5s:.0f} GB/s | cg step {step_ms:.3f} ms | hermop+dot {hermtp_ms:.3f} ms "
         f"({hermtp_ms / tile:.3f} ms per column)")
     b32 = hop_bytes(site, ls, 4)
@@ -424,7 +444,7 @@ def main():
         cpu = cpu_baseline(cfg, U, iters_ref, args.cpu_hermops)
 
     N = V // 2 * ls
-    tr = read_traffic(f"hop_m5s_{'fp16' if prec_tp == 16 else 'fp32'}_{args.config}_tile{tile}_sb{sb}")
+    tr = read_traffic(f"hop_m5s_{'fp16' if prec_tp == 16 else 'fp32'}_arith{arith}_{args.config}_tile{tile}_sb{sb}")
     traffic_bytes = None if tr is None else tr["hbm_bytes_per_launch"]
     line = {
         "metric": METRIC,
@@ -437,15 +457,19 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp16 hop products with fp32 sums, fp32 CG, fp64 reliable updates" if arith == 16 else "fp32",
         "data": "synthetic: seeded exp(i*0.3*H) SU(3) links, antiperiodic t, point sources",
         "config": {
-            "workload": f"12-column point-source propagator {site[0]}^3x{site[3]} Ls={ls} Moebius b=1.5 c=0.5 "
-                        f"M5=1.8 mass=0.01 to a 1e-8 FP64 true residual, columns solved in multi-RHS "
-                        f"blocks of {tile}, FP32 arithmetic, {prec_tp}-bit Krylov storage"
+            "workload": f"12-column point-source propagator {site[0]}^3x{site[3]} Ls={ls} Moebius "
+                        f"b={0.5 * cfg['scale'] + 0.5:g} c={0.5 * cfg['scale'] - 0.5:g} "
+                        f"M5={cfg['m5']:g} mass={cfg['mass']:g} to a 1e-8 FP64 true residual, columns solved in multi-RHS "
+                        f"blocks of {tile}, "
+                        + ("FP16 links and hop products with FP32 sums, FP32 s-matrix and CG, " if arith == 16
+                           else "FP32 arithmetic, ")
+                        + f"{prec_tp}-bit Krylov storage"
                         + (" with per-site scale" if prec_tp == 16 else "") + ", FP64 reliable updates",
             "lattice": list(site), "ls": ls, "columns": ncol, "stop_rsd": 1e-8, "rhs_tile": tile,
-            "solver": args.solver,
+            "solver": args.solver, "hop_arith_bits": arith,
         },
         "mixed": {"seconds": t_mixed, "iters": st_mixed["total_iter"],
                   "reliable_updates": st_mixed["reliable_updates"],
@@ -463,11 +487,13 @@ def main():
             "m5_fp32_ms": m5_32_ms, "hermop_fp32_ms": herm32_ms, "hermop_fp64_ms": herm64_ms,
             "hermop_ref_flops_convention": 5500 * ls * V,
             "three_phase": {"storage_bits": prec_tp, "columns": tile, "sites_per_block": sb, "hop_m5_ms": hm5s_ms,
-                            "hop_m5_gbps": gbs_m5s, "cg_step_ms": step_ms, "hermop_dot_ms": hermtp_ms},
+                            "hop_m5_gbps": gbs_m5s, "hop_arith_bits": arith, "cg_step_ms": step_ms,
+                            "hermop_dot_ms": hermtp_ms},
         },
         "roofline": {
-            "kernel": f"k_hop_m5s<{'Half16' if prec_tp == 16 else 'float'}, {sb}, {ls}> (hop + s-matrix, "
-                      f"{tile} columns per launch, {prec_tp}-bit spinor storage, FP32 links)",
+            "kernel": f"k_hop_m5s<{'Half16' if prec_tp == 16 else 'float'}, {sb}, {ls}, HM={1 if arith == 16 else 0}> "
+                      f"(hop + s-matrix, {tile} columns per launch, {prec_tp}-bit spinor storage, "
+                      f"FP{arith} links and hop arithmetic, FP32 s-matrix)",
             "bound": "hbm",
             "achieved": gbs_m5s,
             "peak": PEAK_HBM_GBS,

@@ -87,6 +87,10 @@ typedef struct {
                               5D system, the is_checking_invert check of
                               invert_with_cg (dslash.h:522-528); 0 when the check
                               is off (qlat_dwf_set_checking_invert)              */
+  int hop_arith_bits;     /* arithmetic of the hop phase of its low-precision
+                             kernels: 16 (FP16 links, packed FP16 projection,
+                             FP16 dot products into FP32 sums) or 32; 0 for the
+                             FP64 solver                                        */
 } qlat_dwf_solve_stats;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -279,15 +283,19 @@ int qlat_dwf_free_invert(const int total_site[4], double mass, double m5, const 
  * sites g < n_sites and m < n_per_site, u the m-th rand_gen draw of
  * rs.newtype(g), where rs is described by the SHA-256 midstate of its
  * message (midstate, num_bytes: a multiple of 64) and its index (index0).
- * One SHA-256 block per four draws; out is host memory. */
+ * One SHA-256 block per four draws; out is host memory.  Each call
+ * allocates and frees a device buffer of n_sites * n_per_site doubles. */
 int qlat_dwf_rng_fill_uniform(const uint32_t midstate[8], uint64_t num_bytes, uint64_t index0, int64_t n_sites,
                               int n_per_site, double upper, double lower, double* out);
 
 /* set_g_rand (qlat core.h:1619-1640, g_rand_gen rng-state.cpp:222-258) on
  * the GPU with the same state description: out[g * n_per_site + m] is the
- * m-th g_rand_gen(center, sigma) of rs.newtype(g), one thread per site.  The
- * logarithm of the polar method is the device's, so values can differ from
- * the host's in the last bit. */
+ * m-th g_rand_gen(center, sigma) of rs.newtype(g), one thread per site
+ * drawing that site's n_per_site values in sequence (the reference's
+ * per-site state: cached words and spare Gaussian).  The logarithm of the
+ * polar method is the device's, so values can differ from the host's in the
+ * last bit.  Each call allocates and frees a device buffer of
+ * n_sites * n_per_site doubles. */
 int qlat_dwf_rng_fill_gaussian(const uint32_t midstate[8], uint64_t num_bytes, uint64_t index0, int64_t n_sites,
                                int n_per_site, double center, double sigma, double* out);
 

@@ -45,6 +45,7 @@ cdef extern from "qlat_dwf.h" nogil:
         int rhs_tile
         int sites_per_block
         double true_residual_5d
+        int hop_arith_bits
 
     ctypedef struct qlat_dwf_source:
         int kind
@@ -145,7 +146,7 @@ SOLVER_MIXED_FP16 = 2
 SRC_POINT, SRC_WALL, SRC_MOM, SRC_U1 = 0, 1, 2, 3
 
 STATS_FIELDS = ("total_iter", "cycles", "reliable_updates", "hermop_fp64", "hermop_low", "true_residual",
-                "seconds", "spinor_bits", "rhs_tile", "sites_per_block", "true_residual_5d")
+                "seconds", "spinor_bits", "rhs_tile", "sites_per_block", "true_residual_5d", "hop_arith_bits")
 
 
 cdef int _check(int rc, str what) except -1:
@@ -229,7 +230,8 @@ def inverter_get_stats(uintptr_t h):
     return {"total_iter": st.total_iter, "cycles": st.cycles, "reliable_updates": st.reliable_updates,
             "hermop_fp64": st.hermop_fp64, "hermop_low": st.hermop_low, "true_residual": st.true_residual,
             "seconds": st.seconds, "spinor_bits": st.spinor_bits, "rhs_tile": st.rhs_tile,
-            "sites_per_block": st.sites_per_block, "true_residual_5d": st.true_residual_5d}
+            "sites_per_block": st.sites_per_block, "true_residual_5d": st.true_residual_5d,
+            "hop_arith_bits": st.hop_arith_bits}
 
 
 def set_checking_invert(bint on):

@@ -1152,6 +1152,11 @@ struct ThreePhase {
     for (void* b : Pbuf) QDWF_HIP(hipMemsetAsync(b, 0, (size_t)fh * slot_bytes() * n, I.stream));
   }
   void* other() const { return Pbuf[cur ^ 1]; }
+  void flip()
+  {
+    cur ^= 1;
+    P = Pbuf[cur];
+  }
   qdwf::RhsSet colset(const std::vector<int>& act) const
   {
     qdwf::RhsSet cs;
@@ -1204,16 +1209,38 @@ struct ThreePhase {
     a.t = T;
     if (!qdwf::launch_cg_step_s(a, storage, I.stream)) throw ArgError("the three-phase CG step needs an even ls in [6, 16]");
     check_launch();
-    if (flip) {
-      cur ^= 1;
-      P = Pbuf[cur];
-    }
+    if (flip) this->flip();
+  }
+  // p = r + beta q into the current p buffer, q the other buffer's p, and
+  // t = K p, for columns whose last direction is in the other buffer (x and
+  // r unchanged)
+  void restart_from_other(const std::vector<int>& cols, const double* beta)
+  {
+    qdwf::CgStepArgs<float> a{};
+    a.geo = I.geo;
+    a.ls = I.ls;
+    for (int i = 0; i < qdwf::kMaxRhs; ++i) a.beta[i] = i < (int)cols.size() ? beta[i] : 0.0;
+    a.p_out = P;
+    a.cols = colset(cols);
+    a.xr_fstride = fr;
+    a.x_fstride = fx;
+    a.sb = I.step_sb;
+    a.mat = nullptr;
+    a.real_mat = mats(a.cols, MAT_K, a.mats);
+    a.x = nullptr;
+    a.p = other();
+    a.r = R;
+    a.t = T;
+    if (!qdwf::launch_cg_step_s(a, storage, I.stream)) throw ArgError("the three-phase CG step needs an even ls in [6, 16]");
+    check_launch();
   }
   // the merged step with its scalars and column modes read from the device
-  // CG state (graph-captured iterations)
+  // CG state (graph-captured iterations); the new p goes to the other
+  // buffer, which holds the previous p for the folded x update
   void step_dev(const std::vector<int>& cols)
   {
     qdwf::CgStepArgs<float> a{};
+    a.p_out = other();
     a.geo = I.geo;
     a.ls = I.ls;
     a.cg = cg;
@@ -1230,6 +1257,7 @@ struct ThreePhase {
     a.t = T;
     if (!qdwf::launch_cg_step_s(a, storage, I.stream)) throw ArgError("the three-phase CG step needs an even ls in [6, 16]");
     check_launch();
+    flip();
   }
   void hopm5(void* out, const void* in, int eo_out, bool dag, int mat, float alpha, const void* x, float beta,
              const void* dotp, const qdwf::RhsSet& cs, bool dotr = false)
@@ -1326,6 +1354,11 @@ int storage_of(const Inv& I)
   return I.fp16_mode;
 }
 
+// Arithmetic of the hop phase of the three-phase kernels in a storage mode
+// (the dispatch of launch_hop_m5s): FP16 for FP16 storage with the FP16
+// links allocated and h2_hop on, FP32 otherwise.
+int hop_arith_bits(const Inv& I, int storage) { return (storage && I.l16 && I.h2_hop) ? 16 : 32; }
+
 void read_red(Inv& I, int n, double* out)
 {
   QDWF_HIP(hipMemcpyAsync(I.red_host, I.red, n * sizeof(double), hipMemcpyDeviceToHost, I.stream));
@@ -1414,7 +1447,7 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
     qdwf::CgState& hs = *I.cg_host;
     hipGraphExec_t exec = nullptr;
     std::vector<int> cap_cols;
-    int cap_k = 0;
+    int cap_k = 0, cap_cur = 0;
     auto drop = [&] {
       if (exec) (void)hipGraphExecDestroy(exec);
       exec = nullptr;
@@ -1425,14 +1458,19 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
         hs = qdwf::CgState{};
         hs.target = target;
         hs.d2 = d2;
+        hs.x_defer = I.x_defer;
         for (int i = 0; i < n; ++i) {
           hs.rr[i] = rr[active[i]];
           hs.maxrr[i] = maxrr[active[i]];
+          hs.pend[i] = pend[active[i]];
         }
         QDWF_HIP(hipMemcpyAsync(I.cg_dev, &hs, sizeof hs, hipMemcpyHostToDevice, I.stream));
         const int k = (int)std::min<long>(I.graph_iters, cap - it);
-        if (!exec || cap_cols != active || cap_k != k) {
+        // the captured kernels name the p buffers: a graph is replayed
+        // only from the buffer parity it was captured at
+        if (!exec || cap_cols != active || cap_k != k || cap_cur != tp.cur) {
           drop();
+          const int cur0 = tp.cur;
           const qdwf::RhsSet cs = tp.colset(active);
           const qlat_dwf_solve_stats saved = I.stats;
           QDWF_HIP(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
@@ -1459,13 +1497,22 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
           I.stats = saved;
           cap_cols = active;
           cap_k = k;
+          cap_cur = cur0;
+        } else {
+          // the capture above advanced the host's buffer parity by k steps;
+          // a replay advances it the same way
+          for (int j = 0; j < k; ++j) tp.flip();
         }
         QDWF_HIP(hipGraphLaunch(exec, I.stream));
         QDWF_HIP(hipMemcpyAsync(&hs, I.cg_dev, sizeof hs, hipMemcpyDeviceToHost, I.stream));
         sync(I);
         it += k;
-        std::vector<int> keep, ru;
-        std::vector<double> ru_beta;
+        // restarted columns whose last p is in the current p buffer (0) or
+        // in the other one (1): a column frozen at step jf of the replay
+        // kept its p in the buffer of that step while the k - jf later
+        // steps flipped the buffers
+        std::vector<int> keep, ru[2];
+        std::vector<double> ru_beta[2];
         for (int i = 0; i < n; ++i) {
           const int c = active[i];
           iters[c] += hs.iters[i];
@@ -1475,10 +1522,12 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
                                              : "CG breakdown: non-finite residual (FP32 three-phase, multi)");
           if (hs.mode[i] == 0) {
             rr[c] = hs.rr[i];
+            pend[c] = hs.pend[i];
             keep.push_back(c);
             continue;
           }
-          // reliable update (x += alpha p was done by the frozen step)
+          pend[c] = 0.0;
+          // reliable update (x += alpha_q q + alpha p was done by the frozen step)
           tp.add_x(x64[c], c, bn[c]);
           tp.zero_x(c, 1);
           const double rr_true = true_residual(c);
@@ -1489,13 +1538,17 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
             continue;  // column retired
           }
           tp.set_r(r64, c, 1.0 / bn[c]);
-          ru.push_back(c);
-          ru_beta.push_back(rr_true / hs.rr[i]);  // hs.rr: |r|^2 before the frozen iteration
+          const int jf = hs.iters[i] - 1;  // the step of this replay that froze the column
+          const int b = (k - jf) & 1;
+          ru[b].push_back(c);
+          ru_beta[b].push_back(rr_true / hs.rr[i]);  // hs.rr: |r|^2 before the frozen iteration
           rr[c] = maxrr[c] = rr_true;
           keep.push_back(c);
         }
-        // p = r + beta p, t = K p for the updated columns (x, r unchanged)
-        if (!ru.empty()) tp.step(ru, zero.data(), ru_beta.data(), true, zero.data());
+        // p = r + beta p into the current buffer, t = K p for the updated
+        // columns (x, r unchanged)
+        if (!ru[0].empty()) tp.step(ru[0], zero.data(), ru_beta[0].data(), true, zero.data());
+        if (!ru[1].empty()) tp.restart_from_other(ru[1], ru_beta[1].data());
         active.swap(keep);
       }
     } catch (...) {
@@ -1582,6 +1635,7 @@ long cg_mixed_multi_s(Inv& I, int ncol, double2* const* b64, double2* const* x64
   if (col_iters)
     for (int c = 0; c < ncol; ++c) col_iters[c] = iters[c];
   I.stats.spinor_bits = storage ? 16 : 32;
+  I.stats.hop_arith_bits = hop_arith_bits(I, storage);
   return total;
 }
 
@@ -1763,6 +1817,7 @@ long invert_halves(Inv& I, const double2* in_e, const double2* in_o, double2* ou
     total_iter = cg_mixed(I, out_o_p, in_o_p, stop_rsd, cap, &true_rel);
     I.stats.cycles += 1;
     I.stats.spinor_bits = 32;
+    I.stats.hop_arith_bits = 32;
   }
   I.stats.total_iter += total_iter;
   if (true_rel > I.stats.true_residual) I.stats.true_residual = true_rel;
@@ -2055,6 +2110,7 @@ long solve_columns(Inv& I, int n, Inv* const* colinv, Load&& load, Store&& store
       if (col_iters)
         for (int j = 0; j < w; ++j) col_iters[c0 + j] = it / w;  // the FP32 fused path counts only the sum
       I.stats.spinor_bits = 32;
+      I.stats.hop_arith_bits = 32;
     }
     I.stats.cycles += 1;
     for (int j = 0; j < w; ++j) {
@@ -2375,7 +2431,7 @@ extern "C" {
 
 const char* qlat_dwf_last_error(void) { return g_last_error.c_str(); }
 
-int qlat_dwf_abi_version(void) { return 3; }
+int qlat_dwf_abi_version(void) { return 4; }
 
 int qlat_dwf_set_checking_invert(int on)
 {
@@ -3552,6 +3608,7 @@ int qlat_dwf_bench_kernel(qlat_dwf_inverter* inv, int which, int precision, int 
       const int tile = std::max(1, std::min(tc.width, qdwf::kMaxRhs));
       I.stats.rhs_tile = tile;
       I.stats.sites_per_block = tc.sb;
+      I.stats.hop_arith_bits = hop_arith_bits(I, storage);
       tp.reset(new ThreePhase(I, tile, storage, tc.sb));
       for (int c = 0; c < tile; ++c) cols.push_back(c);
       // finite, non-trivial input values

@@ -190,6 +190,20 @@ __device__ __forceinline__ void neighbours(const LatGeo& g, const int x[4], int6
   }
 }
 
+// Half index of neighbour d (x+mu for d = mu, x-mu for d = 4+mu) of the
+// device half-site h on parity eo.
+__device__ __forceinline__ int64_t neighbour_of(const LatGeo& g, int64_t h, int eo, int d)
+{
+  int x[4];
+  half_coord(g, h, eo, x);
+  const int mu = d & 3;
+  const bool fwd = d < 4;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    if (c == mu) x[c] = fwd ? wrap_inc(x[c], g.n[c]) : wrap_dec(x[c], g.n[c]);
+  return half_index(g, x);
+}
+
 // Workgroup id remapped so that consecutive logical blocks land on the same
 // XCD (hardware dispatches workgroup b to XCD b % 8).
 __device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t nblk)
@@ -657,6 +671,17 @@ __device__ __forceinline__ void cfma_s(cplx<float>& acc, float s, cplx<float> t)
   acc.im = fmaf(s, t.im, acc.im);
 }
 
+// a.x b.x + a.y b.y + 0 in FP32 from FP16 pairs as one v_dot2_f32_f16 with
+// an inline-constant accumulator: the builtin with a zero accumulator is
+// compiled to a move of 0 plus the two-operand form, one instruction more
+// per dot-product chain.  An ALU instruction only (no memory operand).
+__device__ __forceinline__ float fdot2_z(h16x2 a, h16x2 b)
+{
+  float r;
+  asm("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // acc += sc * (1 + sg gamma_mu) U psi with psi, U in FP16 (la, lb as above)
 template <int MU, int SG>
 __device__ __forceinline__ void hop_term_h2(const h16x2 la[9], const h16x2 lb[9], const h16x2 psi[12], float sc,
@@ -668,9 +693,10 @@ __device__ __forceinline__ void hop_term_h2(const h16x2 la[9], const h16x2 lb[9]
   const float ssg = SG > 0 ? sc : -sc;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    float y0r = 0.f, y0i = 0.f, y1r = 0.f, y1i = 0.f;
+    float y0r = fdot2_z(la[r * 3], w0[0]), y0i = fdot2_z(lb[r * 3], w0[0]);
+    float y1r = fdot2_z(la[r * 3], w1[0]), y1i = fdot2_z(lb[r * 3], w1[0]);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 1; c < 3; ++c) {
       y0r = __builtin_amdgcn_fdot2(la[r * 3 + c], w0[c], y0r, false);
       y0i = __builtin_amdgcn_fdot2(lb[r * 3 + c], w0[c], y0i, false);
       y1r = __builtin_amdgcn_fdot2(la[r * 3 + c], w1[c], y1r, false);

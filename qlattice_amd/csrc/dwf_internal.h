@@ -91,9 +91,9 @@ struct CgStepArgs {
 // i-th column of the launch).  k_cg_scalars turns the hermop reductions
 // into alpha, beta and |r|^2, so a CG iteration needs no host round trip
 // and can be captured in a graph.  mode: 0 iterating; 2 the column reached
-// its reliable-update threshold this iteration (the step only adds alpha p
-// to x); 1 frozen (every kernel skips the column until the host has done
-// the FP64 update and reset the mode).
+// its reliable-update threshold this iteration (the step only adds
+// alpha_q q + alpha p to x); 1 frozen (every kernel skips the column until
+// the host has done the FP64 update and reset the mode).
 struct CgState {
   double rr[kMaxRhs];       // |r|^2 after the last completed step
   double maxrr[kMaxRhs];    // |r|^2 at the last reliable update
@@ -102,8 +102,11 @@ struct CgState {
   double alpha_x[kMaxRhs];  // step coefficients of the next step
   double alpha_r[kMaxRhs];
   double beta[kMaxRhs];
+  double alpha_q[kMaxRhs];  // x coefficient of the previous p (in the other p buffer)
+  double pend[kMaxRhs];     // alpha of a direction not yet added to x
   double target;            // stop_rsd^2 (normalised columns)
   double d2;                // reliable-update factor delta^2
+  int x_defer;              // x gets two directions every other step
   int mode[kMaxRhs];
   int iters[kMaxRhs];       // iterations done since the host last reset them
   int bad[kMaxRhs];         // non-finite alpha or residual

@@ -494,6 +494,107 @@ __device__ __forceinline__ void smat_row(const float4 (*hr)[6][SB], const cplx<f
   }
 }
 
+// The s-matrix phase of a block of 16 sites x ls rows (ls a multiple of 4,
+// ls/4 waves) on the matrix cores: out(s) = sum_t K_chi[s][t] h(t) is, per
+// chirality and real component r of the 12 of that chirality, a product
+// (16 x 16 rows s) x (ls columns t) x (16 sites) done as ls/4 steps of
+// v_mfma_f32_16x16x4_f32 (full FP32 products and sums).  Lane l of a wave
+// supplies A[s = l%16][t = 4 kb + l/16] (the matrix) and B[t = 4 kb + l/16]
+// [site = l%16] (one real of hr), and receives D[s = 4 (l/16) + i][site =
+// l%16], i < 4.  A float4 of hr (reals 4k..4k+3: complex 2k, 2k+1, one
+// chirality) feeds four products at once; wave w takes the float4 columns
+// k = w, w + ls/4, ... < 6.  The results are written back over hr (after
+// every wave has read it), and thread (site, row) then reads its own row.
+// km: the block's matrices, staged in LDS (km[chi][s][t]).
+// Complex (zMoebius) matrices take four products per complex pair
+// (re K re h - im K im h, re K im h + im K re h).
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+template <int LSMAX, bool REAL>
+__device__ __forceinline__ void smat_mfma16(float4 (*hr)[6][16], const cplx<float> (*km)[LSMAX][LSMAX], int ls, int site,
+                                            int row, bool act, cplx<float> v[12])
+{
+  constexpr int MT = (LSMAX + 15) / 16;  // 16-row tiles of s
+  constexpr int KB = LSMAX / 4;          // 4-column steps of t
+  // float4 columns per wave: the dispatch gives this instance ls = 8
+  // (2 waves) for LSMAX 8, else ls > 8 (at least 3 waves)
+  constexpr int UMAX = LSMAX <= 8 ? 3 : 2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15, lq = lane >> 4;
+  const int nw = ls >> 2, nkb = ls >> 2;
+  __syncthreads();  // hr (and the matrices) complete
+  // matrix operands of this lane: K_chi[mt*16 + li][4 kb + lq]
+  float are[2][MT][KB], aim[2][MT][KB];
+#pragma unroll
+  for (int chi = 0; chi < 2; ++chi)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const int s = mt * 16 + li, t = 4 * kb + lq;
+        cplx<float> m = cmk<float>(0.f, 0.f);
+        if (kb < nkb && s < ls) m = km[chi][s][t];
+        are[chi][mt][kb] = m.re;
+        aim[chi][mt][kb] = m.im;
+      }
+  mf4 d[UMAX][MT][4];
+#pragma unroll
+  for (int u = 0; u < UMAX; ++u) {
+    const int k = wv + u * nw;
+    if (k < 6) {  // wave-uniform
+      const int chi = k / 3;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[u][mt][j] = (mf4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        if (kb < nkb) {
+          const float4 b = hr[4 * kb + lq][k][li];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const float ar = chi ? are[1][mt][kb] : are[0][mt][kb];
+            d[u][mt][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b.x, d[u][mt][0], 0, 0, 0);
+            d[u][mt][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b.y, d[u][mt][1], 0, 0, 0);
+            d[u][mt][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b.z, d[u][mt][2], 0, 0, 0);
+            d[u][mt][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b.w, d[u][mt][3], 0, 0, 0);
+            if (!REAL) {
+              const float ai = chi ? aim[1][mt][kb] : aim[0][mt][kb];
+              d[u][mt][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b.y, d[u][mt][0], 0, 0, 0);
+              d[u][mt][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b.x, d[u][mt][1], 0, 0, 0);
+              d[u][mt][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b.w, d[u][mt][2], 0, 0, 0);
+              d[u][mt][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b.z, d[u][mt][3], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();  // every wave has read hr
+#pragma unroll
+  for (int u = 0; u < UMAX; ++u) {
+    const int k = wv + u * nw;
+    if (k < 6) {
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int s = mt * 16 + 4 * lq + i;
+          if (s < ls) hr[s][k][li] = make_float4(d[u][mt][0][i], d[u][mt][1][i], d[u][mt][2][i], d[u][mt][3][i]);
+        }
+    }
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float4 q = hr[row][k][site];
+      v[2 * k] = cmk<float>(q.x, q.y);
+      v[2 * k + 1] = cmk<float>(q.z, q.w);
+    }
+  }
+}
+
 // streaming store that also returns the stored values
 template <class ST>
 __device__ __forceinline__ void store_qs(typename SpinorIO<ST>::slot_t* __restrict__ f, int ls, int s, int64_t h,
@@ -554,15 +655,16 @@ k_hop_m5s(HopM5Args a)
         72 * SB, [&](int i) { return links[(i / SB) * 64 + (i % SB)]; },
         [&](int i, float2 w) { sl[i / SB][i % SB] = w; });
   }
-  // neighbour indices once per site (row 0), shared by the Ls rows
-  __shared__ int nbs[8][SB];
-  if (threadIdx.x < SB && h0 + threadIdx.x < a.geo.vh) {
-    int x[4];
-    half_coord(a.geo, h0 + threadIdx.x, a.eo_out, x);
-    int64_t nb[8];
-    neighbours(a.geo, x, nb);
-#pragma unroll
-    for (int d = 0; d < 8; ++d) nbs[d][threadIdx.x] = (int)nb[d];
+  // the block's s-matrices (both chiralities) in LDS, loaded while the
+  // links and neighbour offsets are staged
+  __shared__ cplx<float> km[SMAT ? 2 : 1][SMAT ? LSMAX : 1][SMAT ? LSMAX : 1];
+  if constexpr (SMAT) stage_km<LSMAX>(km, pick(a.mats, ci), a.ls);
+  // neighbour indices once per site, shared by the Ls rows (thread (site,
+  // d) of the first 8 SB forms neighbour d of its site)
+  __shared__ unsigned nbs[8][SB];
+  for (int i = threadIdx.x; i < 8 * SB; i += blockDim.x) {
+    const int st = i % SB, d = i / SB;
+    if (h0 + st < a.geo.vh) nbs[d][st] = (unsigned)neighbour_of(a.geo, h0 + st, a.eo_out, d);
   }
   __syncthreads();
   const int ls = a.ls;
@@ -574,7 +676,7 @@ k_hop_m5s(HopM5Args a)
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = cmk<float>(0, 0);
   if (active && row < ls) {
-    int64_t nb[8];
+    unsigned nb[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) nb[d] = nbs[d][site];
     const slot_t* __restrict__ in = (const slot_t*)a.in + coff;
@@ -630,18 +732,20 @@ k_hop_m5s(HopM5Args a)
   cplx<float> v[12];
   if constexpr (SMAT) {
     __shared__ float4 hr[LSMAX][6][SB];
-    // the block's s-matrices (both chiralities) in LDS
-    __shared__ cplx<float> km[2][LSMAX][LSMAX];
-    stage_km<LSMAX>(km, pick(a.mats, ci), ls);
     if (active && row < ls) {
 #pragma unroll
       for (int k = 0; k < 6; ++k)
         hr[row][k][site] = make_float4(sc * acc[2 * k].re, sc * acc[2 * k].im, sc * acc[2 * k + 1].re,
                                        sc * acc[2 * k + 1].im);
     }
-    __syncthreads();
-    // thread (site, s) forms its own output row s for all 12 components
-    if (active && row < ls) smat_row<SB, LSMAX, REAL>(hr, km, ls, site, row, v);
+    if constexpr (SB == 16) {
+      // on the matrix cores (ls is a multiple of 4 here: whole waves)
+      smat_mfma16<LSMAX, REAL>(hr, km, ls, site, row, active && row < ls, v);
+    } else {
+      // thread (site, s) forms its own output row s for all 12 components
+      __syncthreads();
+      if (active && row < ls) smat_row<SB, LSMAX, REAL>(hr, km, ls, site, row, v);
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = cmk<float>(sc * acc[k].re, sc * acc[k].im);
@@ -794,13 +898,9 @@ __global__ void __launch_bounds__(SB * LSMAX) k_hop_m5d(HopM5dArgs a)
   const int pslot = (a.eo_out == 2) ? 0 : 1;
   const double2* __restrict__ links = a.links + lidx(a.geo.stride, pslot, 0, h0);
   for (int i = threadIdx.x; i < 72 * SB; i += blockDim.x) buf[i / SB][i % SB] = links[(i / SB) * 64 + (i % SB)];
-  if (threadIdx.x < SB && h0 + threadIdx.x < a.geo.vh) {
-    int x[4];
-    half_coord(a.geo, h0 + threadIdx.x, a.eo_out, x);
-    int64_t nb[8];
-    neighbours(a.geo, x, nb);
-#pragma unroll
-    for (int d = 0; d < 8; ++d) nbs[d][threadIdx.x] = (int)nb[d];
+  for (int i = threadIdx.x; i < 8 * SB; i += blockDim.x) {  // thread (site, d): neighbour d
+    const int st = i % SB, d = i / SB;
+    if (h0 + st < a.geo.vh) nbs[d][st] = (int)neighbour_of(a.geo, h0 + st, a.eo_out, d);
   }
   stage_km_d<SB, LSMAX>(km, a.mat, a.ls);
   __syncthreads();
@@ -955,12 +1055,19 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
     const int mode = a.cg->mode[ci];
     if (mode == 1) return;
     al = (float)a.cg->alpha_x[ci];
-    if (mode == 2) {  // reliable-update point: only x += alpha p
+    aq = a.p_out ? (float)a.cg->alpha_q[ci] : 0.f;
+    if (mode == 2) {  // reliable-update point: only x += alpha_q q + alpha p
       if (UPDX && active && row < ls) {
         cplx<float> pv[12], xv[12];
         SpinorIO<ST>::load((const slot_t*)a.p + coff, ls, row, h, pv);
         xslot_t* x = (xslot_t*)a.x + xoff;
         SpinorIO<SX>::load(x, ls, row, h, xv);
+        if (aq != 0.f) {  // block-uniform
+          cplx<float> qv[12];
+          SpinorIO<ST>::load((const slot_t*)a.p_out + coff, ls, row, h, qv);
+#pragma unroll
+          for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + aq * qv[k].re, xv[k].im + aq * qv[k].im);
+        }
 #pragma unroll
         for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + al * pv[k].re, xv[k].im + al * pv[k].im);
         SpinorIO<SX>::store(x, ls, row, h, xv);
@@ -1088,10 +1195,28 @@ __global__ void __launch_bounds__(64) k_cg_scalars(CgState* __restrict__ cg, con
     cg->mode[i] = 1;
     return;
   }
-  cg->alpha_x[i] = alpha;
+  // x += alpha_x p + alpha_q q (q: the previous p, in the other buffer),
+  // with the folded update of the host loop: a direction's alpha waits one
+  // step in pend and is added together with the next one
+  const double pend = cg->pend[i];
   if (rr_new <= cg->d2 * cg->maxrr[i] || rr_new <= cg->target) {
+    cg->alpha_x[i] = alpha;
+    cg->alpha_q[i] = pend;
+    cg->pend[i] = 0.0;
     cg->mode[i] = 2;
     return;
+  }
+  if (!cg->x_defer) {
+    cg->alpha_x[i] = alpha;
+    cg->alpha_q[i] = 0.0;
+  } else if (pend == 0.0) {
+    cg->alpha_x[i] = 0.0;
+    cg->alpha_q[i] = 0.0;
+    cg->pend[i] = alpha;
+  } else {
+    cg->alpha_x[i] = alpha;
+    cg->alpha_q[i] = pend;
+    cg->pend[i] = 0.0;
   }
   cg->alpha_r[i] = alpha;
   cg->beta[i] = rr_new / rr;
@@ -1766,24 +1891,28 @@ __device__ __forceinline__ void force_mu(const ForceArgs& a, int p, int64_t h, c
   }
 }
 
-__global__ void __launch_bounds__(128) k_force(ForceArgs a)
+// Block = 64 sites x the 4 directions (one wave per mu): the four waves
+// read the sites' own X and Mpc X at every s together, so those come from
+// HBM once and are shared through the caches, not re-read per direction.
+__global__ void __launch_bounds__(256) k_force(ForceArgs a)
 {
-  const int64_t h = (int64_t)blockIdx.x * 128 + threadIdx.x;
+  const int64_t h = (int64_t)blockIdx.x * 64 + threadIdx.x;
   const int p = blockIdx.y;  // parity slot of y: 0 even, 1 odd
+  const int mu = threadIdx.y;  // wave-uniform
   if (h >= a.geo.vh) return;
   int x[4];
   half_coord(a.geo, h, p == 0 ? 2 : 1, x);
   double2* fout = (double2*)a.force;
   double2* gout = (double2*)a.grad;
-  force_mu<0>(a, p, h, x, fout, gout);
-  force_mu<1>(a, p, h, x, fout, gout);
-  force_mu<2>(a, p, h, x, fout, gout);
-  force_mu<3>(a, p, h, x, fout, gout);
+  if (mu == 0) force_mu<0>(a, p, h, x, fout, gout);
+  else if (mu == 1) force_mu<1>(a, p, h, x, fout, gout);
+  else if (mu == 2) force_mu<2>(a, p, h, x, fout, gout);
+  else force_mu<3>(a, p, h, x, fout, gout);
 }
 
 void launch_force(const ForceArgs& a, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_force, dim3((unsigned)((a.geo.vh + 127) / 128), 2), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(k_force, dim3((unsigned)((a.geo.vh + 63) / 64), 2), dim3(64, 4), 0, st, a);
 }
 
 // ================================================================== M5

@@ -41,7 +41,7 @@ static int cpu_checks(void)
   int rc;
   double s = 0;
   long n = 0, c = 0;
-  if (qlat_dwf_abi_version() != 3) {
+  if (qlat_dwf_abi_version() != 4) {
     printf("FAIL abi version %d\n", qlat_dwf_abi_version());
     return 1;
   }

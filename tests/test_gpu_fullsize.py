@@ -108,6 +108,7 @@ def test_24x64_mixed16_propagator_against_fp64(config_b):
     say(f"mixed16: {st16['seconds']:.2f} s, {st16['total_iter']} iterations, tile {st16['rhs_tile']}, "
         f"odd residual {st16['true_residual']:.2e}, 5D residual {st16['true_residual_5d']:.2e}")
     assert st16["spinor_bits"] == 16
+    assert st16["hop_arith_bits"] == 16
     assert st16["true_residual"] <= 1e-8
     assert 0 < st16["true_residual_5d"] <= 1e-6
     say("24^3x64 Ls12: the same propagator with the FP64 solver")

@@ -167,11 +167,14 @@ def test_three_phase_multi_rhs_propagator(monkeypatch, prop_case, solver, tile, 
 @pytest.mark.parametrize("solver", ["mixed16", "mixed"])
 def test_graph_replay_matches_eager_loop(monkeypatch, prop_case, solver):
     """Graph-captured CG iterations (QLAT_DWF_GRAPH = K iterations per
-    replay, scalars on the device) against the loop with one host round trip
-    per iteration (QLAT_DWF_GRAPH=0): the same iteration and reliable-update
-    counts up to the timing of the FP64 updates, every column at the FP64
-    true residual and on the oracle's solution, replayed many times with the
-    FP64 updates between replays."""
+    replay, scalars on the device, x updated every other step with two
+    directions as in the host loop) against the loop with one host round
+    trip per iteration (QLAT_DWF_GRAPH=0): every column does the same
+    arithmetic in the same order (a column that reaches its reliable-update
+    point waits frozen for the end of the replay), so the iteration and
+    reliable-update counts are equal; every column at the FP64 true residual
+    and on the oracle's solution, replayed many times with the FP64 updates
+    between replays, from both p buffers (K odd and even)."""
     print(f"\n[gpu] graph replay vs eager loop ({solver}) 4^3x8 Ls8", flush=True)
     monkeypatch.setenv("QLAT_DWF_RHS_TILE", "12")
     monkeypatch.setenv("QLAT_DWF_THREE_PHASE", "1")
@@ -197,8 +200,8 @@ def test_graph_replay_matches_eager_loop(monkeypatch, prop_case, solver):
     base_sol, base = out["0"]
     for k in ("1", "3", "16"):
         sol, st = out[k]
-        assert abs(st["total_iter"] - base["total_iter"]) <= 0.02 * base["total_iter"], (k, st, base)
-        assert abs(st["reliable_updates"] - base["reliable_updates"]) <= 1, (k, st, base)
+        assert st["total_iter"] == base["total_iter"], (k, st, base)
+        assert st["reliable_updates"] == base["reliable_updates"], (k, st, base)
         assert rel(sol, base_sol) < 1e-7, k
 
 

@@ -114,4 +114,5 @@ def test_mixed_solvers_reach_tolerance(pair, solver):
     inv.set_stop_rsd(1e-8)
     assert st["true_residual"] <= 1e-9
     assert st["spinor_bits"] == (16 if solver == "mixed16" else 32)
+    assert st["hop_arith_bits"] == (16 if solver == "mixed16" else 32)
     assert rel(sol.data, exact) < 1e-6

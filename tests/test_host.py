@@ -27,7 +27,7 @@ def test_library_exports_every_header_symbol():
     for s in syms:
         assert hasattr(so, s), s
     assert sorted(_lib.EXPORTED) == syms
-    assert _lib.load().abi_version() == 3
+    assert _lib.load().abi_version() == 4
 
 
 def test_binding_declares_every_header_symbol():
