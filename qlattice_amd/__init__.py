@@ -44,13 +44,18 @@ from .inverter import (
     multiply_hermop_sym2,
     multiply_hermop_sym2_fp32,
     multiply_m,
+    multiply_m_dwf,
     multiply_m_e_e,
     multiply_m_e_e_inv,
     multiply_m_e_o,
+    multiply_m_eo_eo,
+    multiply_m_full,
+    multiply_m_with_prec_sym2,
     multiply_mdag,
     multiply_mdag_e_e,
     multiply_mdag_e_e_inv,
     multiply_mdag_e_o,
+    multiply_mdag_eo_eo,
     multiply_mpc_sym2,
     multiply_mpcdag_sym2,
     multiply_wilson_d_e_o_no_comm,

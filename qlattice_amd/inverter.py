@@ -17,7 +17,8 @@ import numpy as np
 
 from . import _lib
 from .fermion_action import FermionAction
-from .fields import FermionField4d, FermionField5d, GaugeField, Prop
+from .fields import (FermionField4d, FermionField5d, GaugeField, Prop, fermion_field_4d_from_5d,
+                     fermion_field_5d_from_4d, get_half_fermion, set_half_fermion)
 
 SOLVERS = {"fp64": _lib.QLAT_DWF_SOLVER_FP64, "mixed": _lib.QLAT_DWF_SOLVER_MIXED_FP32,
            "mixed16": _lib.QLAT_DWF_SOLVER_MIXED_FP16}
@@ -161,6 +162,11 @@ class InverterDomainWall(Inverter):
     def __mul__(self, src):
         """prop_src: Prop, FermionField4d, full FermionField5d, or a list."""
         if isinstance(src, list):
+            # inverter.py:106-107 solves the list one by one; propagators
+            # here go through one batched call, all their columns as one
+            # multi-RHS problem
+            if len(src) > 1 and all(isinstance(p, Prop) for p in src):
+                return _invert_props(self, src)
             return [self * p for p in src]
         if self.timer is not None:
             self.timer.start()
@@ -180,6 +186,41 @@ class InverterDomainWall(Inverter):
             if self.timer is not None:
                 self.timer.stop()
         return sol
+
+
+def _invert_props(inv, props):
+    """Several propagators with one inverter in one batched call: every
+    column of every propagator becomes a 5D source by the chiral embedding
+    (qcd-prop.h:96-116) and the 12 n columns are solved together as
+    multi-RHS tiles (qlat_dwf_invert_5d_batch); each solution goes back to
+    4D (qcd-prop.h:118-138).  The same columns as invert(Prop) would solve
+    one propagator at a time (qcd-prop.h:169-186)."""
+    ls = inv.fa.ls()
+    geo = inv.geo_
+    V = geo.total_volume()
+    srcs = []
+    for p in props:
+        d = np.ascontiguousarray(p.data, dtype=np.complex128).reshape(V, 12, 12)
+        for j in range(12):
+            col = FermionField4d(geo)
+            col.data = np.ascontiguousarray(d[:, :, j])
+            srcs.append(fermion_field_5d_from_4d(col, ls))
+    if inv.timer is not None:
+        inv.timer.start()
+    try:
+        sols, _ = invert_batch([inv] * len(srcs), srcs)
+    finally:
+        if inv.timer is not None:
+            inv.timer.stop()
+    out = []
+    for k in range(len(props)):
+        r = Prop(geo)
+        data = np.empty((V, 12, 12), dtype=np.complex128)
+        for j in range(12):
+            data[:, :, j] = fermion_field_4d_from_5d(sols[12 * k + j]).data
+        r.data = data.reshape(r.data.shape)
+        out.append(r)
+    return out
 
 
 # ================================================================ operators
@@ -249,6 +290,23 @@ def multiply_m_e_o(out, in_, inv):
 def multiply_mdag_e_o(out, in_, inv):
     """dslash.cpp:1005-1060."""
     return _apply_half("multiply_mdag_e_o", out, in_, inv, in_.eo, 3 - in_.eo)
+
+
+def multiply_m_eo_eo(out, in_, inv, eo_out, eo_in):
+    """dslash.cpp:1193-1211: the (eo_out, eo_in) block of M, M_ee (or M_oo)
+    for equal parities and M_eo (or M_oe) otherwise."""
+    assert in_.eo == eo_in and eo_out in (1, 2)
+    if eo_out == eo_in:
+        return multiply_m_e_e(out, in_, inv)
+    return multiply_m_e_o(out, in_, inv)
+
+
+def multiply_mdag_eo_eo(out, in_, inv, eo_out, eo_in):
+    """dslash.cpp:1213-1231: the (eo_out, eo_in) block of M^+."""
+    assert in_.eo == eo_in and eo_out in (1, 2)
+    if eo_out == eo_in:
+        return multiply_mdag_e_e(out, in_, inv)
+    return multiply_mdag_e_o(out, in_, inv)
 
 
 def multiply_mpc_sym2(out, in_, inv):
@@ -326,6 +384,48 @@ def multiply_m(out, in_, inv):
 def multiply_mdag(out, in_, inv):
     """dslash.cpp:1082-1100."""
     return _apply_full("multiply_mdag", out, in_, inv)
+
+
+def multiply_m_full(out, in_, inv):
+    """multiply_m_full (dslash.cpp:490-540): M = D_W(-M5)(b + c S) + 1 - S
+    on the full field, the operator of multiply_m (one implementation on
+    the GPU serves both)."""
+    return _apply_full("multiply_m", out, in_, inv)
+
+
+def multiply_m_dwf(out, in_, inv):
+    """multiply_m_dwf (dslash.cpp:329-398): the Shamir operator; like the
+    reference it requires mobius_scale == 1 (:342).  Equal to multiply_m
+    for that action."""
+    if inv.fa.mobius_scale() != 1.0 or inv.fa.is_using_zmobius:
+        raise AssertionError("multiply_m_dwf: Shamir only (mobius_scale == 1, no zMobius)")
+    return _apply_full("multiply_m", out, in_, inv)
+
+
+def multiply_m_with_prec_sym2(out, in_, inv):
+    """multiply_m_with_prec_sym2 (dslash.cpp:1271-1296): M x through the
+    sym2 factorisation, from the half-field operators in the reference's
+    order (each one on the GPU):
+    out_e = M_ee (M_ee^-1 M_eo x_o + x_e),
+    out_o = Mpc M_oo x_o + M_oe M_ee^-1 out_e."""
+    assert isinstance(in_, FermionField5d) and in_.eo == 0
+    in_e = get_half_fermion(in_, 2)
+    in_o = get_half_fermion(in_, 1)
+    out_e = multiply_m_e_e_inv(None, multiply_m_e_o(None, in_o, inv), inv)
+    out_e.data = out_e.data + in_e.data
+    out_o = multiply_m_e_e(None, in_o, inv)
+    out_e = multiply_m_e_e(None, out_e, inv)
+    out_o = multiply_mpc_sym2(None, out_o, inv)
+    tmp = multiply_m_e_o(None, multiply_m_e_e_inv(None, out_e, inv), inv)
+    out_o.data = out_o.data + tmp.data
+    res = FermionField5d(in_.geo, in_.ls, 0)
+    res.data = np.empty_like(np.ascontiguousarray(in_.data, dtype=np.complex128))
+    set_half_fermion(res, out_e, 2)
+    set_half_fermion(res, out_o, 1)
+    if out is not None:
+        out.geo, out.ls, out.eo, out.data = res.geo, res.ls, 0, res.data
+        return out
+    return res
 
 
 def multiply_d_minus(out, in_, inv):

@@ -129,3 +129,74 @@ def test_five_d_residual_of_multi_rhs_columns(pair):
     r = [rel(orc.apply_full("m", x.data), orc.apply_full("d_minus", b)) for x, b in zip(sols, bs)]
     assert st["true_residual"] <= 1e-8
     assert abs(st["true_residual_5d"] - max(r)) < 1e-3 * max(r), (st["true_residual_5d"], r)
+
+
+def test_alternative_factorisations_match_oracle(pair):
+    """multiply_m_with_prec_sym2 (dslash.cpp:1271-1296) from the half-field
+    operators, multiply_m_full (:490-540) and the parity blocks
+    multiply_m_eo_eo / multiply_mdag_eo_eo (:1193-1231), against the
+    oracle's own statements of the same functions."""
+    geo, inv, orc = pair
+    x = rand_field((geo.total_volume(), LS, 12), 820)
+    f = full(geo, x)
+    ref = orc.apply_full("m_with_prec_sym2", x)
+    assert rel(q.multiply_m_with_prec_sym2(None, f, inv).data, ref) < 1e-12
+    assert rel(q.multiply_m(None, f, inv).data, ref) < 1e-12
+    assert rel(q.multiply_m_full(None, f, inv).data, orc.apply_full("m_full", x)) < 1e-12
+    for eo_in in (1, 2):
+        h = q.FermionField5d(geo, LS, eo_in)
+        h.data = rand_field((geo.total_volume() // 2, LS, 12), 821 + eo_in)
+        for eo_out in (1, 2):
+            same = eo_out == eo_in
+            m = q.multiply_m_eo_eo(None, h, inv, eo_out, eo_in)
+            md = q.multiply_mdag_eo_eo(None, h, inv, eo_out, eo_in)
+            assert m.eo == eo_out and md.eo == eo_out
+            assert rel(m.data, orc.apply_half("m_e_e" if same else "m_e_o", h.data, eo_in)) < 1e-12
+            assert rel(md.data, orc.apply_half("mdag_e_e" if same else "mdag_e_o", h.data, eo_in)) < 1e-12
+
+
+def test_multiply_m_dwf_is_shamir_only(pair):
+    """multiply_m_dwf (dslash.cpp:329-398) asserts mobius_scale == 1 (:342);
+    on a Shamir action it is the oracle's m_dwf."""
+    geo, inv, orc = pair
+    x = full(geo, rand_field((geo.total_volume(), LS, 12), 830))
+    with pytest.raises(AssertionError):
+        q.multiply_m_dwf(None, x, inv)
+    U = rand_gauge(SITE, 0.4, 831)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    kw = dict(mass=0.1, ls=LS, m5=1.8, mobius_scale=1.0)
+    shamir = q.InverterDomainWall(gf=gf, fa=q.FermionAction(**kw))
+    ref = O.OracleInverter(SITE, U, **kw).apply_full("m_dwf", x.data)
+    assert rel(q.multiply_m_dwf(None, x, shamir).data, ref) < 1e-12
+
+
+@pytest.mark.parametrize("solver", ["fp64", "mixed16"])
+def test_list_of_propagators_is_one_batch(pair, solver):
+    """inv * [prop, prop] (inverter.py:106-107 solves the list one by one):
+    here all 24 columns go through one batched multi-RHS call; each
+    propagator equals its own inv * prop (FP64: the same solves, 1e-12;
+    mixed16: both at the FP64 true residual, 1e-6) and the oracle's."""
+    geo, inv, orc = pair
+    props = []
+    for k in range(2):
+        p = q.Prop(geo)
+        p.data = rand_field(p.data.shape, 840 + k)
+        props.append(p)
+    inv.set_solver(solver)
+    inv.set_stop_rsd(1e-9)
+    try:
+        both = inv * props
+        one = [inv * p for p in props]
+    finally:
+        inv.set_solver("fp64")
+        inv.set_stop_rsd(1e-8)
+    assert len(both) == 2 and all(isinstance(b, q.Prop) for b in both)
+    for b, o in zip(both, one):
+        assert rel(b.data, o.data) < (1e-12 if solver == "fp64" else 1e-6)
+    orc.set_params(1e-12, 200, 300)
+    V = geo.total_volume()
+    for j in (0, 7):
+        ref, _ = orc.invert_4d(np.ascontiguousarray(props[1].data.reshape(V, 12, 12)[:, :, j]))
+        assert rel(both[1].data.reshape(V, 12, 12)[:, :, j], ref) < 1e-6
+    orc.set_params(1e-8, 200, 300)
