@@ -91,7 +91,7 @@ def cpu_baseline(cfg, U, iters_fp64, sample_hermops):
     """The oracle (C++ restatement of the reference CPU path, FP64, OpenMP)
     on this host: time `sample_hermops` applications of Mpc^+ Mpc at the
     full lattice, then extrapolate the 12-column time-to-solution with the
-    FP64 GPU iteration count (extrapolated, per BASELINE plan)."""
+    FP64 GPU iteration count (an extrapolation: the full CPU solve would take hours)."""
     import numpy as np
 
     from oracle import oracle as O

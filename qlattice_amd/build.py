@@ -18,7 +18,7 @@ import sysconfig
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqlat_dwf_hip.so")
-SOURCES = ["dwf_kernels.hip", "dwf_api.hip"]
+SOURCES = ["dwf_kernels.hip", "dwf_three_phase.hip", "dwf_api.hip"]
 HEADERS = ["dwf_common.h", "dwf_internal.h", os.path.join("..", "..", "include", "qlat_dwf.h")]
 
 INCLUDE = os.path.join(HERE, "..", "include")
@@ -41,7 +41,7 @@ FLAGS = [
 ]
 
 
-def build(verbose=True, jobs=2):
+def build(verbose=True, jobs=3):
     objs = []
     procs = []
     for src in SOURCES:

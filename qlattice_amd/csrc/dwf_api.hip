@@ -1094,7 +1094,12 @@ struct ThreePhase {
   ThreePhase(Inv& inv, int ncol, int storage_, int sb_ = 0)
       : I(inv), storage(storage_), sb(sb_ ? sb_ : inv.tp_sb)
   {
-    if (!qdwf::three_phase_ok(I.ls, sb)) sb = (sb == 32) ? 16 : 32;
+    if (!qdwf::three_phase_ok(I.ls, sb))
+      for (int alt : {16, 32, 8})
+        if (qdwf::three_phase_ok(I.ls, alt)) {
+          sb = alt;
+          break;
+        }
     if (!qdwf::three_phase_ok(I.ls, sb))
       throw ArgError("the three-phase kernels need an even ls in [6, 16], or ls 20 or 24");
     if (storage && fieldh_units(I) * 16 >= (1ll << 32))
@@ -1655,7 +1660,7 @@ TileChoice autotune_tile(Inv& I, int storage, int ncol)
   QDWF_HIP(hipEventCreate(&e1));
   double best = 0;
   TileChoice choice{0, 32};
-  for (int sb : {32, 16}) {
+  for (int sb : {32, 16, 8}) {
     if (!qdwf::three_phase_ok(I.ls, sb)) continue;
     for (int w : {4, 8, 12, 16}) {
       if (w > wmax) continue;
@@ -2525,7 +2530,10 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_FUSED64")) I->fused64 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_THREE_PHASE")) I->three_phase32 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_RHS_TILE")) I->rhs_tile = std::max(0, std::min(qdwf::kMaxRhs, std::atoi(e)));
-    if (const char* e = std::getenv("QLAT_DWF_TP_SB")) I->tp_sb = std::atoi(e) == 16 ? 16 : 32;
+    if (const char* e = std::getenv("QLAT_DWF_TP_SB")) {
+      const int v = std::atoi(e);
+      I->tp_sb = (v == 8 || v == 16) ? v : 32;
+    }
     if (const char* e = std::getenv("QLAT_DWF_SITE_MAJOR")) I->site_major = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_FP16_MODE")) I->fp16_mode = std::max(1, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("QLAT_DWF_XDEFER")) I->x_defer = std::atoi(e) != 0;
@@ -2534,11 +2542,12 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
     if (const char* e = std::getenv("QLAT_DWF_H2")) I->h2_hop = std::atoi(e) != 0 ? 1 : 0;
     {
       // partial sums per column: BLAS grids, one per block of the fused
-      // kernels (32 sites) and of the three-phase kernels (16 sites, up to
-      // three sums each), or one per (256-site block, slot) (s-matrix kernel)
+      // kernels (32 sites) and of the three-phase kernels (8 sites or more,
+      // up to three sums each), or one per (256-site block, slot) (s-matrix
+      // kernel)
       int64_t np = qdwf::kMaxBlasGrid * 4;
       np = std::max<int64_t>(np, (I->geo.vh + 31) / 32 + 16);
-      np = std::max<int64_t>(np, 3 * ((I->geo.vh + 15) / 16) + 16);
+      np = std::max<int64_t>(np, 3 * ((I->geo.vh + 7) / 8) + 16);
       np = std::max<int64_t>(np, (I->geo.vh + qdwf::kM5Block - 1) / qdwf::kM5Block * 12 + 16);
       I->part_cap = np * qdwf::kMaxRhs;
       I->part = (double*)dev_alloc(I->part_cap * sizeof(double));

@@ -237,3 +237,34 @@ def test_folded_x_update_matches_every_step_update(monkeypatch, prop_case, solve
     assert abs(st1["total_iter"] - st0["total_iter"]) <= 0.02 * st0["total_iter"], (st0, st1)
     assert abs(st1["reliable_updates"] - st0["reliable_updates"]) <= 1, (st0, st1)
     assert rel(s1, s0) < 1e-7
+
+
+@pytest.mark.parametrize("solver", ["mixed16", "mixed"])
+def test_cg_step_on_sixteen_site_blocks(monkeypatch, prop_case, solver):
+    """The CG step on 16-site blocks (QLAT_DWF_STEP_SB=16: t = K p on the
+    matrix cores) against the default 32-site step (t = K p row by row in
+    FP32 on the vector units): the same solutions to FP32 rounding, every
+    column at the FP64 true residual and on the oracle's solution."""
+    print(f"\n[gpu] CG step on 16-site blocks ({solver}) 4^3x8 Ls8", flush=True)
+    monkeypatch.setenv("QLAT_DWF_RHS_TILE", "5")
+    site, U, src, exact = prop_case
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    fa = q.FermionAction(mass=0.05, ls=8, m5=1.8, mobius_scale=2.0)
+    out = {}
+    for sb in ("32", "16"):
+        monkeypatch.setenv("QLAT_DWF_STEP_SB", sb)
+        inv = q.InverterDomainWall(gf=gf, fa=fa, solver=solver)
+        inv.set_stop_rsd(1e-9)
+        p = q.Prop(geo)
+        p.data = src
+        sol = inv * p
+        st = inv.stats()
+        assert st["true_residual"] <= 1e-9, sb
+        for j in range(9):
+            assert rel(sol.data[:, :, j], exact[:, :, j]) < 1e-6, (sb, j)
+        out[sb] = (sol.data.copy(), st)
+    (s32, st32), (s16, st16) = out["32"], out["16"]
+    assert abs(st16["total_iter"] - st32["total_iter"]) <= 0.02 * st32["total_iter"]
+    assert rel(s16, s32) < 1e-7

@@ -116,3 +116,47 @@ def test_mixed_solvers_reach_tolerance(pair, solver):
     assert st["spinor_bits"] == (16 if solver == "mixed16" else 32)
     assert st["hop_arith_bits"] == (16 if solver == "mixed16" else 32)
     assert rel(sol.data, exact) < 1e-6
+
+
+@pytest.mark.parametrize("z", [False, True], ids=["moebius_scale4", "zmoebius"])
+def test_ls24_eight_site_blocks(monkeypatch, z):
+    """Ls 24 on blocks of 8 sites x 24 rows (3 waves; the s-matrix phase as
+    products of 8 sites x 2 reals on the matrix cores, complex matrices
+    with the partner real's operand): the FP32- and FP16-storage operators
+    against the oracle and the mixed16 solve at the FP64 true residual."""
+    monkeypatch.setenv("QLAT_DWF_TP_SB", "8")
+    monkeypatch.setenv("QLAT_DWF_RHS_TILE", "4")
+    ls = 24
+    print(f"\n[gpu] Ls 24 on 8-site blocks, {'zMoebius' if z else 'Moebius scale 4'}", flush=True)
+    U = rand_gauge(SITE, 0.4, 760 + z)
+    kw = dict(mass=0.05, ls=ls, m5=1.8, mobius_scale=1.0 if z else 4.0, omega=zmobius_omega(ls) if z else None)
+    geo = q.Geometry(SITE)
+    gf = q.GaugeField(geo)
+    gf.data = U
+    inv = q.InverterDomainWall(gf=gf, fa=q.FermionAction(**kw))
+    orc = O.OracleInverter(SITE, U, **kw)
+    x = rand_field((geo.total_volume() // 2, ls, 12), 761)
+    f = q.FermionField5d(geo, ls, 1)
+    f.data = x
+    ref = orc.apply_half("hermop_sym2", x, 1)
+    inv.set_solver("mixed")
+    e32 = rel(q.multiply_hermop_sym2_fp32(None, f, inv).data, ref)
+    inv.set_solver("mixed16")
+    e16 = rel(q.multiply_hermop_sym2_fp32(None, f, inv).data, ref)
+    assert e32 < 2e-6
+    assert 1e-5 < e16 < 3e-3
+    orc.set_params(1e-12, 200, 300)
+    src = q.FermionField4d(geo)
+    src.data = rand_field((geo.total_volume(), 12), 762)
+    exact, _ = orc.invert_4d(src.data)
+    p = q.Prop(geo)
+    p.data = np.zeros(p.data.shape, dtype=np.complex128)
+    p.data[:, :, 0] = src.data
+    p.data[:, :, 5] = 0.5j * src.data
+    inv.set_stop_rsd(1e-9)
+    sol = inv * p
+    st = inv.stats()
+    assert st["sites_per_block"] == 8 and st["spinor_bits"] == 16
+    assert st["true_residual"] <= 1e-9
+    assert rel(sol.data[:, :, 0], exact) < 1e-6
+    assert rel(sol.data[:, :, 5], 0.5j * exact) < 1e-6
