@@ -234,6 +234,11 @@ def bench_config_e(args):
             f"true residual {st['true_residual']:.2e}, tile {st['rhs_tile']}")
         del src, sol
         torch.cuda.empty_cache()
+    if args.skip_force:
+        print(json.dumps({"propagator": {"seconds": t_prop, "iters": st["total_iter"], "rhs_tile": st["rhs_tile"],
+                                         "sites_per_block": st["sites_per_block"],
+                                         "true_residual": st["true_residual"]}}), flush=True)
+        return
     rng = np.random.default_rng(cfg["seed"] + 2)
     phi = torch.from_numpy(rng.standard_normal((V // 2, ls, 12)) + 1j * rng.standard_normal((V // 2, ls, 12))).cuda()
     force = torch.zeros((V, 4, 3, 3), dtype=torch.complex128, device="cuda")
@@ -283,6 +288,7 @@ def main():
     ap.add_argument("--kernel-iters", type=int, default=50)
     ap.add_argument("--force-only", action="store_true",
                     help="config E: one fermion-force evaluation without the propagator (for profiler runs)")
+    ap.add_argument("--skip-force", action="store_true", help="config E: the propagator only")
     ap.add_argument("--force-stop-rsd", type=float, default=1e-10, help="config E: stop_rsd of the force's CG")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the kernels and stop (for profiler runs)")
@@ -335,6 +341,11 @@ def main():
     step_ms = q.bench_kernel(inv, 6, prec_tp, max(1, args.kernel_iters // 5))
     hermtp_ms = q.bench_kernel(inv, 7, prec_tp, max(1, args.kernel_iters // 10))
     site_bytes = 52 if prec_tp == 16 else 96
+    Vb = 1
+    for n_ in site:
+        Vb *= n_
+    link_bytes = 144 if (prec_tp == 16 and inv.stats()["hop_arith_bits"] == 16) else 288
+    iter_bytes = tile * (Vb // 2) * ls * (11 * site_bytes + 96 + 4 * site_bytes + 384) + 4 * Vb * link_bytes
     bm5s = hop_m5s_bytes(site, ls, tile, site_bytes, 2 if arith == 16 else 4)
     gbs_m5s = bm5s / (hm5s_ms * 1e-3) / 1e9
     log(f"three-phase ({prec_tp}-bit storage, FP{arith} hop, {tile} columns, {sb} sites/block): hop+s-matrix {hm5s_ms:.3f} ms "
@@ -489,6 +500,14 @@ def main():
             "three_phase": {"storage_bits": prec_tp, "columns": tile, "sites_per_block": sb, "hop_m5_ms": hm5s_ms,
                             "hop_m5_gbps": gbs_m5s, "hop_arith_bits": arith, "cg_step_ms": step_ms,
                             "hermop_dot_ms": hermtp_ms},
+            # one CG iteration of the tile: Mpc^+ Mpc with its dots (4 hop kernels: read
+            # t, b, p, y, z, p and r once each, write b, y, z, ap; links 4 times) and the
+            # step with x every step (read p, ap, r, x; write r, x, p', t)
+            "iteration": {
+                "algorithmic_bytes": iter_bytes, "ms": hermtp_ms + step_ms,
+                "tbps": iter_bytes / ((hermtp_ms + step_ms) * 1e-3) / 1e12,
+                "frac_of_hbm_peak": iter_bytes / ((hermtp_ms + step_ms) * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            },
         },
         "roofline": {
             "kernel": f"k_hop_m5s<{'Half16' if prec_tp == 16 else 'float'}, {sb}, {ls}, HM={1 if arith == 16 else 0}> "

@@ -207,7 +207,8 @@ __device__ __forceinline__ void smat_mfma8(float4 (*hr)[6][8], const cplx<float>
 {
   constexpr int MT = (LSMAX + 15) / 16;
   constexpr int KB = LSMAX / 4;
-  constexpr int UMAX = 2;  // float4 columns per wave: 6 over ls/8 = 3 waves
+  // float4 columns per wave: 6 over ls/8 waves (2 waves at Ls 16, 3 at 24)
+  constexpr int UMAX = LSMAX <= 16 ? 3 : 2;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int li = lane & 15, lq = lane >> 4;
   const int ns = li & 7, rr = li >> 3;  // this lane's site and real within the pair
@@ -531,15 +532,17 @@ static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
 // block, which keeps a block's LDS (rows x 12 complex x sites) below 40 KB
 bool three_phase_ok(int ls, int sb)
 {
-  return ls >= 6 && ls <= 24 && (sb == 16 || (sb == 32 && ls <= 16) || (sb == 8 && ls == 24)) && (sb * ls) % 64 == 0;
+  return ls >= 6 && ls <= 24 && (sb == 16 || (sb == 32 && ls <= 16) || (sb == 8 && (ls == 16 || ls == 24))) &&
+         (sb * ls) % 64 == 0;
 }
 int64_t hop_m5s_blocks(const LatGeo& g, int sb) { return (g.vh + sb - 1) / sb; }
 
 template <class ST, int SB, int H2>
 static void launch_hop_m5s_sb(const HopM5Args& a, bool dag, hipStream_t st)
 {
-  if constexpr (SB == 8) {  // Ls 24 only
-    launch_hop_m5s_ls<ST, SB, 24, H2>(a, dag, st);
+  if constexpr (SB == 8) {  // Ls 16 and 24 only
+    if (a.ls == 16) launch_hop_m5s_ls<ST, SB, 16, H2>(a, dag, st);
+    else launch_hop_m5s_ls<ST, SB, 24, H2>(a, dag, st);
     return;
   }
   if (a.ls <= 8) launch_hop_m5s_ls<ST, SB, 8, H2>(a, dag, st);

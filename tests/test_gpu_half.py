@@ -268,3 +268,40 @@ def test_cg_step_on_sixteen_site_blocks(monkeypatch, prop_case, solver):
     (s32, st32), (s16, st16) = out["32"], out["16"]
     assert abs(st16["total_iter"] - st32["total_iter"]) <= 0.02 * st32["total_iter"]
     assert rel(s16, s32) < 1e-7
+
+
+@pytest.mark.parametrize("z", [False, True], ids=["moebius", "zmoebius"])
+def test_ls16_eight_site_blocks(monkeypatch, z):
+    """Ls 16 on blocks of 8 sites x 16 rows (2 waves; the s-matrix phase as
+    matrix-core products of 8 sites x 2 reals, three float4 columns per
+    wave): FP32- and FP16-storage operators against the oracle, and a
+    mixed16 propagator column at the FP64 true residual."""
+    monkeypatch.setenv("QLAT_DWF_TP_SB", "8")
+    monkeypatch.setenv("QLAT_DWF_RHS_TILE", "4")
+    ls = 16
+    print(f"\n[gpu] Ls 16 on 8-site blocks {'zMoebius' if z else 'Moebius'}", flush=True)
+    site = (4, 4, 4, 8)
+    U = rand_gauge(site, 0.4, 45 + z)
+    om = [1.1 * 0.92 ** s for s in range(ls - 4)] + [complex(0.2, -0.02), complex(0.2, 0.02),
+                                                       complex(0.12, -0.03), complex(0.12, 0.03)]
+    geo, inv, orc = make(site, U, mass=0.05, ls=ls, omega=om if z else None)
+    x = rand_field((geo.total_volume() // 2, ls, 12), 650)
+    ref = orc.apply_half("hermop_sym2", x, 1)
+    inv.set_solver("mixed")
+    assert rel(q.multiply_hermop_sym2_fp32(None, half(geo, ls, 1, x), inv).data, ref) < 2e-6
+    inv.set_solver("mixed16")
+    e16 = rel(q.multiply_hermop_sym2_fp32(None, half(geo, ls, 1, x), inv).data, ref)
+    assert 1e-5 < e16 < 3e-3
+    src = q.FermionField4d(geo)
+    src.data = rand_field((geo.total_volume(), 12), 651)
+    orc.set_params(1e-12, 200, 300)
+    exact, _ = orc.invert_4d(src.data)
+    p = q.Prop(geo)
+    p.data = np.zeros(p.data.shape, dtype=np.complex128)
+    p.data[:, :, 3] = src.data
+    inv.set_stop_rsd(1e-9)
+    sol = inv * p
+    st = inv.stats()
+    assert st["sites_per_block"] == 8
+    assert st["true_residual"] <= 1e-9
+    assert rel(sol.data[:, :, 3], exact) < 1e-6
