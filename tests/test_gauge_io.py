@@ -87,3 +87,21 @@ def test_three_row_little_endian_file_and_errors(tmp_path):
     open(bad, "wb").write(head.replace("4D_SU3_GAUGE_3x3", "4D_U1").encode() + data)
     with pytest.raises(ValueError):
         gf.load(bad)
+
+
+def test_checksum_of_the_unit_field_in_closed_form(tmp_path):
+    """field_simple_checksum (field-io.h:9-25): the sum modulo 2^32 of the
+    32-bit words of the two stored rows in native (little-endian) order,
+    qcd.h:167-168.  For the unit field every link stores rows (1,0,0),
+    (0,1,0): two doubles equal to 1.0, whose words are 0x00000000 and
+    0x3FF00000, so the checksum is 4 V * 2 * 0x3FF00000 mod 2^32."""
+    site = (4, 2, 4, 4)
+    geo = q.Geometry(site)
+    gf = q.GaugeField(geo).set_unit()
+    path = str(tmp_path / "unit.nersc")
+    gf.save(path)
+    h = q.read_gauge_field_header(path)
+    V = geo.total_volume()
+    assert h.simple_checksum == (4 * V * 2 * 0x3FF00000) % (1 << 32)
+    text = open(path, "rb").read().split(b"END_HEADER")[0].decode()
+    assert f"CHECKSUM = {(4 * V * 2 * 0x3FF00000) % (1 << 32):08x}" in text
