@@ -17,15 +17,20 @@ pytestmark = pytest.mark.gpu
 def test_propagator_from_a_loaded_configuration(tmp_path):
     site = (4, 4, 4, 8)
     print("\n[gpu] NERSC round trip, then a Ls 24 propagator on the loaded field", flush=True)
-    U = rand_gauge(site, 0.4, 901)
+    # the file holds SU(3) links; the antiperiodic twist (qcd.h:331-346) is
+    # applied after loading, as in the reference's scripts
+    U = rand_gauge(site, 0.4, 901, antiperiodic=False)
     geo = q.Geometry(site)
     gf = q.GaugeField(geo)
-    gf.data = U
+    gf.data = U.copy()
     path = str(tmp_path / "cfg.nersc")
     gf.save(path)
     g2 = q.GaugeField()
     g2.load(path)
     assert np.max(np.abs(g2.data - U)) < 1e-13
+    for g in (gf, g2):
+        g.twist_boundary_at_boundary(-0.5, 3)
+    U = gf.data
     kw = dict(mass=0.05, ls=24, m5=1.8, mobius_scale=4.0)
     src = q.Prop(geo)
     src.data = np.zeros(src.data.shape, dtype=np.complex128)
