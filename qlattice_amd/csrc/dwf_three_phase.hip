@@ -92,9 +92,10 @@ __device__ __forceinline__ void smat_row(const float4 (*hr)[6][SB], const cplx<f
 
 // The s-matrix phase of a block of 16 sites x ls rows (ls a multiple of 4,
 // ls/4 waves) on the matrix cores: out(s) = sum_t K_chi[s][t] h(t) is, per
-// chirality and real component r of the 12 of that chirality, a product
-// (16 x 16 rows s) x (ls columns t) x (16 sites) done as ls/4 steps of
-// v_mfma_f32_16x16x4_f32 (full FP32 products and sums).  Lane l of a wave
+// chirality and real component r of the 12 of that chirality, the product
+// of the matrix (rows s, in tiles of 16; columns t) with the (t x 16 sites)
+// block of that component, done as ls/4 steps of v_mfma_f32_16x16x4_f32
+// (full FP32 products and sums).  Lane l of a wave
 // supplies A[s = l%16][t = 4 kb + l/16] (the matrix) and B[t = 4 kb + l/16]
 // [site = l%16] (one real of hr), and receives D[s = 4 (l/16) + i][site =
 // l%16], i < 4.  A float4 of hr (reals 4k..4k+3: complex 2k, 2k+1, one
@@ -102,8 +103,9 @@ __device__ __forceinline__ void smat_row(const float4 (*hr)[6][SB], const cplx<f
 // k = w, w + ls/4, ... < 6.  The results are written back over hr (after
 // every wave has read it), and thread (site, row) then reads its own row.
 // km: the block's matrices, staged in LDS (km[chi][s][t]).
-// Complex (zMoebius) matrices take four products per complex pair
-// (re K re h - im K im h, re K im h + im K re h).
+// Complex (zMoebius) matrices take a second product per real component:
+// im K times the partner real of the same complex number (-im K against
+// im h for the real part, +im K against re h for the imaginary part).
 typedef float mf4 __attribute__((ext_vector_type(4)));
 
 template <int LSMAX, bool REAL>
