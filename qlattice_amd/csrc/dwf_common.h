@@ -368,10 +368,22 @@ template <>
 struct SpinorIO<Half16> {
   typedef uint4 slot_t;
   typedef float real;
+  // A 64-site block holds Ls x 208 sixteen-byte units: row s of spinors at
+  // units 192 s + 64 j + lane (j < 3), then the block's Ls x 64 FP32
+  // scales, grouped by 16 sites: the scale of (s, lane) at byte
+  // 64 ((lane / 16) Ls + s) + 4 (lane % 16) of that area, so that four
+  // consecutive rows of 16 sites (one wave of a 16-site block) share two
+  // 128-byte lines.
   static constexpr int kUnits = 208;  // 16-byte units per (64-site block, s)
   __device__ __forceinline__ static unsigned base(int ls, int s, int64_t h)
   {
-    return (((unsigned)h >> 6) * (unsigned)ls + (unsigned)s) * (unsigned)kUnits;
+    return ((unsigned)h >> 6) * (unsigned)ls * (unsigned)kUnits + (unsigned)s * 192u;
+  }
+  __device__ __forceinline__ static unsigned scale_byte(int ls, int s, int64_t h)
+  {
+    const unsigned lane = (unsigned)h & 63u;
+    return (((unsigned)h >> 6) * (unsigned)ls * (unsigned)kUnits + (unsigned)ls * 192u) * 16u +
+           (((lane >> 4) * (unsigned)ls + (unsigned)s) << 6) + ((lane & 15u) << 2);
   }
   struct Raw {
     uint4 q[3];
@@ -387,18 +399,18 @@ struct SpinorIO<Half16> {
   {
     return (uint4*)((char*)f + unit * 16u);
   }
-  __device__ __forceinline__ static const float* scale_at(const uint4* f, unsigned b, unsigned lane)
+  __device__ __forceinline__ static const float* scale_at(const uint4* f, unsigned byte)
   {
-    return (const float*)((const char*)f + (b + 192u) * 16u + lane * 4u);
+    return (const float*)((const char*)f + byte);
   }
-  __device__ __forceinline__ static float* scale_at(uint4* f, unsigned b, unsigned lane)
+  __device__ __forceinline__ static float* scale_at(uint4* f, unsigned byte)
   {
-    return (float*)((char*)f + (b + 192u) * 16u + lane * 4u);
+    return (float*)((char*)f + byte);
   }
   __device__ __forceinline__ static void load_raw(const uint4* __restrict__ f, int ls, int s, int64_t h, Raw& r)
   {
     const unsigned b = base(ls, s, h), lane = (unsigned)h & 63u;
-    r.sc = *scale_at(f, b, lane);
+    r.sc = *scale_at(f, scale_byte(ls, s, h));
 #pragma unroll
     for (int j = 0; j < 3; ++j) r.q[j] = *slot_at(f, b + j * 64u + lane);
   }
@@ -416,7 +428,7 @@ struct SpinorIO<Half16> {
                                               cplx<float> v[12])
   {
     const unsigned b = base(ls, s, h), lane = (unsigned)h & 63u;
-    const float sc = *scale_at(f, b, lane);
+    const float sc = *scale_at(f, scale_byte(ls, s, h));
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const uint4 q = *slot_at(f, b + j * 64u + lane);
@@ -455,7 +467,7 @@ struct SpinorIO<Half16> {
       v[4 * j + 2] = unpack_h2(q.z, m);
       v[4 * j + 3] = unpack_h2(q.w, m);
     }
-    *scale_at(f, b, lane) = m;
+    *scale_at(f, scale_byte(ls, s, h)) = m;
   }
   __device__ __forceinline__ static void store_q(uint4* __restrict__ f, int ls, int s, int64_t h, cplx<float> v[12])
   {
