@@ -205,6 +205,7 @@ struct qlat_dwf_inverter {
   // arithmetic).  Measured at 24^3x64 Ls12, b = K H t over 4 columns:
   // 1.063 ms against 1.177 ms
   int h2_hop = 1;
+  int plain64 = 1;  // plain FP16 hop on 64-site blocks (QLAT_DWF_PLAIN64)
   cplx<double>* mat64 = nullptr;  // [MAT_COUNT][2][ls][ls]
   cplx<float>* mat32 = nullptr;
   cplx<double>* beta64 = nullptr;  // [2][ls]: ones, c~
@@ -1284,6 +1285,7 @@ struct ThreePhase {
     a.links = I.l32;
     a.links16 = half() ? I.l16 : nullptr;
     a.hop_mode = I.h2_hop;
+    a.plain64 = I.plain64;
     a.sb = sb;
     a.site_major = I.site_major;
     a.cg = cg;
@@ -2535,6 +2537,7 @@ int qlat_dwf_inverter_create(qlat_dwf_inverter** out, const int total_site[4], c
       I->tp_sb = (v == 8 || v == 16) ? v : 32;
     }
     if (const char* e = std::getenv("QLAT_DWF_SITE_MAJOR")) I->site_major = std::atoi(e) != 0;
+    if (const char* e = std::getenv("QLAT_DWF_PLAIN64")) I->plain64 = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_FP16_MODE")) I->fp16_mode = std::max(1, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("QLAT_DWF_XDEFER")) I->x_defer = std::atoi(e) != 0;
     if (const char* e = std::getenv("QLAT_DWF_STEP_SB")) I->step_sb = std::atoi(e) == 16 ? 16 : 32;

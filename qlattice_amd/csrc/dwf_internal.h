@@ -138,10 +138,11 @@ struct HopM5Args {
   const void* dotr;
   int64_t dotr_fstride;
   // three-phase variant: the s-matrix of each column (mats[0] null: plain
-  // hop, no s-matrix), sites per block (16 or 32), block order
+  // hop, no s-matrix), sites per block (8, 16 or 32), block order
   const cplx<float>* mats[kMaxRhs];
   int sb;
   int site_major;
+  int plain64;  // the plain FP16-arithmetic hop on 64-site blocks (Ls <= 16)
   const CgState* cg;  // optional: blocks of columns with cg->mode != 0 do nothing
 };
 

@@ -322,7 +322,17 @@ k_hop_m5s(HopM5Args a)
   // links of the block's sites: FP32 complex, or (H2) the FP16 forms
   // la = (re, -im), lb = (im, re) of hop_term_h2
   __shared__ float2 sl[H2 ? 1 : 72][SB];
-  __shared__ h16x2 sla[H2 ? 72 : 1][SB], slb[H2 ? 72 : 1][SB];
+  // With FP16 arithmetic and an s-matrix phase the links and the hop
+  // results (hr below) share one LDS area (21 KB per 16-site block at Ls 12
+  // instead of 30 KB: 7 blocks per CU instead of 5); the hop phase ends at a
+  // barrier before the results go in
+  constexpr bool ALIAS = H2 && SMAT;
+  constexpr unsigned kLinkBytes = 2u * 72u * SB * 4u;
+  constexpr unsigned kHrBytes = (unsigned)LSMAX * 6u * SB * 16u;
+  __shared__ __attribute__((aligned(16))) unsigned char pool[ALIAS ? (kLinkBytes > kHrBytes ? kLinkBytes : kHrBytes) : 16];
+  __shared__ h16x2 sla_own[H2 && !ALIAS ? 72 : 1][SB], slb_own[H2 && !ALIAS ? 72 : 1][SB];
+  h16x2(*sla)[SB] = ALIAS ? (h16x2(*)[SB])pool : sla_own;
+  h16x2(*slb)[SB] = ALIAS ? (h16x2(*)[SB])(pool + 72u * SB * 4u) : slb_own;
   // block numbers are below 2^32: 32-bit divisions
   const unsigned lbc = (unsigned)xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
   const int ncol = a.cols.n;
@@ -426,7 +436,9 @@ k_hop_m5s(HopM5Args a)
   const float sc = -0.5f * a.alpha;
   cplx<float> v[12];
   if constexpr (SMAT) {
-    __shared__ float4 hr[LSMAX][6][SB];
+    __shared__ float4 hr_own[ALIAS ? 1 : LSMAX][ALIAS ? 1 : 6][ALIAS ? 1 : SB];
+    float4(*hr)[6][SB] = ALIAS ? (float4(*)[6][SB])pool : (float4(*)[6][SB])hr_own;
+    if constexpr (ALIAS) __syncthreads();  // every wave is done with the links
     if (active && row < ls) {
 #pragma unroll
       for (int k = 0; k < 6; ++k)
@@ -496,6 +508,19 @@ k_hop_m5s(HopM5Args a)
   }
 }
 
+// plain hop: out = alpha H in + beta x (no s-matrix phase)
+template <class ST, int SB, int LSMAX, int H2>
+static void launch_hop_plain_ls(const HopM5Args& a, bool dag, hipStream_t st)
+{
+  const dim3 grid((unsigned)((a.geo.vh + SB - 1) / SB * a.cols.n));
+  const dim3 blk(SB * a.ls);
+  const bool xpay = a.x != nullptr;
+  if (!dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, false, true, false, false, H2>), grid, blk, 0, st, a);
+  if (!dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, true, true, false, false, H2>), grid, blk, 0, st, a);
+  if (dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, false, true, false, false, H2>), grid, blk, 0, st, a);
+  if (dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, true, true, false, false, H2>), grid, blk, 0, st, a);
+}
+
 template <class ST, int SB, int LSMAX, int H2>
 static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
 {
@@ -503,11 +528,8 @@ static void launch_hop_m5s_ls(const HopM5Args& a, bool dag, hipStream_t st)
   const dim3 blk(SB * a.ls);
   const bool xpay = a.x != nullptr;
 #define QDWF_L(D, X, R, DT) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, D, X, R, DT, true, H2>), grid, blk, 0, st, a)
-  if (!a.mats[0]) {  // plain hop: out = alpha H in + beta x
-    if (!dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, false, true, false, false, H2>), grid, blk, 0, st, a);
-    if (!dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, false, true, true, false, false, H2>), grid, blk, 0, st, a);
-    if (dag && !xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, false, true, false, false, H2>), grid, blk, 0, st, a);
-    if (dag && xpay) hipLaunchKernelGGL((k_hop_m5s<ST, SB, LSMAX, true, true, true, false, false, H2>), grid, blk, 0, st, a);
+  if (!a.mats[0]) {
+    launch_hop_plain_ls<ST, SB, LSMAX, H2>(a, dag, st);
     return;
   }
   if (a.dotp) {
@@ -557,6 +579,20 @@ bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
 {
   if (!three_phase_ok(a.ls, a.sb)) return false;
   const bool h2 = half && a.links16 && a.hop_mode != 0;
+  if (h2 && a.plain64 && !a.mats[0] && !a.dotp && a.ls <= 16) {
+    // The plain hop (no s-matrix phase, no barrier after the hop) on blocks
+    // of 64 sites: a wave then holds 64 sites of one row, so that a
+    // neighbour gather shifted by a few sites touches 9 cache lines per 16
+    // bytes of spinor instead of 12 (blocks of 16 sites x 4 rows).  At
+    // 24^3x64 Ls12 over 4 columns: 77.7M L1-to-L2 requests against 98.8M,
+    // 8.5% faster.  The s-matrix kernels stay on 16-site blocks: with 32 or
+    // 64 sites their block-wide barriers leave fewer requests in flight,
+    // and they ran 10% and 40% slower.
+    if (a.ls <= 8) launch_hop_plain_ls<Half16, 64, 8, 1>(a, dag, st);
+    else if (a.ls <= 12) launch_hop_plain_ls<Half16, 64, 12, 1>(a, dag, st);
+    else launch_hop_plain_ls<Half16, 64, 16, 1>(a, dag, st);
+    return true;
+  }
   if (h2) {
     if (a.sb == 8) launch_hop_m5s_sb<Half16, 8, 1>(a, dag, st);
     else if (a.sb == 16) launch_hop_m5s_sb<Half16, 16, 1>(a, dag, st);

@@ -65,6 +65,23 @@ def test_fp16_storage_hermop_matches_oracle(monkeypatch, ls, omega, h2):
         assert 1e-5 < err < 3e-3, err
 
 
+@pytest.mark.parametrize("ls", [6, 8, 12, 16])
+def test_plain_hop_block_shapes_agree_bitwise(monkeypatch, ls):
+    """The plain FP16 hop pass (y = p - H b) runs on 64-site blocks by
+    default and on the three-phase block shape with QLAT_DWF_PLAIN64=0: the
+    same arithmetic per (site, s), so the operator is bit-identical."""
+    print(f"\n[gpu] plain hop block shapes Ls{ls}", flush=True)
+    site = (4, 4, 4, 8)
+    U = rand_gauge(site, 0.4, 44)
+    outs = []
+    for v in ("1", "0"):
+        monkeypatch.setenv("QLAT_DWF_PLAIN64", v)
+        geo, inv, _ = make(site, U, mass=0.05, ls=ls, omega=ZMOBIUS_OMEGA if ls == 12 else None)
+        x = rand_field((geo.total_volume() // 2, ls, 12), 604)
+        outs.append(q.multiply_hermop_sym2_fp32(None, half(geo, ls, 1, x), inv).data)
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_fp16_storage_zero_and_scale_invariance():
     """A zero field stays exactly zero (scale 0), and scaling the input by a
     power of two scales the output exactly (the per-site scale absorbs it)."""
