@@ -102,14 +102,15 @@ __device__ __forceinline__ void smat_row(const float4 (*hr)[6][SB], const cplx<f
 // chirality) feeds four products at once; wave w takes the float4 columns
 // k = w, w + ls/4, ... < 6.  The results are written back over hr (after
 // every wave has read it), and thread (site, row) then reads its own row.
-// km: the block's matrices, staged in LDS (km[chi][s][t]).
+// km(chi, s, t): the block's matrices, staged in LDS (complex, or their
+// real parts for real matrices).
 // Complex (zMoebius) matrices take a second product per real component:
 // im K times the partner real of the same complex number (-im K against
 // im h for the real part, +im K against re h for the imaginary part).
 typedef float mf4 __attribute__((ext_vector_type(4)));
 
-template <int LSMAX, bool REAL>
-__device__ __forceinline__ void smat_mfma16(float4 (*hr)[6][16], const cplx<float> (*km)[LSMAX][LSMAX], int ls, int site,
+template <int LSMAX, bool REAL, class KM>
+__device__ __forceinline__ void smat_mfma16(float4 (*hr)[6][16], KM km, int ls, int site,
                                             int row, bool act, cplx<float> v[12])
 {
   constexpr int MT = (LSMAX + 15) / 16;  // 16-row tiles of s
@@ -131,7 +132,7 @@ __device__ __forceinline__ void smat_mfma16(float4 (*hr)[6][16], const cplx<floa
       for (int kb = 0; kb < KB; ++kb) {
         const int s = mt * 16 + li, t = 4 * kb + lq;
         cplx<float> m = cmk<float>(0.f, 0.f);
-        if (kb < nkb && s < ls) m = km[chi][s][t];
+        if (kb < nkb && s < ls) m = km(chi, s, t);
         are[chi][mt][kb] = m.re;
         aim[chi][mt][kb] = m.im;
       }
@@ -203,8 +204,8 @@ __device__ __forceinline__ float f4c(const float4& q, int c)
 {
   return c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w;
 }
-template <int LSMAX, bool REAL>
-__device__ __forceinline__ void smat_mfma8(float4 (*hr)[6][8], const cplx<float> (*km)[LSMAX][LSMAX], int ls, int site,
+template <int LSMAX, bool REAL, class KM>
+__device__ __forceinline__ void smat_mfma8(float4 (*hr)[6][8], KM km, int ls, int site,
                                            int row, bool act, cplx<float> v[12])
 {
   constexpr int MT = (LSMAX + 15) / 16;
@@ -225,7 +226,7 @@ __device__ __forceinline__ void smat_mfma8(float4 (*hr)[6][8], const cplx<float>
       for (int kb = 0; kb < KB; ++kb) {
         const int s = mt * 16 + li, t = 4 * kb + lq;
         cplx<float> m = cmk<float>(0.f, 0.f);
-        if (kb < nkb && s < ls) m = km[chi][s][t];
+        if (kb < nkb && s < ls) m = km(chi, s, t);
         are[chi][mt][kb] = m.re;
         aim[chi][mt][kb] = m.im;
       }
@@ -321,7 +322,7 @@ k_hop_m5s(HopM5Args a)
   static_assert(!H2 || std::is_same<ST, Half16>::value, "FP16 arithmetic needs FP16 storage");
   // links of the block's sites: FP32 complex, or (H2) the FP16 forms
   // la = (re, -im), lb = (im, re) of hop_term_h2
-  __shared__ float2 sl[H2 ? 1 : 72][SB];
+  __shared__ float2 sl[H2 ? 1 : 72][H2 ? 1 : SB];
   // With FP16 arithmetic and an s-matrix phase the links and the hop
   // results (hr below) share one LDS area (21 KB per 16-site block at Ls 12
   // instead of 30 KB: 7 blocks per CU instead of 5); the hop phase ends at a
@@ -330,9 +331,10 @@ k_hop_m5s(HopM5Args a)
   constexpr unsigned kLinkBytes = 2u * 72u * SB * 4u;
   constexpr unsigned kHrBytes = (unsigned)LSMAX * 6u * SB * 16u;
   __shared__ __attribute__((aligned(16))) unsigned char pool[ALIAS ? (kLinkBytes > kHrBytes ? kLinkBytes : kHrBytes) : 16];
-  __shared__ h16x2 sla_own[H2 && !ALIAS ? 72 : 1][SB], slb_own[H2 && !ALIAS ? 72 : 1][SB];
-  h16x2(*sla)[SB] = ALIAS ? (h16x2(*)[SB])pool : sla_own;
-  h16x2(*slb)[SB] = ALIAS ? (h16x2(*)[SB])(pool + 72u * SB * 4u) : slb_own;
+  constexpr int LO = H2 && !ALIAS ? 72 : 1, LS_ = H2 && !ALIAS ? SB : 1;
+  __shared__ h16x2 sla_own[LO][LS_], slb_own[LO][LS_];
+  h16x2(*sla)[SB] = ALIAS ? (h16x2(*)[SB])pool : (h16x2(*)[SB])sla_own;
+  h16x2(*slb)[SB] = ALIAS ? (h16x2(*)[SB])(pool + 72u * SB * 4u) : (h16x2(*)[SB])slb_own;
   // block numbers are below 2^32: 32-bit divisions
   const unsigned lbc = (unsigned)xcd_logical_block(blockIdx.x, (int64_t)gridDim.x);
   const int ncol = a.cols.n;
@@ -362,8 +364,21 @@ k_hop_m5s(HopM5Args a)
   }
   // the block's s-matrices (both chiralities) in LDS, loaded while the
   // links and neighbour offsets are staged
-  __shared__ cplx<float> km[SMAT ? 2 : 1][SMAT ? LSMAX : 1][SMAT ? LSMAX : 1];
-  if constexpr (SMAT) stage_km<LSMAX>(km, pick(a.mats, ci), a.ls);
+  // (real matrices on the matrix-core paths: their real parts only, 1.2 KB
+  // less per block at Ls 12, which lets 8 blocks of 16 sites share a CU)
+  constexpr bool KMR = SMAT && REAL && (SB == 16 || SB == 8);
+  constexpr int KC = SMAT && !KMR ? LSMAX : 1, KRN = KMR ? LSMAX : 1;
+  __shared__ cplx<float> km[SMAT && !KMR ? 2 : 1][KC][KC];
+  __shared__ float kmr[KMR ? 2 : 1][KRN][KRN];
+  if constexpr (KMR) {
+    const cplx<float>* __restrict__ M = pick(a.mats, ci);
+    for (int i = threadIdx.x; i < 2 * LSMAX * LSMAX; i += blockDim.x) {
+      const int chi = i / (LSMAX * LSMAX), r = (i / LSMAX) % LSMAX, t = i % LSMAX;
+      if (r < a.ls && t < a.ls) kmr[chi][r][t] = M[(chi * a.ls + r) * a.ls + t].re;
+    }
+  } else if constexpr (SMAT) {
+    stage_km<LSMAX>(km, pick(a.mats, ci), a.ls);
+  }
   // neighbour indices once per site, shared by the Ls rows (thread (site,
   // d) of the first 8 SB forms neighbour d of its site)
   __shared__ unsigned nbs[8][SB];
@@ -447,9 +462,21 @@ k_hop_m5s(HopM5Args a)
     }
     if constexpr (SB == 16) {
       // on the matrix cores (ls is a multiple of 4 here: whole waves)
-      smat_mfma16<LSMAX, REAL>(hr, km, ls, site, row, active && row < ls, v);
+      if constexpr (KMR)
+        smat_mfma16<LSMAX, REAL>(
+            hr, [&](int c, int r, int t) { return cmk<float>(kmr[c][r][t], 0.f); }, ls, site, row,
+            active && row < ls, v);
+      else
+        smat_mfma16<LSMAX, REAL>(
+            hr, [&](int c, int r, int t) { return km[c][r][t]; }, ls, site, row, active && row < ls, v);
     } else if constexpr (SB == 8) {
-      smat_mfma8<LSMAX, REAL>(hr, km, ls, site, row, active && row < ls, v);
+      if constexpr (KMR)
+        smat_mfma8<LSMAX, REAL>(
+            hr, [&](int c, int r, int t) { return cmk<float>(kmr[c][r][t], 0.f); }, ls, site, row,
+            active && row < ls, v);
+      else
+        smat_mfma8<LSMAX, REAL>(
+            hr, [&](int c, int r, int t) { return km[c][r][t]; }, ls, site, row, active && row < ls, v);
     } else {
       // thread (site, s) forms its own output row s for all 12 components
       __syncthreads();
@@ -701,7 +728,8 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
   if constexpr (SB == 16) {
     // t = Mat p on the matrix cores (16-site blocks: whole waves)
     cplx<float> v[12];
-    smat_mfma16<LSMAX, REAL>(hr, km, ls, site, row, active && row < ls, v);
+    smat_mfma16<LSMAX, REAL>(
+        hr, [&](int c, int r, int t) { return km[c][r][t]; }, ls, site, row, active && row < ls, v);
     if (active && row < ls) store_q<ST>((slot_t*)a.t + coff, ls, row, h, v);
   } else {
     __syncthreads();
