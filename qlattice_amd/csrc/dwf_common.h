@@ -319,6 +319,18 @@ struct SpinorIO<float> {
     for (int k = 0; k < 6; ++k)
       store_nt(&p[k * 64], make_float4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im));
   }
+  __device__ __forceinline__ static void load_stream(const float4* __restrict__ f, int ls, int s, int64_t h,
+                                                     cplx<float> v[12])
+  {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const float4* p = f + sidx<6>(ls, s, 0, h);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const v4f a = __builtin_nontemporal_load((const v4f*)&p[k * 64]);
+      v[2 * k] = cmk<float>(a.x, a.y);
+      v[2 * k + 1] = cmk<float>(a.z, a.w);
+    }
+  }
   // raw loads (issued together, decoded later: more loads in flight per wave)
   struct Raw {
     float4 q[6];
@@ -432,6 +444,21 @@ struct SpinorIO<Half16> {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const uint4 q = *slot_at(f, b + j * 64u + lane);
+      v[4 * j + 0] = unpack_h2(q.x, sc);
+      v[4 * j + 1] = unpack_h2(q.y, sc);
+      v[4 * j + 2] = unpack_h2(q.z, sc);
+      v[4 * j + 3] = unpack_h2(q.w, sc);
+    }
+  }
+  // the same as load with non-temporal loads (streaming kernels)
+  __device__ __forceinline__ static void load_stream(const uint4* __restrict__ f, int ls, int s, int64_t h,
+                                                     cplx<float> v[12])
+  {
+    const unsigned b = base(ls, s, h), lane = (unsigned)h & 63u;
+    const float sc = __builtin_nontemporal_load(scale_at(f, scale_byte(ls, s, h)));
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const u32x4 q = __builtin_nontemporal_load((const u32x4*)slot_at(f, b + j * 64u + lane));
       v[4 * j + 0] = unpack_h2(q.x, sc);
       v[4 * j + 1] = unpack_h2(q.y, sc);
       v[4 * j + 2] = unpack_h2(q.z, sc);

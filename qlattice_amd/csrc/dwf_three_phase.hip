@@ -692,33 +692,34 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
   }
   slot_t* pout = (slot_t*)(a.p_out ? a.p_out : a.p) + coff;  // may be a.p
   if (active && row < ls) {
+    // every field is streamed once (non-temporal loads and stores)
     cplx<float> pv[12], rv[12];
-    SpinorIO<ST>::load((const slot_t*)a.p + coff, ls, row, h, pv);
-    SpinorIO<SR>::load((const rslot_t*)a.r + roff, ls, row, h, rv);
+    SpinorIO<ST>::load_stream((const slot_t*)a.p + coff, ls, row, h, pv);
+    SpinorIO<SR>::load_stream((const rslot_t*)a.r + roff, ls, row, h, rv);
     if (a.ap) {  // merged form: r -= alpha_r ap
       cplx<float> av[12];
-      SpinorIO<ST>::load((const slot_t*)a.ap + coff, ls, row, h, av);
+      SpinorIO<ST>::load_stream((const slot_t*)a.ap + coff, ls, row, h, av);
 #pragma unroll
       for (int k = 0; k < 12; ++k) rv[k] = cmk<float>(rv[k].re - ar * av[k].re, rv[k].im - ar * av[k].im);
-      store_q<SR>((rslot_t*)a.r + roff, ls, row, h, rv);
+      store_qs<SR>((rslot_t*)a.r + roff, ls, row, h, rv);
     }
     if (UPDX && (al != 0.f || aq != 0.f)) {  // block-uniform
       cplx<float> xv[12];
       xslot_t* x = (xslot_t*)a.x + xoff;
-      SpinorIO<SX>::load(x, ls, row, h, xv);
+      SpinorIO<SX>::load_stream(x, ls, row, h, xv);
       if (aq != 0.f) {  // the previous iteration's deferred alpha q (q in p_out, not yet replaced)
         cplx<float> qv[12];
-        SpinorIO<ST>::load(pout, ls, row, h, qv);
+        SpinorIO<ST>::load_stream(pout, ls, row, h, qv);
 #pragma unroll
         for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + aq * qv[k].re, xv[k].im + aq * qv[k].im);
       }
 #pragma unroll
       for (int k = 0; k < 12; ++k) xv[k] = cmk<float>(xv[k].re + al * pv[k].re, xv[k].im + al * pv[k].im);
-      SpinorIO<SX>::store(x, ls, row, h, xv);
+      store_qs<SX>(x, ls, row, h, xv);
     }
 #pragma unroll
     for (int k = 0; k < 12; ++k) pv[k] = cmk<float>(be * pv[k].re + rv[k].re, be * pv[k].im + rv[k].im);
-    store_q<ST>(pout, ls, row, h, pv);
+    store_qs<ST>(pout, ls, row, h, pv);
 #pragma unroll
     for (int k = 0; k < 6; ++k)
       hr[row][k][site] = make_float4(pv[2 * k].re, pv[2 * k].im, pv[2 * k + 1].re, pv[2 * k + 1].im);
@@ -730,14 +731,14 @@ __global__ void __launch_bounds__(SB * LSMAX) k_cg_step_s(CgStepArgs<float> a)
     cplx<float> v[12];
     smat_mfma16<LSMAX, REAL>(
         hr, [&](int c, int r, int t) { return km[c][r][t]; }, ls, site, row, active && row < ls, v);
-    if (active && row < ls) store_q<ST>((slot_t*)a.t + coff, ls, row, h, v);
+    if (active && row < ls) store_qs<ST>((slot_t*)a.t + coff, ls, row, h, v);
   } else {
     __syncthreads();
     // t = Mat p: thread (site, s) forms its own row s
     if (active && row < ls) {
       cplx<float> v[12];
       smat_row<SB, LSMAX, REAL>(hr, km, ls, site, row, v);
-      store_q<ST>((slot_t*)a.t + coff, ls, row, h, v);
+      store_qs<ST>((slot_t*)a.t + coff, ls, row, h, v);
     }
   }
 }
