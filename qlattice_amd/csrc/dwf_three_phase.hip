@@ -488,22 +488,22 @@ k_hop_m5s(HopM5Args a)
   }
   double dot = 0.0, dotr = 0.0, dota = 0.0;
   if (active && row < ls) {
-    if (XPAY) {
+    if (XPAY) {  // (the streamed inputs are read with non-temporal loads)
       cplx<float> xv[12];
-      SpinorIO<ST>::load((const slot_t*)a.x + coff, ls, row, h, xv);
+      SpinorIO<ST>::load_stream((const slot_t*)a.x + coff, ls, row, h, xv);
 #pragma unroll
       for (int k = 0; k < 12; ++k) v[k] = cmk<float>(v[k].re + a.beta * xv[k].re, v[k].im + a.beta * xv[k].im);
     }
     store_qs<ST>((slot_t*)a.out + coff, ls, row, h, v);
     if (DOT) {
       cplx<float> pv[12];
-      SpinorIO<ST>::load((const slot_t*)a.dotp + coff, ls, row, h, pv);
+      SpinorIO<ST>::load_stream((const slot_t*)a.dotp + coff, ls, row, h, pv);
 #pragma unroll
       for (int k = 0; k < 12; ++k) dot += (double)pv[k].re * v[k].re + (double)pv[k].im * v[k].im;
       if (a.dotr) {
         cplx<float> rv[12];
-        SpinorIO<float>::load((const float4*)a.dotr + (int64_t)pick(a.cols.rhs, ci) * a.dotr_fstride, ls, row, h,
-                              rv);
+        SpinorIO<float>::load_stream((const float4*)a.dotr + (int64_t)pick(a.cols.rhs, ci) * a.dotr_fstride, ls,
+                                     row, h, rv);
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
           dotr += (double)rv[k].re * v[k].re + (double)rv[k].im * v[k].im;
