@@ -324,9 +324,10 @@ k_hop_m5s(HopM5Args a)
   // la = (re, -im), lb = (im, re) of hop_term_h2
   __shared__ float2 sl[H2 ? 1 : 72][H2 ? 1 : SB];
   // With FP16 arithmetic and an s-matrix phase the links and the hop
-  // results (hr below) share one LDS area (21 KB per 16-site block at Ls 12
-  // instead of 30 KB: 7 blocks per CU instead of 5); the hop phase ends at a
-  // barrier before the results go in
+  // results (hr below) share one LDS area; the hop phase ends at a barrier
+  // before the results go in.  With the real matrices' real parts staged
+  // alone (kmr below) a 16-site block at Ls 12 takes 20 KB instead of 30 KB:
+  // 8 blocks per CU instead of 5
   constexpr bool ALIAS = H2 && SMAT;
   constexpr unsigned kLinkBytes = 2u * 72u * SB * 4u;
   constexpr unsigned kHrBytes = (unsigned)LSMAX * 6u * SB * 16u;
