@@ -1298,7 +1298,8 @@ struct ThreePhase {
       for (auto*& m : a.mats) m = nullptr;
       a.real_mat = true;
     }
-    qdwf::launch_hop_m5s(a, dag, half(), I.stream);
+    if (!qdwf::launch_hop_m5s(a, dag, half(), I.stream))
+      throw ArgError("the three-phase hop has no kernel for this Ls and block size");
     check_launch();
   }
   int64_t hop_blocks() const { return qdwf::hop_m5s_blocks(I.geo, sb); }
