@@ -610,8 +610,9 @@ bool launch_hop_m5s(const HopM5Args& a, bool dag, bool half, hipStream_t st)
   if (h2 && a.plain64 && !a.mats[0] && !a.dotp && a.ls <= 16) {
     // The plain hop (no s-matrix phase, no barrier after the hop) on blocks
     // of 64 sites: a wave then holds 64 sites of one row, so that a
-    // neighbour gather shifted by a few sites touches 9 cache lines per 16
-    // bytes of spinor instead of 12 (blocks of 16 sites x 4 rows).  At
+    // neighbour gather that is not line-aligned touches 9 cache lines per
+    // 16-byte unit of the wave's sites (1 KB) instead of 12 (blocks of 16
+    // sites x 4 rows: 4 spans of 256 bytes, 3 lines each).  At
     // 24^3x64 Ls12 over 4 columns: 77.7M L1-to-L2 requests against 98.8M,
     // 8.5% faster.  The s-matrix kernels stay on 16-site blocks: with 32 or
     // 64 sites their block-wide barriers leave fewer requests in flight,
